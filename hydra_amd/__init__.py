@@ -1,0 +1,257 @@
+"""hydra_amd -- MI355X-native contour-advection step (Hydra ca/plane/qg/caps).
+
+Host-side mirror of the reference's module interface over the C ABI of
+libhydra_amd.so (include/hydra_amd.h).  Method names and argument meaning
+follow the Fortran routines they replace:
+
+    ptospc / spctop / xderiv / yderiv      lib/stafft/sta2dfft.f90:103-245
+    main_invert / gradient                 ca/plane/qg/caps/spectral.f90:304-366
+    velint / con2grid                      ca/plane/qg/caps/contours.f90:136-346
+    set_field                              ca/plane/qg/caps/caps.f90:143-183
+    evolution_init / inversion / advance   ca/plane/qg/caps/evolution.f90
+
+Errors raise HydraError (the reference stops the program instead).  There is
+no CPU fallback: if the HIP library is missing or no GPU is present, the
+constructor raises.
+
+Array layouts are the reference's: a physical field (ny,nx) is a float64
+numpy array of shape (nx, ny) in C order; a spectral field (nx,ny) has
+shape (ny, nx).  Contour index arrays are 1-based int32.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhydra_amd.so")
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+FIELDS = {"qs": 0, "qd": 1, "qc": 2, "qq": 3, "pp": 4, "uu": 5, "vv": 6, "qr": 7, "epq": 8,
+          "emq": 9}
+SCALARS = {"t": 0, "dt": 1, "twist": 2, "zzrms": 3, "zzmax": 4, "uumax": 5, "qjump": 6,
+           "dtmax": 7}
+TABLES = {"green": 1, "flo": 2, "fhi": 3, "qdiss": 4, "rksq": 5, "hrkx": 6, "rkx": 7, "hrky": 8,
+          "rky": 9, "xtrig": 10, "ytrig": 11}
+
+
+class HydraError(RuntimeError):
+    pass
+
+
+class HydParams(C.Structure):
+    _fields_ = [
+        ("ng", C.c_int32), ("ncontq", C.c_int32),
+        ("ellx", C.c_double), ("elly", C.c_double), ("kd", C.c_double), ("beta", C.c_double),
+        ("rtherm", C.c_double), ("rekman", C.c_double), ("cdamp", C.c_double), ("nnu", C.c_double),
+        ("tgsave", C.c_double), ("tcsave", C.c_double), ("dt_fixed", C.c_double),
+        ("node_capacity", C.c_int64), ("precision", C.c_int32), ("device", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libhydra_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HydraError(f"{LIB_PATH} is missing: run `python -m hydra_amd.build` first "
+                             "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        L.hyd_last_error.restype = C.c_char_p
+        L.hyd_init.argtypes = [C.POINTER(HydParams), C.POINTER(C.c_void_p)]
+        L.hyd_destroy.argtypes = [C.c_void_p]
+        for fn in ("hyd_ptospc", "hyd_spctop", "hyd_xderiv", "hyd_yderiv"):
+            getattr(L, fn).argtypes = [C.c_void_p, _D, _D]
+        L.hyd_main_invert.argtypes = [C.c_void_p, _D, _D, _D, _D]
+        L.hyd_gradient.argtypes = [C.c_void_p, _D, _D, _D]
+        L.hyd_get_table.argtypes = [C.c_void_p, C.c_int32, _D]
+        L.hyd_velint.argtypes = [C.c_void_p, _D, _D, _D, _D, C.c_int32, _D, _D]
+        L.hyd_con2grid.argtypes = [C.c_void_p, _D, _D, _I, C.c_int32, C.c_double, _D, C.c_void_p]
+        L.hyd_sum_sequential.argtypes = [C.c_void_p, _D, C.c_int64, C.POINTER(C.c_double)]
+        L.hyd_set_field.argtypes = [C.c_void_p, _D, C.c_double]
+        L.hyd_set_contours.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _D, _D, _I, _I, _I, _I,
+                                       _I, C.c_double]
+        L.hyd_get_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.hyd_get_contours.argtypes = [C.c_void_p, _D, _D, _I, _I, _I, _I, _I]
+        L.hyd_get_field.argtypes = [C.c_void_p, C.c_int32, _D]
+        L.hyd_put_field.argtypes = [C.c_void_p, C.c_int32, _D]
+        L.hyd_get_scalar.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
+        L.hyd_set_scalar.argtypes = [C.c_void_p, C.c_int32, C.c_double]
+        for fn in ("hyd_evolution_init", "hyd_inversion", "hyd_advance", "hyd_sync"):
+            getattr(L, fn).argtypes = [C.c_void_p]
+        L.hyd_set_timing.argtypes = [C.c_void_p, C.c_int32]
+        L.hyd_kernel_times.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
+                                       np.ctypeslib.ndpointer(dtype=np.float64),
+                                       np.ctypeslib.ndpointer(dtype=np.int64)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HydraError(f"hydra_amd error {rc}: {lib().hyd_last_error().decode()}")
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
+class Hydra:
+    """One flow on one GPU: the reference's module state, device resident."""
+
+    def __init__(self, ng, ncontq=80, ellx=2 * np.pi, elly=None, kd=0.0, beta=0.0, rekman=0.0,
+                 cdamp=2.0, nnu=3.0, tgsave=1000.0, tcsave=1000.0, dt_fixed=0.0,
+                 node_capacity=0, device=0):
+        self.L = lib()
+        self.ng = ng
+        self.params = HydParams(ng=ng, ncontq=ncontq, ellx=ellx, elly=ellx if elly is None else elly,
+                                kd=kd, beta=beta, rtherm=0.0, rekman=rekman, cdamp=cdamp, nnu=nnu,
+                                tgsave=tgsave, tcsave=tcsave, dt_fixed=dt_fixed,
+                                node_capacity=node_capacity, precision=0, device=device)
+        h = C.c_void_p()
+        _check(self.L.hyd_init(C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hyd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- sta2dfft / spectral ---------------------------------------------------
+    def _grid_op(self, fn, a, nout=1):
+        a = _f64(a, (self.ng, self.ng))
+        outs = [np.empty_like(a) for _ in range(nout)]
+        _check(fn(self.h, a, *outs))
+        return outs[0] if nout == 1 else tuple(outs)
+
+    def ptospc(self, rvar):
+        return self._grid_op(self.L.hyd_ptospc, rvar)
+
+    def spctop(self, svar):
+        return self._grid_op(self.L.hyd_spctop, svar)
+
+    def xderiv(self, var):
+        return self._grid_op(self.L.hyd_xderiv, var)
+
+    def yderiv(self, var):
+        return self._grid_op(self.L.hyd_yderiv, var)
+
+    def main_invert(self, qq):
+        """returns (uu, vv, pp)"""
+        return self._grid_op(self.L.hyd_main_invert, qq, 3)
+
+    def gradient(self, ff):
+        return self._grid_op(self.L.hyd_gradient, ff, 2)
+
+    def table(self, name):
+        n = self.ng
+        size = {1: n * n, 2: n * n, 3: n * n, 4: n * n, 5: n * n, 10: 2 * n, 11: 2 * n}.get(
+            TABLES[name], n)
+        out = np.empty(size)
+        _check(self.L.hyd_get_table(self.h, TABLES[name], out))
+        return out
+
+    # -- contours ----------------------------------------------------------------
+    def velint(self, uu, vv, xq, yq):
+        xq, yq = _f64(xq), _f64(yq)
+        n = len(xq)
+        uq, vq = np.empty(n), np.empty(n)
+        _check(self.L.hyd_velint(self.h, _f64(uu, (self.ng, self.ng)), _f64(vv, (self.ng, self.ng)),
+                                 xq, yq, n, uq, vq))
+        return uq, vq
+
+    def con2grid(self, xq, yq, nextq, qjump, with_counts=False):
+        xq, yq = _f64(xq), _f64(yq)
+        nextq = np.ascontiguousarray(nextq, dtype=np.int32)
+        qc = np.empty((self.ng, self.ng))
+        cnt = np.empty((4 * self.ng, 4 * self.ng), dtype=np.int32) if with_counts else None
+        _check(self.L.hyd_con2grid(self.h, xq, yq, nextq, len(xq), float(qjump), qc,
+                                   cnt.ctypes.data_as(C.c_void_p) if with_counts else None))
+        return (qc, cnt) if with_counts else qc
+
+    def sum_sequential(self, a):
+        a = _f64(a).ravel()
+        out = C.c_double()
+        _check(self.L.hyd_sum_sequential(self.h, a, len(a), C.byref(out)))
+        return out.value
+
+    # -- device-resident state ----------------------------------------------------
+    def set_field(self, qq, t0=0.0):
+        _check(self.L.hyd_set_field(self.h, _f64(qq, (self.ng, self.ng)), float(t0)))
+
+    def set_contours(self, c):
+        n, nc = len(c["x"]), len(c["i1q"])
+        ii = {k: np.ascontiguousarray(c[k], dtype=np.int32) for k in ("nextq", "i1q", "i2q", "npq", "indq")}
+        _check(self.L.hyd_set_contours(self.h, n, nc, _f64(c["x"]), _f64(c["y"]), ii["nextq"],
+                                       ii["i1q"], ii["i2q"], ii["npq"], ii["indq"], float(c["qjump"])))
+
+    def counts(self):
+        a, b = C.c_int32(), C.c_int32()
+        _check(self.L.hyd_get_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def contours(self):
+        n, nc = self.counts()
+        x, y = np.empty(max(n, 1)), np.empty(max(n, 1))
+        nxt = np.empty(max(n, 1), dtype=np.int32)
+        arrs = [np.empty(max(nc, 1), dtype=np.int32) for _ in range(4)]
+        _check(self.L.hyd_get_contours(self.h, x, y, nxt, *arrs))
+        return dict(x=x[:n], y=y[:n], nextq=nxt[:n], i1q=arrs[0][:nc], i2q=arrs[1][:nc],
+                    npq=arrs[2][:nc], indq=arrs[3][:nc], qjump=self.scalar("qjump"))
+
+    def field(self, name):
+        out = np.empty((self.ng, self.ng))
+        _check(self.L.hyd_get_field(self.h, FIELDS[name], out))
+        return out
+
+    def put_field(self, name, a):
+        _check(self.L.hyd_put_field(self.h, FIELDS[name], _f64(a, (self.ng, self.ng))))
+
+    def scalar(self, name):
+        out = C.c_double()
+        _check(self.L.hyd_get_scalar(self.h, SCALARS[name], C.byref(out)))
+        return out.value
+
+    def set_scalar(self, name, v):
+        _check(self.L.hyd_set_scalar(self.h, SCALARS[name], float(v)))
+
+    def evolution_init(self):
+        _check(self.L.hyd_evolution_init(self.h))
+
+    def inversion(self):
+        _check(self.L.hyd_inversion(self.h))
+
+    def advance(self):
+        _check(self.L.hyd_advance(self.h))
+
+    def sync(self):
+        _check(self.L.hyd_sync(self.h))
+
+    # -- timing -----------------------------------------------------------------
+    def set_timing(self, on=True):
+        _check(self.L.hyd_set_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self, max_records=1 << 16):
+        names = C.create_string_buffer(32 * max_records)
+        ms = np.zeros(max_records)
+        by = np.zeros(max_records, dtype=np.int64)
+        n = self.L.hyd_kernel_times(self.h, max_records, names, ms, by)
+        raw = names.raw
+        return [(raw[32 * i:32 * i + 32].split(b"\0")[0].decode(), ms[i], int(by[i]))
+                for i in range(n)]

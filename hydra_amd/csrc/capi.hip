@@ -1,0 +1,607 @@
+// capi.hip -- the extern "C" boundary (include/hydra_amd.h): context life
+// cycle, host-array entry points that mirror the reference routines one to
+// one, and the device-resident step.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "hyd_internal.h"
+
+namespace hyd {
+
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+
+int inversion(Ctx &c);
+int adapt(Ctx &c);
+int advance(Ctx &c);
+int evolution_init(Ctx &c);
+
+KTimer::KTimer(Ctx &ctx, const char *name, int64_t bytes) : c(ctx) {
+  if (!c.timing) return;
+  Ctx::KRec r;
+  r.name = name;
+  r.bytes = bytes;
+  if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
+  hipEventRecord(r.a, c.s);
+  c.krec.push_back(r);
+  idx = (int)c.krec.size() - 1;
+}
+KTimer::~KTimer() {
+  if (idx >= 0) hipEventRecord(c.krec[idx].b, c.s);
+}
+
+__global__ void k_xderiv(const double *__restrict__ var, double *__restrict__ der,
+                         const double *__restrict__ dk, int nx, int ny) {
+  const int kx = blockIdx.x * blockDim.x + threadIdx.x;  // 0-based
+  const int ky = blockIdx.y;
+  if (kx >= nx) return;
+  const int h = nx / 2;
+  double v;
+  if (kx == 0 || kx == h) v = 0.0;
+  else if (kx < h) v = -dk[2 * kx - 1] * var[(int64_t)ky * nx + (nx - kx)];
+  else v = dk[2 * (nx - kx) - 1] * var[(int64_t)ky * nx + (nx - kx)];
+  der[(int64_t)ky * nx + kx] = v;
+}
+
+__global__ void k_yderiv(const double *__restrict__ var, double *__restrict__ der,
+                         const double *__restrict__ dk, int nx, int ny) {
+  const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ky = blockIdx.y;
+  if (kx >= nx) return;
+  const int h = ny / 2;
+  double v;
+  if (ky == 0 || ky == h) v = 0.0;
+  else if (ky < h) v = -dk[2 * ky - 1] * var[(int64_t)(ny - ky) * nx + kx];
+  else v = dk[2 * (ny - ky) - 1] * var[(int64_t)(ny - ky) * nx + kx];
+  der[(int64_t)ky * nx + kx] = v;
+}
+
+int launch_xderiv(Ctx &c, const double *var, double *der) {
+  hipLaunchKernelGGL(k_xderiv, dim3((c.c.nx + 255) / 256, c.c.ny), dim3(256), 0, c.s, var, der,
+                     c.hrkx, c.c.nx, c.c.ny);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+int launch_yderiv(Ctx &c, const double *var, double *der) {
+  hipLaunchKernelGGL(k_yderiv, dim3((c.c.nx + 255) / 256, c.c.ny), dim3(256), 0, c.s, var, der,
+                     c.hrky, c.c.nx, c.c.ny);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+template <class T>
+static int dalloc(T **p, int64_t n) {
+  HYD_HIP(hipMalloc((void **)p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
+  HYD_HIP(hipMemset(*p, 0, sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
+  return HYD_OK;
+}
+
+template <class T>
+static int upload(T *d, const T *h, int64_t n, hipStream_t s) {
+  HYD_HIP(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+  return HYD_OK;
+}
+
+static void free_ctx(Ctx *c) {
+  if (!c) return;
+  void *ptrs[] = {c->green, c->flo, c->fhi, c->hrkx, c->hrky, c->xtrig, c->ytrig, c->xgf,
+                  c->qdiss_map, c->qdiss_exp, c->qs, c->qd, c->qc, c->qq, c->pp, c->uu, c->vv,
+                  c->qr, c->emq, c->epq, c->qsi, c->qsf, c->sqs, c->qdi, c->qdf, c->sqd, c->wa,
+                  c->wb, c->wc, c->wd, c->cnt, c->fq, c->mq, c->qjx_cnt, c->qbot, c->ixc,
+                  c->nxc, c->ndx, c->ndy, c->flags, c->replay, c->replay_val, c->xq, c->yq,
+                  c->xqi, c->yqi, c->xqf, c->yqf, c->uq, c->vq, c->nextq, c->nxt_tmp, c->dsc, c->dsum,
+                  c->seq_work};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (c->hsc) hipHostFree(c->hsc);
+  for (auto &r : c->krec) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  if (c->s) hipStreamDestroy(c->s);
+  delete c;
+}
+
+static int create(const hyd_params *p, hyd_ctx **out) {
+  if (!p || !out) {
+    set_error("null argument");
+    return HYD_EARG;
+  }
+  Ctx *c = new Ctx();
+  c->p = *p;
+  int rc = build_consts(*p, c->c);
+  if (rc == HYD_OK) rc = build_tables(*p, c->c, c->ht);
+  if (rc != HYD_OK) {
+    delete c;
+    return rc;
+  }
+  const int n = p->ng;
+  int log2n = 0;
+  while ((1 << log2n) < n) log2n++;
+  c->plan.n = n;
+  c->plan.n2 = log2n & 1;
+  c->plan.n4 = log2n / 2;
+  c->plan.fnorm = 1.0 / sqrt((double)n);
+  c->plan.rnorm = 2.0 / sqrt((double)n);
+  c->N = (int64_t)n * n;
+  c->cap = p->node_capacity > 0 ? p->node_capacity : 8 * c->N + 65536;
+  c->dev = p->device;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    set_error("no HIP device available: the hydra_amd library runs on the GPU only");
+    delete c;
+    return HYD_EHIP;
+  }
+  if (hipSetDevice(c->dev) != hipSuccess) {
+    set_error("hipSetDevice failed");
+    delete c;
+    return HYD_EHIP;
+  }
+#define TRY(x)             \
+  do {                     \
+    int r_ = (x);          \
+    if (r_ != HYD_OK) {    \
+      free_ctx(c);         \
+      return r_;           \
+    }                      \
+  } while (0)
+  if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    delete c;
+    return HYD_EHIP;
+  }
+  const int64_t N = c->N;
+  double **fields[] = {&c->qs,  &c->qd,  &c->qc,  &c->qq,  &c->pp,  &c->uu, &c->vv,
+                       &c->qr,  &c->emq, &c->epq, &c->qsi, &c->qsf, &c->sqs, &c->qdi,
+                       &c->qdf, &c->sqd, &c->wa,  &c->wb,  &c->wc,  &c->wd};
+  for (double **f : fields) TRY(dalloc(f, N));
+  TRY(dalloc(&c->green, N));
+  TRY(dalloc(&c->flo, N));
+  TRY(dalloc(&c->fhi, N));
+  TRY(dalloc(&c->hrkx, n));
+  TRY(dalloc(&c->hrky, n));
+  TRY(dalloc(&c->xtrig, 2 * n));
+  TRY(dalloc(&c->ytrig, 2 * n));
+  TRY(dalloc(&c->xgf, c->c.nxf));
+  TRY(dalloc(&c->qdiss_map, N));
+  c->nqu = (int)c->ht.qdiss_u.size();
+  TRY(dalloc(&c->qdiss_exp, c->nqu));
+  const int64_t nfine = (int64_t)c->c.nxf * c->c.nyf;
+  TRY(dalloc(&c->cnt, nfine));
+  TRY(dalloc(&c->fq, nfine));
+  TRY(dalloc(&c->mq, nfine / 4));
+  TRY(dalloc(&c->qjx_cnt, c->c.nxf));
+  TRY(dalloc(&c->qbot, c->c.nxf));
+  TRY(dalloc(&c->flags, 16));
+  c->replay_cap = 1 << 16;
+  TRY(dalloc(&c->replay, 3 * c->replay_cap));
+  TRY(dalloc(&c->replay_val, c->replay_cap));
+  const int64_t P = c->cap;
+  TRY(dalloc(&c->ixc, P));
+  TRY(dalloc(&c->nxc, P));
+  TRY(dalloc(&c->ndx, P));
+  TRY(dalloc(&c->ndy, P));
+  double **nodes[] = {&c->xq, &c->yq, &c->xqi, &c->yqi, &c->xqf, &c->yqf, &c->uq, &c->vq};
+  for (double **f : nodes) TRY(dalloc(f, P));
+  TRY(dalloc(&c->nextq, P));
+  TRY(dalloc(&c->nxt_tmp, P));
+  TRY(dalloc(&c->dsc, 1));
+  TRY(dalloc(&c->dsum, 8));
+  c->seq_cap = (std::max<int64_t>(N, 4 * N) + 255) / 256 + 64;
+  {
+    void *w = nullptr;
+    if (hipMalloc(&w, seqsum_work_bytes(4 * N)) != hipSuccess) {
+      set_error("hipMalloc seqsum work");
+      free_ctx(c);
+      return HYD_ENOMEM;
+    }
+    c->seq_work = w;
+    c->seq_cap = seqsum_work_bytes(4 * N) / 56;
+  }
+  if (hipHostMalloc((void **)&c->hsc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
+    set_error("hipHostMalloc failed");
+    free_ctx(c);
+    return HYD_ENOMEM;
+  }
+  memset(c->hsc, 0, sizeof(Scalars));
+  TRY(upload(c->green, c->ht.green.data(), N, c->s));
+  TRY(upload(c->flo, c->ht.flo.data(), N, c->s));
+  TRY(upload(c->fhi, c->ht.fhi.data(), N, c->s));
+  TRY(upload(c->hrkx, c->ht.hrkx.data(), n, c->s));
+  TRY(upload(c->hrky, c->ht.hrky.data(), n, c->s));
+  TRY(upload(c->xtrig, c->ht.xtrig.data(), 2 * n, c->s));
+  TRY(upload(c->ytrig, c->ht.ytrig.data(), 2 * n, c->s));
+  TRY(upload(c->qdiss_map, c->ht.qdiss_map.data(), N, c->s));
+  {
+    std::vector<double> xgf(c->c.nxf);
+    for (int ix = 1; ix <= c->c.nxf; ix++) xgf[ix - 1] = c->c.xmin + c->c.glxf * (double)(ix - 1);
+    TRY(upload(c->xgf, xgf.data(), c->c.nxf, c->s));
+    if (hipStreamSynchronize(c->s) != hipSuccess) {
+      set_error("stream sync failed");
+      free_ctx(c);
+      return HYD_EHIP;
+    }
+  }
+#undef TRY
+  *out = (hyd_ctx *)c;
+  return HYD_OK;
+}
+
+static Ctx *C(hyd_ctx *h) { return (Ctx *)h; }
+
+// Staging helpers for the host-array entry points.
+static int h2d(Ctx &c, double *d, const double *h, int64_t n) {
+  HYD_HIP(hipMemcpyAsync(d, h, sizeof(double) * n, hipMemcpyHostToDevice, c.s));
+  return HYD_OK;
+}
+static int d2h(Ctx &c, double *h, const double *d, int64_t n) {
+  HYD_HIP(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+}  // namespace hyd
+
+using namespace hyd;
+
+extern "C" {
+
+const char *hyd_last_error(void) { return g_err.c_str(); }
+
+int hyd_init(const hyd_params *p, hyd_ctx **ctx) { return create(p, ctx); }
+
+int hyd_destroy(hyd_ctx *ctx) {
+  free_ctx(C(ctx));
+  return HYD_OK;
+}
+
+int hyd_ptospc(hyd_ctx *h, const double *rvar, double *svar) {
+  Ctx &c = *C(h);
+  HYD_CHECK(h2d(c, c.wa, rvar, c.N));
+  HYD_CHECK(fft_ptospc(c, c.wa, c.wb));
+  return d2h(c, svar, c.wb, c.N);
+}
+
+int hyd_spctop(hyd_ctx *h, const double *svar, double *rvar) {
+  Ctx &c = *C(h);
+  HYD_CHECK(h2d(c, c.wa, svar, c.N));
+  HYD_CHECK(fft_spctop(c, c.wa, c.wb, PRE_NONE, 1.0, c.wc));
+  return d2h(c, rvar, c.wb, c.N);
+}
+
+int hyd_xderiv(hyd_ctx *h, const double *var, double *der) {
+  Ctx &c = *C(h);
+  HYD_CHECK(h2d(c, c.wa, var, c.N));
+  HYD_CHECK(launch_xderiv(c, c.wa, c.wb));
+  return d2h(c, der, c.wb, c.N);
+}
+
+int hyd_yderiv(hyd_ctx *h, const double *var, double *der) {
+  Ctx &c = *C(h);
+  HYD_CHECK(h2d(c, c.wa, var, c.N));
+  HYD_CHECK(launch_yderiv(c, c.wa, c.wb));
+  return d2h(c, der, c.wb, c.N);
+}
+
+int hyd_main_invert(hyd_ctx *h, const double *qq, double *uu, double *vv, double *pp) {
+  Ctx &c = *C(h);
+  // wa..wd are scratch for the host-array entry points
+  HYD_CHECK(h2d(c, c.wa, qq, c.N));
+  HYD_CHECK(main_invert_dev(c, c.wa, c.wb, c.wc, c.wd, c.mq));
+  HYD_CHECK(d2h(c, uu, c.wb, c.N));
+  HYD_CHECK(d2h(c, vv, c.wc, c.N));
+  return d2h(c, pp, c.wd, c.N);
+}
+
+int hyd_gradient(hyd_ctx *h, const double *ff, double *ffx, double *ffy) {
+  Ctx &c = *C(h);
+  HYD_CHECK(h2d(c, c.wa, ff, c.N));
+  HYD_CHECK(gradient_dev(c, c.wa, c.wb, c.wc, c.mq));
+  HYD_CHECK(d2h(c, ffx, c.wb, c.N));
+  return d2h(c, ffy, c.wc, c.N);
+}
+
+int hyd_get_table(hyd_ctx *h, int32_t id, double *out) {
+  Ctx &c = *C(h);
+  const HostTables &t = c.ht;
+  const std::vector<double> *v = nullptr;
+  switch (id) {
+    case 1: v = &t.green; break;
+    case 2: v = &t.flo; break;
+    case 3: v = &t.fhi; break;
+    case 4: v = &t.qdiss; break;
+    case 5: v = &t.rksq; break;
+    case 6: v = &t.hrkx; break;
+    case 7: v = &t.rkx; break;
+    case 8: v = &t.hrky; break;
+    case 9: v = &t.rky; break;
+    case 10: v = &t.xtrig; break;
+    case 11: v = &t.ytrig; break;
+    default: set_error("unknown table id"); return HYD_EARG;
+  }
+  memcpy(out, v->data(), sizeof(double) * v->size());
+  return HYD_OK;
+}
+
+int hyd_velint(hyd_ctx *h, const double *uu, const double *vv, const double *xq, const double *yq,
+               int32_t nptq, double *uq, double *vq) {
+  Ctx &c = *C(h);
+  if (nptq < 0 || nptq > c.cap) {
+    set_error("velint: node count outside [0, capacity]");
+    return HYD_ECAPACITY;
+  }
+  if (nptq == 0) return HYD_OK;
+  HYD_CHECK(h2d(c, c.wa, uu, c.N));
+  HYD_CHECK(h2d(c, c.wb, vv, c.N));
+  HYD_CHECK(h2d(c, c.xqi, xq, nptq));
+  HYD_CHECK(h2d(c, c.yqi, yq, nptq));
+  HYD_CHECK(velint(c, c.wa, c.wb, c.xqi, c.yqi, nptq, c.uq, c.vq));
+  HYD_CHECK(d2h(c, uq, c.uq, nptq));
+  return d2h(c, vq, c.vq, nptq);
+}
+
+static int check_links(const int32_t *nextq, int32_t n) {
+  for (int32_t i = 0; i < n; i++)
+    if (nextq[i] < 1 || nextq[i] > n) {
+      set_error("nextq holds a node number outside 1..nptq");
+      return HYD_EARG;
+    }
+  return HYD_OK;
+}
+
+int hyd_con2grid(hyd_ctx *h, const double *xq, const double *yq, const int32_t *nextq,
+                 int32_t nptq, double qjump, double *qc, int32_t *counts) {
+  Ctx &c = *C(h);
+  if (nptq < 0 || nptq > c.cap) {
+    set_error("con2grid: node count outside [0, capacity]");
+    return HYD_ECAPACITY;
+  }
+  HYD_CHECK(check_links(nextq, nptq));
+  if (nptq > 0) {
+    HYD_CHECK(h2d(c, c.xqi, xq, nptq));
+    HYD_CHECK(h2d(c, c.yqi, yq, nptq));
+    HYD_HIP(hipMemcpyAsync(c.nxt_tmp, nextq, sizeof(int32_t) * nptq, hipMemcpyHostToDevice, c.s));
+  }
+  const double saved = c.hsc->qjump;
+  c.hsc->qjump = qjump;
+  int rc = con2grid(c, c.xqi, c.yqi, c.nxt_tmp, nptq, c.wa, counts);
+  c.hsc->qjump = saved;
+  HYD_CHECK(rc);
+  HYD_CHECK(d2h(c, qc, c.wa, c.N));
+  int32_t fl[8];
+  HYD_HIP(hipMemcpy(fl, c.flags, sizeof(fl), hipMemcpyDeviceToHost));
+  if (fl[3]) {
+    set_error("con2grid: too many order-dependent cells for the replay list");
+    return HYD_ECAPACITY;
+  }
+  return HYD_OK;
+}
+
+int hyd_sum_sequential(hyd_ctx *h, const double *a, int64_t n, double *sum) {
+  Ctx &c = *C(h);
+  if (n < 0 || n > 4 * c.N) {
+    set_error("sum_sequential: length outside [0, 4*ng*ng]");
+    return HYD_EARG;
+  }
+  if (n == 0) {
+    *sum = 0.0;
+    return HYD_OK;
+  }
+  double *d = c.fq;  // 4N doubles of scratch
+  HYD_CHECK(h2d(c, d, a, n));
+  HYD_CHECK(seqsum(c, d, n, c.dsum));
+  return d2h(c, sum, c.dsum, 1);
+}
+
+int hyd_set_field(hyd_ctx *h, const double *qq, double t0) {
+  Ctx &c = *C(h);
+  if (c.c.beffect) {
+    set_error("beta != 0 is not supported by the device step");
+    return HYD_EARG;
+  }
+  // contint, generic.f90:216-229
+  double mx = qq[0], mn = qq[0];
+  for (int64_t k = 0; k < c.N; k++) {
+    mx = qq[k] > mx ? qq[k] : mx;
+    mn = qq[k] < mn ? qq[k] : mn;
+  }
+  memset(c.hsc, 0, sizeof(Scalars));
+  c.hsc->qjump = (mx - mn) / (double)c.p.ncontq;
+  c.hsc->t = t0;
+  c.hsc->dt = 0.0;
+  HYD_CHECK(h2d(c, c.qr, qq, c.N));
+  HYD_CHECK(fft_ptospc(c, c.qr, c.qs));
+  HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  c.nptq = 0;
+  c.nq = 0;
+  c.have_field = true;
+  return HYD_OK;
+}
+
+int hyd_set_contours(hyd_ctx *h, int32_t nptq, int32_t nq, const double *xq, const double *yq,
+                     const int32_t *nextq, const int32_t *i1q, const int32_t *i2q,
+                     const int32_t *npq, const int32_t *indq, double qjump) {
+  Ctx &c = *C(h);
+  if (nptq < 0 || nptq > c.cap || nq < 0) {
+    set_error("set_contours: counts outside capacity");
+    return HYD_ECAPACITY;
+  }
+  HYD_CHECK(check_links(nextq, nptq));
+  if (nptq > 0) {
+    HYD_CHECK(h2d(c, c.xq, xq, nptq));
+    HYD_CHECK(h2d(c, c.yq, yq, nptq));
+    HYD_HIP(hipMemcpyAsync(c.nextq, nextq, sizeof(int32_t) * nptq, hipMemcpyHostToDevice, c.s));
+  }
+  c.i1q.assign(i1q, i1q + nq);
+  c.i2q.assign(i2q, i2q + nq);
+  c.npq.assign(npq, npq + nq);
+  c.indq.assign(indq, indq + nq);
+  c.nptq = nptq;
+  c.nq = nq;
+  HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  c.hsc->qjump = qjump;
+  HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+int hyd_get_counts(hyd_ctx *h, int32_t *nptq, int32_t *nq) {
+  Ctx &c = *C(h);
+  *nptq = c.nptq;
+  *nq = c.nq;
+  return HYD_OK;
+}
+
+int hyd_get_contours(hyd_ctx *h, double *xq, double *yq, int32_t *nextq, int32_t *i1q,
+                     int32_t *i2q, int32_t *npq, int32_t *indq) {
+  Ctx &c = *C(h);
+  if (c.nptq > 0) {
+    HYD_HIP(hipMemcpyAsync(xq, c.xq, sizeof(double) * c.nptq, hipMemcpyDeviceToHost, c.s));
+    HYD_HIP(hipMemcpyAsync(yq, c.yq, sizeof(double) * c.nptq, hipMemcpyDeviceToHost, c.s));
+    HYD_HIP(hipMemcpyAsync(nextq, c.nextq, sizeof(int32_t) * c.nptq, hipMemcpyDeviceToHost, c.s));
+    HYD_HIP(hipStreamSynchronize(c.s));
+  }
+  std::copy(c.i1q.begin(), c.i1q.end(), i1q);
+  std::copy(c.i2q.begin(), c.i2q.end(), i2q);
+  std::copy(c.npq.begin(), c.npq.end(), npq);
+  std::copy(c.indq.begin(), c.indq.end(), indq);
+  return HYD_OK;
+}
+
+static double *field_ptr(Ctx &c, int id) {
+  double *f[] = {c.qs, c.qd, c.qc, c.qq, c.pp, c.uu, c.vv, c.qr, c.epq, c.emq};
+  return (id >= 0 && id < 10) ? f[id] : nullptr;
+}
+
+int hyd_get_field(hyd_ctx *h, int32_t id, double *out) {
+  Ctx &c = *C(h);
+  double *f = field_ptr(c, id);
+  if (!f) {
+    set_error("unknown field id");
+    return HYD_EARG;
+  }
+  return d2h(c, out, f, c.N);
+}
+
+int hyd_put_field(hyd_ctx *h, int32_t id, const double *in) {
+  Ctx &c = *C(h);
+  double *f = field_ptr(c, id);
+  if (!f) {
+    set_error("unknown field id");
+    return HYD_EARG;
+  }
+  HYD_CHECK(h2d(c, f, in, c.N));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+static double *scalar_slot(Scalars &s, int id) {
+  double *v[] = {&s.t, &s.dt, &s.twist, &s.zzrms, &s.zzmax, &s.uumax, &s.qjump, &s.dtmax};
+  return (id >= 0 && id < 8) ? v[id] : nullptr;
+}
+
+int hyd_get_scalar(hyd_ctx *h, int32_t id, double *out) {
+  Ctx &c = *C(h);
+  HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  double *v = scalar_slot(*c.hsc, id);
+  if (!v) {
+    set_error("unknown scalar id");
+    return HYD_EARG;
+  }
+  *out = *v;
+  return HYD_OK;
+}
+
+int hyd_set_scalar(hyd_ctx *h, int32_t id, double val) {
+  Ctx &c = *C(h);
+  HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  double *v = scalar_slot(*c.hsc, id);
+  if (!v) {
+    set_error("unknown scalar id");
+    return HYD_EARG;
+  }
+  *v = val;
+  HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+int hyd_evolution_init(hyd_ctx *h) {
+  Ctx &c = *C(h);
+  if (!c.have_field) {
+    set_error("evolution_init before set_field");
+    return HYD_ESTATE;
+  }
+  HYD_CHECK(evolution_init(c));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+int hyd_inversion(hyd_ctx *h) {
+  Ctx &c = *C(h);
+  HYD_CHECK(inversion(c));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+int hyd_advance(hyd_ctx *h) {
+  Ctx &c = *C(h);
+  if (!c.have_field) {
+    set_error("advance before set_field");
+    return HYD_ESTATE;
+  }
+  HYD_CHECK(advance(c));
+  int32_t fl[8];
+  HYD_HIP(hipMemcpyAsync(fl, c.flags, sizeof(fl), hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  if (fl[3]) {
+    set_error("con2grid: too many order-dependent cells for the replay list");
+    return HYD_ECAPACITY;
+  }
+  return HYD_OK;
+}
+
+int hyd_sync(hyd_ctx *h) {
+  HYD_HIP(hipStreamSynchronize(C(h)->s));
+  return HYD_OK;
+}
+
+int hyd_set_timing(hyd_ctx *h, int32_t on) {
+  Ctx &c = *C(h);
+  HYD_HIP(hipStreamSynchronize(c.s));
+  for (auto &r : c.krec) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  c.krec.clear();
+  c.timing = on != 0;
+  return HYD_OK;
+}
+
+// names: max*32 bytes (NUL padded); returns the number of records.
+int hyd_kernel_times(hyd_ctx *h, int32_t max, char *names, double *ms, int64_t *bytes) {
+  Ctx &c = *C(h);
+  HYD_HIP(hipStreamSynchronize(c.s));
+  int n = 0;
+  for (auto &r : c.krec) {
+    if (n >= max) break;
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, r.a, r.b) != hipSuccess) t = -1.0f;
+    strncpy(names + 32 * n, r.name.c_str(), 31);
+    names[32 * n + 31] = 0;
+    ms[n] = t;
+    bytes[n] = r.bytes;
+    n++;
+  }
+  return n;
+}
+
+}  // extern "C"

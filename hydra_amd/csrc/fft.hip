@@ -1,0 +1,429 @@
+// fft.hip -- stafft-faithful real FFTs on gfx950.
+//
+// Each launch transforms a batch of m vectors of length n (a power of two)
+// entirely in LDS: load once (optionally transposed and with a fused
+// spectral derivative / revfft pre-scaling), run every radix stage between
+// two LDS buffers, store once (optionally transposed, with the 1/sqrt(n) or
+// 2/sqrt(n) normalisation and a sign folded in).
+//
+// The butterflies are those of forrdx2/forrdx4 (stafft.f90:890-997,
+// 1080-1130) and revrdx4/revrdx2 (stafft.f90:1463-1571, 1656-1706), applied
+// in the factor order of forfft/revfft (stafft.f90:212-271, 331-390), with
+// the host-built trig table, evaluated with the same expression order and
+// no FMA contraction.  The result is therefore bit-identical to the
+// reference, while the data movement is one HBM read and one HBM write per
+// 1D transform batch.
+//
+// Layouts: "strided" means element e of vector v is x[v + m*e] (the
+// reference's x(m,n) with the vector index fastest); "rows" means
+// x[e + n*v].  ptospc = x-transform reading strided (ny vectors) and writing
+// rows (this is the transpose of sta2dfft.f90:122-126), then an in-place
+// strided y-transform.  spctop = strided y-inverse into a work array, then
+// an x-inverse reading rows and writing strided (sta2dfft.f90:152-162).
+#include "hyd_internal.h"
+
+namespace hyd {
+
+struct FftArgs {
+  const double *in;
+  double *out;
+  const double *trig;   // 2n: cos then -sin (stafft.f90:103-122)
+  const double *dk;     // filtered wavenumbers for a fused derivative
+  const double *mulk;   // optional per-element multiplier applied before the
+                        // derivative (unused: kept null)
+  int n, m, V;
+  int in_rows, out_rows;
+  int pre;              // 0 none, 1 derivative across vectors, 2 along elements
+  int n2, n4;
+  double post;          // normalisation (and sign) folded into the store
+};
+
+// ---------------------------------------------------------------------------
+// forward stages (decimation in time)
+
+__device__ __forceinline__ void fwd_r2(const double *__restrict__ a, double *__restrict__ b,
+                                       int n, int V, int rem, int lv,
+                                       const double *__restrict__ cs,
+                                       const double *__restrict__ sn) {
+  const int nh = (lv - 1) / 2;
+  const int nk = 1 + nh;
+  const int per = rem * nk;
+  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+    const int v = it / per, r = it - v * per;
+    const int kk = r / rem, irem = r - kk * rem;
+    const double *av = a + v * n;
+    double *bv = b + v * n;
+#define A(j, k) av[irem + rem * ((j) + 2 * (k))]
+#define B(k, j) bv[irem + rem * ((k) + lv * (j))]
+    if (kk == 0) {
+      const double a00 = A(0, 0), a10 = A(1, 0);
+      B(0, 0) = a00 + a10;
+      B(0, 1) = a00 - a10;
+    } else {
+      const int k = kk, kc = lv - k;
+      const double c1k = cs[k], s1k = sn[k];
+      const double a1k = A(1, k), a1kc = A(1, kc), a0k = A(0, k), a0kc = A(0, kc);
+      const double x1 = c1k * a1k - s1k * a1kc;
+      const double y1 = c1k * a1kc + s1k * a1k;
+      B(k, 0) = a0k + x1;
+      B(kc, 0) = a0k - x1;
+      B(k, 1) = y1 - a0kc;
+      B(kc, 1) = a0kc + y1;
+    }
+#undef A
+#undef B
+  }
+}
+
+__device__ __forceinline__ void fwd_r4(const double *__restrict__ a, double *__restrict__ b,
+                                       int n, int V, int rem, int lv,
+                                       const double *__restrict__ cs,
+                                       const double *__restrict__ sn) {
+  const double rtf12 = 0.7071067811865475244008443621048490392848359376884740365883398689953662;
+  const int nh = (lv - 1) / 2;
+  const int even = (lv & 1) == 0;
+  const int nk = 1 + nh + even;
+  const int per = rem * nk;
+  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+    const int v = it / per, r = it - v * per;
+    const int kk = r / rem, irem = r - kk * rem;
+    const double *av = a + v * n;
+    double *bv = b + v * n;
+#define A(j, k) av[irem + rem * ((j) + 4 * (k))]
+#define B(k, j) bv[irem + rem * ((k) + lv * (j))]
+    if (kk == 0) {
+      const double a0 = A(0, 0), a1 = A(1, 0), a2 = A(2, 0), a3 = A(3, 0);
+      const double t1r = a0 + a2;
+      const double t2r = a1 + a3;
+      B(0, 0) = t1r + t2r;
+      B(0, 1) = a0 - a2;
+      B(0, 2) = t1r - t2r;
+      B(0, 3) = a3 - a1;
+    } else if (kk <= nh) {
+      const int k = kk, kc = lv - k;
+      const double c1k = cs[k], s1k = sn[k];
+      const double c2k = cs[k + lv], s2k = sn[k + lv];
+      const double c3k = cs[k + 2 * lv], s3k = sn[k + 2 * lv];
+      const double a0k = A(0, k), a0kc = A(0, kc), a1k = A(1, k), a1kc = A(1, kc);
+      const double a2k = A(2, k), a2kc = A(2, kc), a3k = A(3, k), a3kc = A(3, kc);
+      const double x1p = c1k * a1k - s1k * a1kc;
+      const double y1p = c1k * a1kc + s1k * a1k;
+      const double x2p = c2k * a2k - s2k * a2kc;
+      const double y2p = c2k * a2kc + s2k * a2k;
+      const double x3p = c3k * a3k - s3k * a3kc;
+      const double y3p = c3k * a3kc + s3k * a3k;
+      const double t1r = a0k + x2p;
+      const double t1i = a0kc + y2p;
+      const double t2r = x1p + x3p;
+      const double t2i = y1p + y3p;
+      const double t3r = a0k - x2p;
+      const double t3i = a0kc - y2p;
+      const double t4r = x3p - x1p;
+      const double t4i = y1p - y3p;
+      B(k, 0) = t1r + t2r;
+      B(kc, 0) = t3r - t4i;
+      B(k, 1) = t3r + t4i;
+      B(kc, 1) = t1r - t2r;
+      B(k, 2) = t2i - t1i;
+      B(kc, 2) = t3i + t4r;
+      B(k, 3) = t4r - t3i;
+      B(kc, 3) = t1i + t2i;
+    } else {
+      const int l2 = lv / 2;
+      const double a0 = A(0, l2), a1 = A(1, l2), a2 = A(2, l2), a3 = A(3, l2);
+      const double q1 = rtf12 * (a1 - a3);
+      const double q2 = rtf12 * (a1 + a3);
+      B(l2, 0) = a0 + q1;
+      B(l2, 1) = a0 - q1;
+      B(l2, 2) = a2 - q2;
+      B(l2, 3) = -a2 - q2;
+    }
+#undef A
+#undef B
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reverse stages (decimation in frequency); nv = cum, lv = rem
+
+__device__ __forceinline__ void rev_r4(const double *__restrict__ a, double *__restrict__ b,
+                                       int n, int V, int cum, int lv,
+                                       const double *__restrict__ cs,
+                                       const double *__restrict__ sn) {
+  const double rtf12 = 0.7071067811865475244008443621048490392848359376884740365883398689953662;
+  const int nh = (lv - 1) / 2;
+  const int even = (lv & 1) == 0;
+  const int nk = 1 + nh + even;
+  const int per = cum * nk;
+  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+    const int v = it / per, r = it - v * per;
+    const int kk = r / cum, ic = r - kk * cum;
+    const double *av = a + v * n;
+    double *bv = b + v * n;
+#define A(k, j) av[ic + cum * ((k) + lv * (j))]
+#define B(j, k) bv[ic + cum * ((j) + 4 * (k))]
+    if (kk == 0) {
+      const double a0 = A(0, 0), a1 = A(0, 1), a2 = A(0, 2), a3 = A(0, 3);
+      const double t1r = a0 + a2;
+      const double t2r = a1;
+      const double t3r = a0 - a2;
+      const double t4r = a3;
+      B(0, 0) = t1r + t2r;
+      B(1, 0) = t3r + t4r;
+      B(2, 0) = t1r - t2r;
+      B(3, 0) = t3r - t4r;
+    } else if (kk <= nh) {
+      const int k = kk, kc = lv - k;
+      const double c1k = cs[k], s1k = sn[k];
+      const double c2k = cs[k + lv], s2k = sn[k + lv];
+      const double c3k = cs[k + 2 * lv], s3k = sn[k + 2 * lv];
+      const double ak0 = A(k, 0), ak1 = A(k, 1), ak2 = A(k, 2), ak3 = A(k, 3);
+      const double akc0 = A(kc, 0), akc1 = A(kc, 1), akc2 = A(kc, 2), akc3 = A(kc, 3);
+      const double t1r = ak0 + akc1;
+      const double t1i = akc3 - ak2;
+      const double t2r = ak1 + akc0;
+      const double t2i = akc2 - ak3;
+      const double t3r = ak0 - akc1;
+      const double t3i = akc3 + ak2;
+      const double t4r = ak1 - akc0;
+      const double t4i = akc2 + ak3;
+      const double x1p = t3r + t4i;
+      const double y1p = t3i - t4r;
+      const double x2p = t1r - t2r;
+      const double y2p = t1i - t2i;
+      const double x3p = t3r - t4i;
+      const double y3p = t3i + t4r;
+      B(0, k) = t1r + t2r;
+      B(0, kc) = t1i + t2i;
+      B(1, k) = c1k * x1p - s1k * y1p;
+      B(1, kc) = c1k * y1p + s1k * x1p;
+      B(2, k) = c2k * x2p - s2k * y2p;
+      B(2, kc) = c2k * y2p + s2k * x2p;
+      B(3, k) = c3k * x3p - s3k * y3p;
+      B(3, kc) = c3k * y3p + s3k * x3p;
+    } else {
+      const int l2 = lv / 2;
+      const double a0 = A(l2, 0), a1 = A(l2, 1), a2 = A(l2, 2), a3 = A(l2, 3);
+      B(0, l2) = a0 + a1;
+      B(2, l2) = a3 - a2;
+      const double t3r = a0 - a1;
+      const double t4r = a3 + a2;
+      B(1, l2) = rtf12 * (t3r + t4r);
+      B(3, l2) = rtf12 * (t4r - t3r);
+    }
+#undef A
+#undef B
+  }
+}
+
+__device__ __forceinline__ void rev_r2(const double *__restrict__ a, double *__restrict__ b,
+                                       int n, int V, int cum, int lv,
+                                       const double *__restrict__ cs,
+                                       const double *__restrict__ sn) {
+  const int nh = (lv - 1) / 2;
+  const int nk = 1 + nh;
+  const int per = cum * nk;
+  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+    const int v = it / per, r = it - v * per;
+    const int kk = r / cum, ic = r - kk * cum;
+    const double *av = a + v * n;
+    double *bv = b + v * n;
+#define A(k, j) av[ic + cum * ((k) + lv * (j))]
+#define B(j, k) bv[ic + cum * ((j) + 2 * (k))]
+    if (kk == 0) {
+      const double a0 = A(0, 0), a1 = A(0, 1);
+      B(0, 0) = a0 + a1;
+      B(1, 0) = a0 - a1;
+    } else {
+      const int k = kk, kc = lv - k;
+      const double c1k = cs[k], s1k = sn[k];
+      const double ak0 = A(k, 0), akc0 = A(kc, 0), ak1 = A(k, 1), akc1 = A(kc, 1);
+      const double x1p = ak0 - akc0;
+      const double y1p = akc1 + ak1;
+      B(0, k) = ak0 + akc0;
+      B(0, kc) = akc1 - ak1;
+      B(1, k) = c1k * x1p - s1k * y1p;
+      B(1, kc) = c1k * y1p + s1k * x1p;
+    }
+#undef A
+#undef B
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+template <bool FWD>
+__global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
+  extern __shared__ double lds[];
+  const int n = g.n, m = g.m, V = g.V;
+  const int v0 = blockIdx.x * V;
+  double *bufA = lds, *bufB = lds + V * n;
+  const int tot = V * n;
+  const int nhalf = n / 2;
+  // ---- load
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    int v, e;
+    if (g.in_rows) {
+      v = idx / n;
+      e = idx - v * n;
+    } else {
+      e = idx / V;
+      v = idx - e * V;
+    }
+    const int vg = v0 + v;
+    double x;
+    if (FWD) {
+      x = g.in_rows ? g.in[(int64_t)vg * n + e] : g.in[vg + (int64_t)m * e];
+    } else {
+      // optional fused derivative (sta2dfft.f90:169-245)
+      if (g.pre == 2) {
+        // along elements: der(e) from var(n-e)
+        if (e == 0 || e == nhalf) {
+          x = 0.0;
+        } else {
+          const int em = n - e;
+          const double var = g.in_rows ? g.in[(int64_t)vg * n + em] : g.in[vg + (int64_t)m * em];
+          x = (e < nhalf) ? -g.dk[2 * e - 1] * var : g.dk[2 * em - 1] * var;
+        }
+      } else if (g.pre == 1) {
+        // across vectors (vector index is the derivative's wavenumber, m = its length)
+        const int mh = m / 2;
+        if (vg == 0 || vg == mh) {
+          x = 0.0;
+        } else {
+          const int vm = m - vg;
+          const double var = g.in_rows ? g.in[(int64_t)vm * n + e] : g.in[vm + (int64_t)m * e];
+          x = (vg < mh) ? -g.dk[2 * vg - 1] * var : g.dk[2 * vm - 1] * var;
+        }
+      } else {
+        x = g.in_rows ? g.in[(int64_t)vg * n + e] : g.in[vg + (int64_t)m * e];
+      }
+      // revfft pre-processing, stafft.f90:311-325
+      if (e >= nhalf + 1) x = -x;
+      if (e == 0 || e == nhalf) x = 0.5 * x;
+    }
+    bufA[v * n + e] = x;
+  }
+  __syncthreads();
+  // ---- stages
+  double *a = bufA, *b = bufB;
+  const double *trig = g.trig;
+  if (FWD) {
+    int rem = n, cum = 1;
+    for (int s = 0; s < g.n2; s++) {
+      rem /= 2;
+      const int iloc = (rem - 1) * 2 * cum;
+      fwd_r2(a, b, n, V, rem, cum, trig + iloc, trig + n + iloc);
+      __syncthreads();
+      double *t = a; a = b; b = t;
+      cum *= 2;
+    }
+    for (int s = 0; s < g.n4; s++) {
+      rem /= 4;
+      const int iloc = (rem - 1) * 4 * cum;
+      fwd_r4(a, b, n, V, rem, cum, trig + iloc, trig + n + iloc);
+      __syncthreads();
+      double *t = a; a = b; b = t;
+      cum *= 4;
+    }
+  } else {
+    int cum = 1, rem = n;
+    for (int s = 0; s < g.n4; s++) {
+      rem /= 4;
+      const int iloc = (cum - 1) * 4 * rem;
+      rev_r4(a, b, n, V, cum, rem, trig + iloc, trig + n + iloc);
+      __syncthreads();
+      double *t = a; a = b; b = t;
+      cum *= 4;
+    }
+    for (int s = 0; s < g.n2; s++) {
+      rem /= 2;
+      const int iloc = (cum - 1) * 2 * rem;
+      rev_r2(a, b, n, V, cum, rem, trig + iloc, trig + n + iloc);
+      __syncthreads();
+      double *t = a; a = b; b = t;
+      cum *= 2;
+    }
+  }
+  // ---- store (normalisation folded in, stafft.f90:275-284 / 394-403)
+  const double post = g.post;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    int v, e;
+    if (g.out_rows) {
+      v = idx / n;
+      e = idx - v * n;
+      g.out[(int64_t)(v0 + v) * n + e] = a[v * n + e] * post;
+    } else {
+      e = idx / V;
+      v = idx - e * V;
+      g.out[(v0 + v) + (int64_t)m * e] = a[v * n + e] * post;
+    }
+  }
+}
+
+static int fft_launch(Ctx &c, bool fwd, const double *in, double *out, int n, int m,
+                      int in_rows, int out_rows, int pre, const double *dk, double post,
+                      const double *trig) {
+  FftArgs g;
+  g.in = in;
+  g.out = out;
+  g.trig = trig;
+  g.dk = dk;
+  g.mulk = nullptr;
+  g.n = n;
+  g.m = m;
+  int V = 4096 / n;
+  if (V < 1) V = 1;
+  if (V > m) V = m;
+  while (m % V) V /= 2;
+  g.V = V;
+  g.in_rows = in_rows;
+  g.out_rows = out_rows;
+  g.pre = pre;
+  g.n2 = c.plan.n2;
+  g.n4 = c.plan.n4;
+  g.post = post;
+  const size_t shm = sizeof(double) * 2 * (size_t)V * n;
+  const int grid = m / V;
+  if (fwd)
+    hipLaunchKernelGGL(fft_kernel<true>, dim3(grid), dim3(256), shm, c.s, g);
+  else
+    hipLaunchKernelGGL(fft_kernel<false>, dim3(grid), dim3(256), shm, c.s, g);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+// ptospc, sta2dfft.f90:103-132: x-transform (ny strided vectors) written as
+// rows = svar(kx,iy); then the y-transform of svar in place.
+int fft_ptospc(Ctx &c, const double *rvar, double *svar) {
+  const int n = c.c.nx;
+  KTimer t(c, "fft_ptospc", 4 * 8 * c.N);
+  HYD_CHECK(fft_launch(c, true, rvar, svar, n, n, 0, 1, 0, nullptr, c.plan.fnorm, c.xtrig));
+  HYD_CHECK(fft_launch(c, true, svar, svar, n, n, 0, 0, 0, nullptr, c.plan.fnorm, c.ytrig));
+  return HYD_OK;
+}
+
+// spctop, sta2dfft.f90:136-165, with an optional fused xderiv / yderiv of
+// the input (spectral.f90:328-332, 359-363) and a sign folded into the last
+// store (spectral.f90:335-339).  tmp is an (nx,ny) work array.
+int fft_spctop(Ctx &c, const double *svar, double *rvar, int pre, double post_sign, double *tmp) {
+  const int n = c.c.nx;
+  KTimer t(c, "fft_spctop", 4 * 8 * c.N);
+  const double *dk = pre == PRE_XDERIV ? c.hrkx : (pre == PRE_YDERIV ? c.hrky : nullptr);
+  // y-inverse: vectors kx (strided), elements ky.  xderiv acts across the
+  // vectors (pre=1), yderiv along the elements (pre=2).
+  HYD_CHECK(fft_launch(c, false, svar, tmp, n, n, 0, 0, pre, dk, c.plan.rnorm, c.ytrig));
+  HYD_CHECK(fft_launch(c, false, tmp, rvar, n, n, 1, 0, 0, nullptr, c.plan.rnorm * post_sign,
+                       c.xtrig));
+  return HYD_OK;
+}
+
+// forfft(m, ng, x) / revfft(m, ng, x) on a strided (m, ng) array in place.
+int fft_forward_1d(Ctx &c, double *x, int m) {
+  return fft_launch(c, true, x, x, c.c.nx, m, 0, 0, 0, nullptr, c.plan.fnorm, c.xtrig);
+}
+int fft_reverse_1d(Ctx &c, double *x, int m) {
+  return fft_launch(c, false, x, x, c.c.nx, m, 0, 0, 0, nullptr, c.plan.rnorm, c.xtrig);
+}
+
+}  // namespace hyd

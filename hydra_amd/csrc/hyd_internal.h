@@ -1,0 +1,164 @@
+// hyd_internal.h -- shared definitions of the MI355X contour-advection library.
+//
+// Device layouts (all float64 unless noted):
+//   physical field  (ny,nx) column-major, iy fastest     -> f[ix*ny + iy]
+//   spectral field  (nx,ny) column-major, kx fastest     -> s[ky*nx + kx]
+//   fine grid of con2grid, column-major by fine column ix, rows iy=1..nyf:
+//     cnt[(ix-1)*nyf + (iy-1)] (packed int32 +/- crossing counts) and the
+//     swept field fq[(ix-1)*nyf + (iy-1)] (float64)
+//   contour nodes: structure of arrays xq,yq,nextq (1-based successor)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hydra_amd.h"
+
+namespace hyd {
+
+void set_error(const std::string &msg);
+
+#define HYD_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      ::hyd::set_error(std::string(#call) + ": " + hipGetErrorString(e_));        \
+      return HYD_EHIP;                                                             \
+    }                                                                              \
+  } while (0)
+
+#define HYD_CHECK(expr)          \
+  do {                           \
+    int rc_ = (expr);            \
+    if (rc_ != HYD_OK) return rc_; \
+  } while (0)
+
+// Host-side constants: the run-time restatement of constants.f90 and the
+// parameter block of contours.f90:17-42.
+struct Consts {
+  int nx, ny, nxf, nyf, nxu, nyu;
+  double ellx, elly, hlx, hlxi, xmin, xmax, hly, hlyi, ymin, ymax;
+  double kdsq, srwfm, glx, glxi, gly, glyi, garea, dsumi;
+  double glxf, glxfi, glyf, glyfi, glxu, glxui, glyu, glyui;
+  double amu, ell, dm, dm2, d4small, dmsq, dmi, elf, densf;
+  double oms, small, small3;
+  bool btropic, beffect;
+};
+
+// Spectral tables built on the host (tables.cpp).
+struct HostTables {
+  std::vector<double> green, flo, fhi, qdiss, rksq, rk;   // (nx,ny)
+  std::vector<double> rkx, hrkx, rky, hrky, bety;          // 1-d
+  std::vector<double> xtrig, ytrig;                        // 2n
+  int xfactors[5], yfactors[5];
+  // distinct values of qdiss and the map cell -> distinct index (for the
+  // host-evaluated integrating factor, evolution.f90:603-605)
+  std::vector<double> qdiss_u;
+  std::vector<int32_t> qdiss_map;
+};
+
+int build_consts(const hyd_params &p, Consts &c);
+int build_tables(const hyd_params &p, const Consts &c, HostTables &t);
+
+// FFT plan for one length n = 2^p (radix-2 stage then radix-4 stages, the
+// factor order of stafft.f90:146-178 and 212-271).
+struct FftPlan {
+  int n;
+  int n2;  // number of radix-2 factors (0 or 1)
+  int n4;  // number of radix-4 factors
+  double fnorm, rnorm;  // 1/sqrt(n) and 2/sqrt(n)
+};
+
+// Device context.
+struct Scalars {  // lives in device memory, mirrored on the host
+  double t, dt, dt2, dt3, dt6, twist, zzrms, zzmax, uumax, qjump, dtmax, dtacc;
+};
+
+struct Ctx {
+  hyd_params p;
+  Consts c;
+  HostTables ht;
+  FftPlan plan;
+  hipStream_t s = nullptr;
+  int dev = 0;
+  int64_t N = 0;        // ng*ng
+  int64_t cap = 0;      // node capacity
+  // tables on device
+  double *green = nullptr, *flo = nullptr, *fhi = nullptr, *hrkx = nullptr, *hrky = nullptr;
+  double *xtrig = nullptr, *ytrig = nullptr, *xgf = nullptr;
+  int32_t *qdiss_map = nullptr;
+  double *qdiss_exp = nullptr;  // exp(dfac*qdiss_u) for the current step
+  int nqu = 0;
+  // fields
+  double *qs = nullptr, *qd = nullptr, *qc = nullptr, *qq = nullptr, *pp = nullptr;
+  double *uu = nullptr, *vv = nullptr, *qr = nullptr, *emq = nullptr, *epq = nullptr;
+  double *qsi = nullptr, *qsf = nullptr, *sqs = nullptr, *qdi = nullptr, *qdf = nullptr;
+  double *sqd = nullptr, *wa = nullptr, *wb = nullptr, *wc = nullptr, *wd = nullptr;
+  // con2grid work
+  int32_t *cnt = nullptr;      // (nyf+1) x nxf packed counts
+  double *fq = nullptr;        // nyf x nxf swept fine field
+  double *mq = nullptr;        // (2ng)^2 mid grid
+  int32_t *qjx_cnt = nullptr;  // nxf packed bottom-row counts
+  double *qbot = nullptr;      // nxf
+  int32_t *ixc = nullptr, *nxc = nullptr;
+  double *ndx = nullptr, *ndy = nullptr;
+  int32_t *flags = nullptr;    // small device counters
+  int32_t *replay = nullptr;   // rare order-dependent cells: (cell, node, sign)
+  double *replay_val = nullptr;
+  int64_t replay_cap = 0;
+  // nodes
+  double *xq = nullptr, *yq = nullptr, *xqi = nullptr, *yqi = nullptr, *xqf = nullptr;
+  double *yqf = nullptr, *uq = nullptr, *vq = nullptr;
+  int32_t *nextq = nullptr;
+  int32_t *nxt_tmp = nullptr;  // staging for the host-array entry points
+  std::vector<int32_t> i1q, i2q, npq, indq;  // contour table (host copy)
+  int nptq = 0, nq = 0;
+  bool contex = false;
+  bool have_field = false;
+  // scalars
+  Scalars *dsc = nullptr;   // device
+  Scalars *hsc = nullptr;   // pinned host mirror
+  double *dsum = nullptr;   // device scratch for sequential sums (8 slots)
+  std::vector<double> exp_host;  // host-evaluated exp table of the step
+  void *seq_work = nullptr; // chunk table of seqsum
+  int64_t seq_cap = 0;      // chunks it holds
+  // timing
+  bool timing = false;
+  struct KRec { std::string name; hipEvent_t a, b; int64_t bytes; };
+  std::vector<KRec> krec;
+};
+
+// ---- launchers (each enqueues on ctx.s) ----------------------------------
+// FFT: ptospc / spctop with optional fused spectral pre-operations.
+enum SpecPre { PRE_NONE = 0, PRE_XDERIV = 1, PRE_YDERIV = 2 };
+int fft_ptospc(Ctx &c, const double *rvar, double *svar);
+int fft_spctop(Ctx &c, const double *svar, double *rvar, int pre, double post_sign,
+               double *tmp);
+int fft_forward_1d(Ctx &c, double *x, int m);   // forfft(m, ng, x)
+int fft_reverse_1d(Ctx &c, double *x, int m);   // revfft(m, ng, x)
+
+int main_invert_dev(Ctx &c, const double *qq, double *uu, double *vv, double *pp, double *tmp);
+int gradient_dev(Ctx &c, const double *ff, double *fx, double *fy, double *tmp);
+int launch_xderiv(Ctx &c, const double *var, double *der);
+int launch_yderiv(Ctx &c, const double *var, double *der);
+
+int seqsum(Ctx &c, const double *a, int64_t n, double *dout);
+int64_t seqsum_work_bytes(int64_t n);
+
+int velint(Ctx &c, const double *uu, const double *vv, const double *xq, const double *yq,
+           int n, double *uq, double *vq);
+int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n,
+             double *qc, int32_t *counts_out);
+
+// timing helpers
+struct KTimer {
+  Ctx &c;
+  int idx = -1;
+  KTimer(Ctx &ctx, const char *name, int64_t bytes);
+  ~KTimer();
+};
+
+}  // namespace hyd

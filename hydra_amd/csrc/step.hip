@@ -1,0 +1,376 @@
+// step.hip -- the RK4 step of evolution.f90 on the device-resident state.
+//
+//   inversion  evolution.f90:494-520   con2grid -> ptospc -> qq -> main_invert
+//   adapt      evolution.f90:524-608   spctop(qq+kd^2 pp) -> sums / maxima -> dt
+//   source     evolution.f90:432-490   gradients -> products -> ptospc
+//   advance    evolution.f90:166-326   four stages of the above plus the
+//                                      node and spectral RK updates
+//
+// Pointwise kernels keep the reference's expression order.  The two
+// order-sensitive reductions (qavg0 inside con2grid, zzrms here) use the
+// exact sequential sum; the maxima are order-free.  The integrating factor
+// exp(dt/2*zzrms*qdiss) is evaluated on the host with the same libm as the
+// reference, once per distinct qdiss value, and expanded on the device.
+#include <math.h>
+
+#include <thread>
+
+#include "hyd_internal.h"
+
+namespace hyd {
+
+int rk_nodes(Ctx &c, int stage);
+
+#define PW_KERNEL(name, params, body)                                   \
+  __global__ void name params {                                         \
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   \
+    if (k >= n) return;                                                 \
+    body;                                                               \
+  }
+
+// qq = flo*(qs-qc)+qc+qd  (evolution.f90:514)
+PW_KERNEL(k_combine_qq,
+          (double *__restrict__ qq, const double *__restrict__ flo, const double *__restrict__ qs,
+           const double *__restrict__ qc, const double *__restrict__ qd, int64_t n),
+          { qq[k] = flo[k] * (qs[k] - qc[k]) + qc[k] + qd[k]; })
+
+// pp = green*qq  (spectral.f90:321-325)
+PW_KERNEL(k_green, (double *__restrict__ pp, const double *__restrict__ green,
+                    const double *__restrict__ qq, int64_t n),
+          { pp[k] = green[k] * qq[k]; })
+
+// qs = flo*(qs-qc)+qc+qd ; qd = fhi*(qs-qc)  (evolution.f90:199-200, 126)
+PW_KERNEL(k_reset, (double *__restrict__ qs, double *__restrict__ qd,
+                    const double *__restrict__ qc, const double *__restrict__ flo,
+                    const double *__restrict__ fhi, int64_t n),
+          {
+            const double s = flo[k] * (qs[k] - qc[k]) + qc[k] + qd[k];
+            qs[k] = s;
+            qd[k] = fhi[k] * (s - qc[k]);
+          })
+
+// qd = fhi*(qs-qc)  (evolution.f90:126)
+PW_KERNEL(k_init_qd, (double *__restrict__ qd, const double *__restrict__ qs,
+                      const double *__restrict__ qc, const double *__restrict__ fhi, int64_t n),
+          { qd[k] = fhi[k] * (qs[k] - qc[k]); })
+
+// ss = qq + kdsq*pp  (evolution.f90:546)
+PW_KERNEL(k_qq_kd, (double *__restrict__ ss, const double *__restrict__ qq,
+                    const double *__restrict__ pp, double kdsq, int64_t n),
+          { ss[k] = qq[k] + kdsq * pp[k]; })
+
+// zz = zz**2 and the maxima of zz**2 and uu**2+vv**2 (evolution.f90:552-556)
+__global__ void k_adapt_reduce(double *__restrict__ zz, const double *__restrict__ uu,
+                               const double *__restrict__ vv, int64_t n,
+                               unsigned long long *__restrict__ mx) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double z2 = 0.0, u2 = 0.0;
+  if (k < n) {
+    const double z = zz[k];
+    z2 = z * z;
+    zz[k] = z2;
+    const double u = uu[k], v = vv[k];
+    u2 = u * u + v * v;
+  }
+  // non-negative doubles order like their bit patterns
+  for (int off = 32; off > 0; off >>= 1) {
+    z2 = fmax(z2, __shfl_down(z2, off));
+    u2 = fmax(u2, __shfl_down(u2, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&mx[0], (unsigned long long)__double_as_longlong(z2));
+    atomicMax(&mx[1], (unsigned long long)__double_as_longlong(u2));
+  }
+}
+
+struct AdaptArgs {
+  double dsumi, small, glx, srwfm, dt_fixed, tgsave, tcsave;
+};
+
+// evolution.f90:553-578: scalars of adapt.
+__global__ void k_adapt_scalars(Scalars *sc, const double *sum, const unsigned long long *mx,
+                                AdaptArgs a) {
+  if (threadIdx.x != 0) return;
+  const double dtfac = 3.141592653589793238462643383279502884197169399375105820974944592307816 / 10.0;
+  const double cflmax = 0.7;
+  const double zzrms = sqrt(a.dsumi * sum[0]);
+  const double zzmax = sqrt(__longlong_as_double((long long)mx[0])) + a.small;
+  const double uumax = sqrt(__longlong_as_double((long long)mx[1])) + a.small;
+  const double t1 = a.glx * cflmax / uumax;
+  const double zm = zzmax > a.srwfm ? zzmax : a.srwfm;
+  const double t2 = dtfac / zm;
+  const double dtacc = t1 < t2 ? t1 : t2;
+  double dt = sc->dt;
+  if (a.dt_fixed > 0.0) {
+    dt = a.dt_fixed;
+  } else if (dt > 0.0) {
+    dt = dtacc < sc->dtmax ? dtacc : sc->dtmax;
+  } else {
+    const double tm = a.tgsave < a.tcsave ? a.tgsave : a.tcsave;
+    sc->dtmax = 0.2 * tm;
+    dt = dtacc < sc->dtmax ? dtacc : sc->dtmax;
+  }
+  sc->dt = dt;
+  sc->dt2 = dt * 0.5;
+  sc->dt3 = dt * (1.0 / 3.0);
+  sc->dt6 = dt * (1.0 / 6.0);
+  sc->twist = sc->twist + dt * zzmax;
+  sc->zzrms = zzrms;
+  sc->zzmax = zzmax;
+  sc->uumax = uumax;
+  sc->dtacc = dtacc;
+}
+
+// epq = exp(dfac*qdiss) from the host-evaluated distinct values; emq = 1/epq
+PW_KERNEL(k_expand_epq, (double *__restrict__ epq, double *__restrict__ emq,
+                         const int32_t *__restrict__ map, const double *__restrict__ e,
+                         int64_t n),
+          {
+            const double v = e[map[k]];
+            epq[k] = v;
+            emq[k] = 1.0 / v;
+          })
+
+// qqx = -uu*qqx - vv*qqy  (evolution.f90:449, 460) / with beta (458)
+PW_KERNEL(k_advect_prod, (double *__restrict__ qqx, const double *__restrict__ qqy,
+                          const double *__restrict__ uu, const double *__restrict__ vv,
+                          double beta, int withbeta, int64_t n),
+          {
+            if (withbeta)
+              qqx[k] = -uu[k] * qqx[k] - vv[k] * (qqy[k] + beta);
+            else
+              qqx[k] = -uu[k] * qqx[k] - vv[k] * qqy[k];
+          })
+
+// sqd = sqd - rekman*(qq+kdsq*pp), then the integrating factor (476, 483-487)
+PW_KERNEL(k_source_finish, (double *__restrict__ sqd, const double *__restrict__ qq,
+                            const double *__restrict__ pp, const double *__restrict__ epq,
+                            double rekman, double kdsq, int lev, int64_t n),
+          {
+            double s = sqd[k] - rekman * (qq[k] + kdsq * pp[k]);
+            if (lev == 1) s = epq[k] * s;
+            else if (lev == 2) s = epq[k] * epq[k] * s;
+            sqd[k] = s;
+          })
+
+// spectral RK updates, evolution.f90:228-233, 259-262, 287-291, 313-314
+__global__ void k_rk_fields(double *__restrict__ qs, double *__restrict__ qd,
+                            double *__restrict__ qsi, double *__restrict__ qsf,
+                            double *__restrict__ qdi, double *__restrict__ qdf,
+                            const double *__restrict__ sqs, const double *__restrict__ sqd,
+                            double *__restrict__ emq, int stage, const Scalars *__restrict__ sc,
+                            int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double dt = sc->dt, dt2 = sc->dt2, dt3 = sc->dt3, dt6 = sc->dt6;
+  if (stage == 1) {
+    const double s0 = qs[k], d0 = qd[k];
+    qsi[k] = s0;
+    qs[k] = s0 + dt2 * sqs[k];
+    qsf[k] = s0 + dt6 * sqs[k];
+    qdi[k] = d0;
+    qd[k] = emq[k] * (d0 + dt2 * sqd[k]);
+    qdf[k] = d0 + dt6 * sqd[k];
+  } else if (stage == 2) {
+    qs[k] = qsi[k] + dt2 * sqs[k];
+    qsf[k] = qsf[k] + dt3 * sqs[k];
+    qd[k] = emq[k] * (qdi[k] + dt2 * sqd[k]);
+    qdf[k] = qdf[k] + dt3 * sqd[k];
+  } else if (stage == 3) {
+    qs[k] = qsi[k] + dt * sqs[k];
+    qsf[k] = qsf[k] + dt3 * sqs[k];
+    const double e = emq[k] * emq[k];
+    emq[k] = e;
+    qd[k] = e * (qdi[k] + dt * sqd[k]);
+    qdf[k] = qdf[k] + dt3 * sqd[k];
+  } else {
+    qs[k] = qsf[k] + dt6 * sqs[k];
+    qd[k] = emq[k] * (qdf[k] + dt6 * sqd[k]);
+  }
+}
+
+static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+int inversion(Ctx &c) {
+  const int64_t N = c.N;
+  HYD_CHECK(con2grid(c, c.xq, c.yq, c.nextq, c.nptq, c.wa, nullptr));
+  HYD_CHECK(fft_ptospc(c, c.wa, c.qc));
+  {
+    KTimer t(c, "combine_qq", 40 * N);
+    hipLaunchKernelGGL(k_combine_qq, grid1(N), dim3(256), 0, c.s, c.qq, c.flo, c.qs, c.qc, c.qd, N);
+    hipLaunchKernelGGL(k_green, grid1(N), dim3(256), 0, c.s, c.pp, c.green, c.qq, N);
+  }
+  HYD_HIP(hipGetLastError());
+  // main_invert: vv = spctop(xderiv(pp)), uu = -spctop(yderiv(pp))
+  HYD_CHECK(fft_spctop(c, c.pp, c.vv, PRE_XDERIV, 1.0, c.wc));
+  HYD_CHECK(fft_spctop(c, c.pp, c.uu, PRE_YDERIV, -1.0, c.wc));
+  return HYD_OK;
+}
+
+// main_invert (spectral.f90:304-342) on device arrays; tmp is a work array.
+int main_invert_dev(Ctx &c, const double *qq, double *uu, double *vv, double *pp, double *tmp) {
+  const int64_t N = c.N;
+  hipLaunchKernelGGL(k_green, grid1(N), dim3(256), 0, c.s, pp, c.green, qq, N);
+  HYD_HIP(hipGetLastError());
+  HYD_CHECK(fft_spctop(c, pp, vv, PRE_XDERIV, 1.0, tmp));
+  HYD_CHECK(fft_spctop(c, pp, uu, PRE_YDERIV, -1.0, tmp));
+  return HYD_OK;
+}
+
+// gradient (spectral.f90:345-366) on device arrays.
+int gradient_dev(Ctx &c, const double *ff, double *fx, double *fy, double *tmp) {
+  HYD_CHECK(fft_spctop(c, ff, fx, PRE_XDERIV, 1.0, tmp));
+  HYD_CHECK(fft_spctop(c, ff, fy, PRE_YDERIV, 1.0, tmp));
+  return HYD_OK;
+}
+
+// exp(dfac*qdiss_u[j]) on the host, split over a few threads.
+static void host_exp_table(const std::vector<double> &qu, double dfac, double *out) {
+  const size_t n = qu.size();
+  unsigned nt = std::thread::hardware_concurrency();
+  if (nt > 8) nt = 8;
+  if (nt < 1 || n < 4096) nt = 1;
+  auto work = [&](size_t b, size_t e) {
+    for (size_t j = b; j < e; j++) out[j] = exp(dfac * qu[j]);
+  };
+  if (nt == 1) {
+    work(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; t++) {
+    const size_t b = t * per, e = std::min(n, b + per);
+    if (b < e) th.emplace_back(work, b, e);
+  }
+  for (auto &x : th) x.join();
+}
+
+int adapt(Ctx &c) {
+  const int64_t N = c.N;
+  double *ss = c.wa, *zz = c.wb;
+  const double *src = c.qq;
+  if (!c.c.btropic) {
+    hipLaunchKernelGGL(k_qq_kd, grid1(N), dim3(256), 0, c.s, ss, c.qq, c.pp, c.c.kdsq, N);
+    src = ss;
+  }
+  HYD_CHECK(fft_spctop(c, src, zz, PRE_NONE, 1.0, c.wc));
+  unsigned long long *mx = (unsigned long long *)(c.dsum + 4);
+  HYD_HIP(hipMemsetAsync(mx, 0, 2 * sizeof(unsigned long long), c.s));
+  {
+    KTimer t(c, "adapt_reduce", 32 * N);
+    hipLaunchKernelGGL(k_adapt_reduce, grid1(N), dim3(256), 0, c.s, zz, c.uu, c.vv, N, mx);
+  }
+  HYD_CHECK(seqsum(c, zz, N, c.dsum + 1));
+  AdaptArgs a;
+  a.dsumi = c.c.dsumi;
+  a.small = c.c.small;
+  a.glx = c.c.glx;
+  a.srwfm = c.c.srwfm;
+  a.dt_fixed = c.p.dt_fixed;
+  a.tgsave = c.p.tgsave;
+  a.tcsave = c.p.tcsave;
+  hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, mx, a);
+  HYD_HIP(hipGetLastError());
+  // integrating factors, evolution.f90:603-605
+  HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  const double dfac = c.hsc->dt2 * c.hsc->zzrms;
+  std::vector<double> &tab = c.exp_host;
+  tab.resize(c.ht.qdiss_u.size());
+  host_exp_table(c.ht.qdiss_u, dfac, tab.data());
+  HYD_HIP(hipMemcpyAsync(c.qdiss_exp, tab.data(), sizeof(double) * tab.size(),
+                         hipMemcpyHostToDevice, c.s));
+  hipLaunchKernelGGL(k_expand_epq, grid1(N), dim3(256), 0, c.s, c.epq, c.emq, c.qdiss_map,
+                     c.qdiss_exp, N);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+int source(Ctx &c, int lev) {
+  const int64_t N = c.N;
+  const int wb = c.c.beffect ? 1 : 0;
+  // qd source
+  HYD_CHECK(fft_spctop(c, c.qd, c.wa, PRE_XDERIV, 1.0, c.wc));
+  HYD_CHECK(fft_spctop(c, c.qd, c.wb, PRE_YDERIV, 1.0, c.wc));
+  {
+    KTimer t(c, "advect_prod", 40 * N);
+    hipLaunchKernelGGL(k_advect_prod, grid1(N), dim3(256), 0, c.s, c.wa, c.wb, c.uu, c.vv,
+                       c.p.beta, 0, N);
+  }
+  HYD_CHECK(fft_ptospc(c, c.wa, c.sqd));
+  // qs source
+  HYD_CHECK(fft_spctop(c, c.qs, c.wa, PRE_XDERIV, 1.0, c.wc));
+  HYD_CHECK(fft_spctop(c, c.qs, c.wb, PRE_YDERIV, 1.0, c.wc));
+  {
+    KTimer t(c, "advect_prod", 40 * N);
+    hipLaunchKernelGGL(k_advect_prod, grid1(N), dim3(256), 0, c.s, c.wa, c.wb, c.uu, c.vv,
+                       c.p.beta, wb, N);
+  }
+  HYD_CHECK(fft_ptospc(c, c.wa, c.sqs));
+  {
+    KTimer t(c, "source_finish", 40 * N);
+    hipLaunchKernelGGL(k_source_finish, grid1(N), dim3(256), 0, c.s, c.sqd, c.qq, c.pp, c.epq,
+                       c.p.rekman, c.c.kdsq, lev, N);
+  }
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+static int rk_fields(Ctx &c, int stage) {
+  const int64_t N = c.N;
+  static const int64_t words[5] = {0, 11, 10, 12, 6};
+  KTimer t(c, "rk_fields", words[stage] * 8 * N);
+  hipLaunchKernelGGL(k_rk_fields, grid1(N), dim3(256), 0, c.s, c.qs, c.qd, c.qsi, c.qsf, c.qdi,
+                     c.qdf, c.sqs, c.sqd, c.emq, stage, c.dsc, N);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+// Advance the host mirror of t by dt/2 (t=t+dt2, evolution.f90:237, 295).
+__global__ void k_tick(Scalars *sc) {
+  if (threadIdx.x == 0) sc->t = sc->t + sc->dt2;
+}
+
+int advance(Ctx &c) {
+  const int64_t N = c.N;
+  // stage 1
+  HYD_CHECK(inversion(c));
+  hipLaunchKernelGGL(k_reset, grid1(N), dim3(256), 0, c.s, c.qs, c.qd, c.qc, c.flo, c.fhi, N);
+  HYD_CHECK(adapt(c));
+  HYD_CHECK(source(c, 0));
+  HYD_CHECK(rk_nodes(c, 1));
+  HYD_CHECK(rk_fields(c, 1));
+  // stage 2
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
+  HYD_CHECK(inversion(c));
+  HYD_CHECK(source(c, 1));
+  HYD_CHECK(rk_nodes(c, 2));
+  HYD_CHECK(rk_fields(c, 2));
+  // stage 3
+  HYD_CHECK(inversion(c));
+  HYD_CHECK(source(c, 1));
+  HYD_CHECK(rk_nodes(c, 3));
+  HYD_CHECK(rk_fields(c, 3));
+  // stage 4
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
+  HYD_CHECK(inversion(c));
+  HYD_CHECK(source(c, 2));
+  HYD_CHECK(rk_nodes(c, 4));
+  HYD_CHECK(rk_fields(c, 4));
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+// evolution.f90:96-131 (init): qc from the contours and qd = fhi*(qs-qc).
+int evolution_init(Ctx &c) {
+  const int64_t N = c.N;
+  c.contex = c.nptq > 0;
+  HYD_CHECK(con2grid(c, c.xq, c.yq, c.nextq, c.nptq, c.wa, nullptr));
+  HYD_CHECK(fft_ptospc(c, c.wa, c.qc));
+  hipLaunchKernelGGL(k_init_qd, grid1(N), dim3(256), 0, c.s, c.qd, c.qs, c.qc, c.fhi, N);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+}  // namespace hyd

@@ -1,0 +1,141 @@
+/*
+ * hydra_amd.h -- C ABI of the MI355X contour-advection step.
+ *
+ * A drop-in for the doubly periodic CAPS step of Hydra (ca/plane/qg/caps and
+ * lib/stafft).  Every entry point below replaces one reference routine,
+ * cited as file:line (paths relative to basic_hydra/).  All state lives in an
+ * opaque context; arrays crossing the boundary are plain pointers and sizes.
+ *
+ * Conventions, identical to the reference's Fortran so that a host can pass
+ * its module arrays unchanged:
+ *   - physical fields are (ny,nx) column-major (iy fastest), length ng*ng
+ *   - spectral fields are (nx,ny) column-major (kx fastest) in the stafft
+ *     half-complex layout
+ *   - contour index arrays (nextq, i1q, i2q) hold 1-based node numbers;
+ *     contours of one level are contiguous and levels ascend (indq)
+ *   - "host" entry points take host pointers and copy in/out; the
+ *     device-resident step (hyd_advance / hyd_advect_step) keeps the whole
+ *     state in HBM between calls
+ *
+ * Every function returns 0 on success or a negative HYD_E* code; the text
+ * of the last error is available from hyd_last_error().  (The reference
+ * stops the program on the same conditions: stafft.f90:85-93,
+ * sta2dfft.f90:70-93.)
+ */
+#ifndef HYDRA_AMD_H
+#define HYDRA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HYD_OK 0
+#define HYD_EARG (-1)      /* invalid argument                             */
+#define HYD_EFACTOR (-2)   /* grid length not a power of two (radix 2/4)  */
+#define HYD_ECAPACITY (-3) /* node / contour / crossing capacity exceeded */
+#define HYD_EHIP (-4)      /* HIP runtime error                            */
+#define HYD_ENOMEM (-5)
+#define HYD_ESTATE (-6)    /* call out of order (no contours, no field)   */
+
+typedef struct hyd_ctx hyd_ctx;
+
+/* Run-time form of the compile-time parameters of parameters.f90:14-36 and
+   the capacities of constants.f90:95-101. */
+typedef struct hyd_params {
+  int32_t ng;            /* nx = ny, a power of two >= 8                   */
+  int32_t ncontq;        /* PV contour intervals (parameters.f90:17)       */
+  double ellx, elly;     /* domain lengths (parameters.f90:29)             */
+  double kd, beta;       /* parameters.f90:30                              */
+  double rtherm, rekman; /* parameters.f90:31 (rtherm must be 0)           */
+  double cdamp, nnu;     /* parameters.f90:32                              */
+  double tgsave, tcsave; /* parameters.f90:23 (only dtmax depends on them) */
+  double dt_fixed;       /* > 0: fixed time step instead of adapt's choice */
+  int64_t node_capacity; /* device node capacity (0: 64*ng*ng)             */
+  int32_t precision;     /* 0: plain FP64, operation-order faithful        */
+  int32_t device;        /* HIP device ordinal                             */
+} hyd_params;
+
+const char *hyd_last_error(void);
+
+/* init_spectral + init_contours: spectral.f90:42-158, contours.f90:48-132,
+   sta2dfft.f90:40-97, stafft.f90:63-125.  Tables are built on the host with
+   the same libm calls as the reference and uploaded once. */
+int hyd_init(const hyd_params *p, hyd_ctx **ctx);
+int hyd_destroy(hyd_ctx *ctx);
+
+/* ---- lib/stafft/sta2dfft.f90 (doubly periodic part) ---------------------- */
+/* ptospc(nx,ny,rvar,svar,...) sta2dfft.f90:103-132.  rvar(ny,nx) -> svar(nx,ny).
+   (The reference destroys rvar; here it is left unchanged.) */
+int hyd_ptospc(hyd_ctx *ctx, const double *rvar, double *svar);
+/* spctop(nx,ny,svar,rvar,...) sta2dfft.f90:136-165 */
+int hyd_spctop(hyd_ctx *ctx, const double *svar, double *rvar);
+/* xderiv / yderiv with the module's filtered wavenumbers hrkx / hrky,
+   sta2dfft.f90:169-245 */
+int hyd_xderiv(hyd_ctx *ctx, const double *var, double *der);
+int hyd_yderiv(hyd_ctx *ctx, const double *var, double *der);
+
+/* ---- ca/plane/qg/caps/spectral.f90 --------------------------------------- */
+/* main_invert(qq,uu,vv,pp) spectral.f90:304-342 */
+int hyd_main_invert(hyd_ctx *ctx, const double *qq, double *uu, double *vv, double *pp);
+/* gradient(ff,ffx,ffy) spectral.f90:345-366 */
+int hyd_gradient(hyd_ctx *ctx, const double *ff, double *ffx, double *ffy);
+/* Spectral tables of init_spectral (spectral.f90:14-22): id 1 green, 2 flo,
+   3 fhi, 4 qdiss, 5 rksq (ng*ng each); 6 hrkx, 7 rkx, 8 hrky, 9 rky (ng);
+   10 xtrig, 11 ytrig (2*ng). */
+int hyd_get_table(hyd_ctx *ctx, int32_t id, double *out);
+
+/* ---- ca/plane/qg/caps/contours.f90 ---------------------------------------- */
+/* velint(uu,vv,uq,vq) contours.f90:136-169 over nodes (xq,yq) */
+int hyd_velint(hyd_ctx *ctx, const double *uu, const double *vv, const double *xq,
+               const double *yq, int32_t nptq, double *uq, double *vq);
+/* con2grid(qc) contours.f90:173-346.  The module contour arrays are passed
+   explicitly.  If counts != NULL the integer PV level counts of the 4ng
+   fine grid after the y sweep (round(qa/qjump), (4ng,4ng) column-major) are
+   returned as well. */
+int hyd_con2grid(hyd_ctx *ctx, const double *xq, const double *yq, const int32_t *nextq,
+                 int32_t nptq, double qjump, double *qc, int32_t *counts);
+
+/* ---- exact sequential reduction ------------------------------------------ */
+/* The value gfortran's SUM(a) (and the explicit loops of contours.f90:330-336)
+   produce: ((a1 + a2) + a3) + ... in storage order, bit for bit.  Used for
+   qavg0 (contours.f90:331-336) and zzrms (evolution.f90:553). */
+int hyd_sum_sequential(hyd_ctx *ctx, const double *a, int64_t n, double *sum);
+
+/* ---- device-resident state and the RK4 step ------------------------------ */
+/* caps.f90:143-183 (loopinit=0): read the full PV qq(ny,nx) at time t0,
+   set qjump (contint, generic.f90:216-229), qs = ptospc(qq), no contours. */
+int hyd_set_field(hyd_ctx *ctx, const double *qq, double t0);
+/* Replace the device contour store (contours.f90:13-15). */
+int hyd_set_contours(hyd_ctx *ctx, int32_t nptq, int32_t nq, const double *xq, const double *yq,
+                     const int32_t *nextq, const int32_t *i1q, const int32_t *i2q,
+                     const int32_t *npq, const int32_t *indq, double qjump);
+int hyd_get_counts(hyd_ctx *ctx, int32_t *nptq, int32_t *nq);
+int hyd_get_contours(hyd_ctx *ctx, double *xq, double *yq, int32_t *nextq, int32_t *i1q,
+                     int32_t *i2q, int32_t *npq, int32_t *indq);
+/* Fields: 0 qs, 1 qd, 2 qc, 3 qq, 4 pp, 5 uu, 6 vv, 7 qr, 8 epq, 9 emq */
+int hyd_get_field(hyd_ctx *ctx, int32_t id, double *out);
+int hyd_put_field(hyd_ctx *ctx, int32_t id, const double *in);
+/* Scalars: 0 t, 1 dt, 2 twist, 3 zzrms, 4 zzmax, 5 uumax, 6 qjump, 7 dtmax */
+int hyd_get_scalar(hyd_ctx *ctx, int32_t id, double *out);
+int hyd_set_scalar(hyd_ctx *ctx, int32_t id, double v);
+
+/* init (evolution.f90:96-131): qc from the contours, qd = fhi*(qs-qc). */
+int hyd_evolution_init(hyd_ctx *ctx);
+/* inversion (evolution.f90:494-520) on the device state. */
+int hyd_inversion(hyd_ctx *ctx);
+/* advance (evolution.f90:166-326): one RK4 step of nodes and fields. */
+int hyd_advance(hyd_ctx *ctx);
+
+/* Synchronise the context's stream (timing helpers). */
+int hyd_sync(hyd_ctx *ctx);
+/* Kernel-time accounting of the dominant kernels, measured with HIP events
+   on the context's stream: returns the number of records written. */
+int hyd_kernel_times(hyd_ctx *ctx, int32_t max, char *names, double *ms, int64_t *bytes);
+int hyd_set_timing(hyd_ctx *ctx, int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

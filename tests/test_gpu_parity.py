@@ -1,0 +1,130 @@
+"""Bit-for-bit parity of the HIP path against the oracle (run on the GPU).
+
+Every comparison here is exact (integer or float64 bit patterns): the plain
+FP64 path follows the reference's operation order, so any difference is a
+defect."""
+import numpy as np
+import pytest
+
+from tests.helpers import bits_equal, oracle_state
+
+pytestmark = pytest.mark.gpu
+
+NGS = [32, 64]
+
+
+@pytest.fixture(scope="module")
+def hyd_cache():
+    from hydra_amd import Hydra
+    cache = {}
+
+    def get(ng):
+        if ng not in cache:
+            cache[ng] = Hydra(ng)
+        return cache[ng]
+    return get
+
+
+@pytest.mark.parametrize("ng", NGS)
+def test_tables_match_oracle(ng, hyd_cache):
+    from oracle.hyo import Oracle
+    o = Oracle(ng)
+    h = hyd_cache(ng)
+    for name, n in [("green", None), ("flo", None), ("fhi", None), ("qdiss", None),
+                    ("hrkx", ng), ("hrky", ng), ("rkx", ng), ("xtrig", 2 * ng)]:
+        a = h.table(name)
+        b = o.d(name, n).ravel() if n else o.d(name).ravel()
+        assert bits_equal(a, b), name
+
+
+@pytest.mark.parametrize("ng", NGS + [256])
+def test_fft_bitwise(ng, hyd_cache):
+    from oracle.hyo import Oracle
+    o = Oracle(ng)
+    h = hyd_cache(ng)
+    rng = np.random.default_rng(ng)
+    f = rng.standard_normal((ng, ng))
+    assert bits_equal(h.ptospc(f), o.ptospc(f))
+    s = rng.standard_normal((ng, ng))
+    assert bits_equal(h.spctop(s), o.spctop(s))
+
+
+@pytest.mark.parametrize("ng", NGS)
+def test_invert_and_gradient_bitwise(ng, hyd_cache):
+    from oracle.hyo import Oracle
+    o = Oracle(ng)
+    h = hyd_cache(ng)
+    rng = np.random.default_rng(7 + ng)
+    s = rng.standard_normal((ng, ng))
+    for a, b in zip(h.main_invert(s), o.main_invert(s)):
+        assert bits_equal(a, b)
+    for a, b in zip(h.gradient(s), o.gradient(s)):
+        assert bits_equal(a, b)
+
+
+@pytest.mark.parametrize("ng", NGS)
+def test_velint_bitwise(ng, hyd_cache):
+    o, qq, c = oracle_state(ng)
+    h = hyd_cache(ng)
+    rng = np.random.default_rng(3)
+    uu, vv = rng.standard_normal((2, ng, ng))
+    a = h.velint(uu, vv, c["x"], c["y"])
+    b = o.velint(uu, vv)
+    assert bits_equal(a[0], b[0]) and bits_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("ng", NGS)
+def test_con2grid_bitwise(ng, hyd_cache):
+    o, qq, c = oracle_state(ng)
+    h = hyd_cache(ng)
+    qc_h, cnt_h = h.con2grid(c["x"], c["y"], c["nextq"], c["qjump"], with_counts=True)
+    qc_o, cnt_o = o.con2grid(with_counts=True)
+    assert np.array_equal(cnt_h.ravel(), cnt_o.ravel())
+    assert bits_equal(qc_h, qc_o)
+
+
+def test_sum_sequential_bitwise(hyd_cache):
+    h = hyd_cache(64)
+    rng = np.random.default_rng(11)
+    cases = [rng.standard_normal(4096 * 4) * 10 + 3, rng.standard_normal(4096 * 4),
+             np.round(rng.standard_normal(12345) * 64) / 64 + 2.0 ** -30,
+             rng.integers(-5, 6, 9999) * 0.1, rng.standard_normal(777) ** 2,
+             np.full(5000, 0.1), np.array([1.0, 1e-16, 1e-16, 1e-16, -1.0])]
+    for a in cases:
+        ref = 0.0
+        for x in a:
+            ref = ref + float(x)
+        got = h.sum_sequential(a)
+        assert np.float64(got).view(np.int64) == np.float64(ref).view(np.int64), (len(a), got, ref)
+
+
+@pytest.mark.parametrize("ng", [32, 64])
+def test_advance_bitwise(ng):
+    """Several RK4 steps of the device-resident state, every field and node
+    position bit for bit."""
+    from hydra_amd import Hydra
+    from oracle.hyo import Oracle
+    from hydra_amd.ic import random_pv
+    qq = random_pv(ng, seed=12345)
+    o = Oracle(ng)
+    o.init_from_field(qq)
+    o.recontour(o.d("qr").copy())
+    c = o.contours()
+    o.evolution_init()
+    h = Hydra(ng)
+    h.set_field(qq)
+    assert h.scalar("qjump") == o.getd("qjump")
+    h.set_contours(c)
+    h.evolution_init()
+    for name in ("qc", "qd", "qs"):
+        assert bits_equal(h.field(name), o.d(name)), name
+    for step in range(4):
+        o.advance()
+        h.advance()
+        for name in ("qs", "qd", "qq", "uu", "vv", "pp"):
+            assert bits_equal(h.field(name), o.d(name)), (step, name)
+        cg = h.contours()
+        n = o.get("nptq")
+        assert bits_equal(cg["x"], o.d("xq", n)) and bits_equal(cg["y"], o.d("yq", n)), step
+        assert h.scalar("t") == o.getd("t") and h.scalar("twist") == o.getd("twist")
+    h.close()
