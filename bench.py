@@ -1,0 +1,184 @@
+"""Contour node-updates/s of the plain FP64 contour-advection step on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--ng 1024] [--gpus 1]
+
+Workload: the doubly periodic 2D Euler CAPS step at ng=1024 (contour grid
+16ng for recontouring, 4ng inside con2grid), a seeded random PV field with
+1D spectrum ~ k^3 exp(-2 (k/6)^2) and rms 4*pi, contoured into 80 levels
+on the GPU (congen), then K RK4 steps (evolution.f90 advance) of the
+device-resident state.  One node-update = one contour node advanced through
+one full RK4 step (four stages: con2grid, FFT inversion, sources, velint,
+node and spectral updates).  Inputs are resident in HBM before timing.
+
+Besides the JSON line the script prints nothing on stdout.  The roofline
+entry is for the kernel with the largest share of the timed region, timed
+with HIP events on the library's stream; cpu_baseline is the oracle (C
+restatement of the reference, one core) on a bounded sample of the same
+state.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(name, N, P):
+    """Algorithmic HBM bytes per launch of a kernel group (logical fields read
+    and written once, at their storage precision; fine-grid and FFT-internal
+    passes excluded)."""
+    s = 8
+    return {
+        # one 1D batch pass of a 2D transform: read and write one field
+        "fft_ptospc": 2 * 2 * s * N,          # two passes per ptospc call
+        "fft_spctop": 2 * 2 * s * N,
+        # con2grid: read node x,y,next (20 B/node), write qc
+        "con2grid": 20 * P + s * N,
+        "velint": 32 * P,
+        "combine_qq": 5 * s * N,
+        "advect_prod": 5 * s * N,
+        "source_finish": 5 * s * N,
+        "seqsum": s * N,
+        "adapt_reduce": 4 * s * N,
+    }.get(name)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ng", type=int, default=1024)
+    ap.add_argument("--seed", type=int, default=12345)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs one flow on one GPU; multi-GPU runs are not part of it "
+              "(use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from hydra_amd import Hydra
+    from hydra_amd.ic import random_pv
+
+    ng = args.ng
+    qq = random_pv(ng, seed=args.seed)
+    h = Hydra(ng, ncontq=80)
+    t0 = time.perf_counter()
+    h.set_field(qq)
+    h.recontour()
+    h.evolution_init()
+    h.sync()
+    t_setup = time.perf_counter() - t0
+    P, nq = h.counts()
+    log(f"setup: ng={ng} contours={nq} nodes={P} recontour+init {t_setup:.2f}s")
+    N = ng * ng
+
+    for _ in range(args.warmup):
+        h.advance()
+    h.sync()
+    # timed region: K RK4 steps; kernel events recorded on the library stream
+    h.set_timing(True)
+    h.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h.advance()
+    h.sync()
+    wall = time.perf_counter() - t0
+    recs = h.kernel_times()
+    h.set_timing(False)
+    P_end, _ = h.counts()
+    ms_step = 1e3 * wall / args.steps
+    value = P * args.steps / wall
+
+    # per-kernel-group totals over the timed region
+    tot = {}
+    cnt = {}
+    for name, ms, by in recs:
+        tot[name] = tot.get(name, 0.0) + ms
+        cnt[name] = cnt.get(name, 0) + 1
+    groups = sorted(tot.items(), key=lambda kv: -kv[1])
+    for name, ms in groups:
+        log(f"  {name:16s} {ms / args.steps:8.3f} ms/step  {cnt[name] // args.steps} calls/step")
+    dom = None
+    for name, ms in groups:
+        if algorithmic_bytes(name, N, P) is not None:
+            dom = name
+            break
+    avg_ms = tot[dom] / cnt[dom]
+    alg = algorithmic_bytes(dom, N, P)
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_launch_ms": round(avg_ms, 4),
+                "algorithmic_bytes_per_launch": int(alg)}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(h, qq, ng, args.cpu_steps)
+        except Exception as e:  # the baseline is a report, not the product
+            log(f"cpu baseline failed: {e}")
+    h.close()
+    out = {
+        "metric": "contour node-updates/s at ng=1024: plain FP64 vs FP32/batched at matched error",
+        "value": round(value, 1),
+        "unit": "node-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain FP64 CAPS RK4 step, 2D Euler, ng={ng}, 80 PV levels, "
+                               f"seeded k^3 exp(-2(k/6)^2) field (rms 4pi)",
+                   "ng": ng, "nodes": P, "contours": nq, "precision": "plain-fp64",
+                   "dt": "adaptive (reference adapt)", "seed": args.seed},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(h, qq, ng, nsteps):
+    """The oracle (C restatement, one core) advancing the same state for a
+    bounded number of steps; returns node-updates/s."""
+    import subprocess
+    from oracle.hyo import Oracle, build
+    if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
+        build()
+    c = h.contours()
+    o = Oracle(ng, npm=len(c["x"]) * 4 + 65536)
+    o.init_from_field(qq)
+    o.set_contours(c)
+    o.evolution_init()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        o.advance()
+    wall = time.perf_counter() - t0
+    P = len(c["x"])
+    return {"value": round(P * nsteps / wall, 1), "unit": "node-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{nsteps} RK4 step(s) of the oracle (C restatement of evolution.f90 "
+                      f"advance, gcc -O2) from the same ng={ng} state with {P} nodes, "
+                      f"{wall:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

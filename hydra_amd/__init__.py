@@ -84,8 +84,11 @@ def lib():
         L.hyd_put_field.argtypes = [C.c_void_p, C.c_int32, _D]
         L.hyd_get_scalar.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         L.hyd_set_scalar.argtypes = [C.c_void_p, C.c_int32, C.c_double]
-        for fn in ("hyd_evolution_init", "hyd_inversion", "hyd_advance", "hyd_sync"):
+        for fn in ("hyd_evolution_init", "hyd_inversion", "hyd_advance", "hyd_sync",
+                   "hyd_prepare", "hyd_recontour"):
             getattr(L, fn).argtypes = [C.c_void_p]
+        L.hyd_renode.argtypes = [C.c_void_p, _D, _D, C.c_int32, _D, _D, C.c_int32,
+                                 C.POINTER(C.c_int32)]
         L.hyd_set_timing.argtypes = [C.c_void_p, C.c_int32]
         L.hyd_kernel_times.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
                                        np.ctypeslib.ndpointer(dtype=np.float64),
@@ -184,6 +187,23 @@ class Hydra:
         _check(self.L.hyd_con2grid(self.h, xq, yq, nextq, len(xq), float(qjump), qc,
                                    cnt.ctypes.data_as(C.c_void_p) if with_counts else None))
         return (qc, cnt) if with_counts else qc
+
+    def renode(self, xd, yd, cap=None):
+        """contours.f90:687-933: returns (xr, yr); empty when the contour is dropped."""
+        xd, yd = _f64(xd), _f64(yd)
+        cap = cap or (4 * len(xd) + 4096)
+        xr, yr = np.empty(cap), np.empty(cap)
+        npr = C.c_int32(0)
+        _check(self.L.hyd_renode(self.h, xd, yd, len(xd), xr, yr, cap, C.byref(npr)))
+        return xr[: npr.value].copy(), yr[: npr.value].copy()
+
+    def recontour(self):
+        """congen.f90:24-128 on the device state (residual qr)."""
+        _check(self.L.hyd_recontour(self.h))
+
+    def prepare(self):
+        """evolution.f90:135-162."""
+        _check(self.L.hyd_prepare(self.h))
 
     def sum_sequential(self, a):
         a = _f64(a).ravel()

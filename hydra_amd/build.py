@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libhydra_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HYDRA_AMD_ARCH", "gfx950")
 
-SOURCES = ["tables.cpp", "fft.hip", "con2grid.hip", "seqsum.hip", "nodes.hip", "step.hip",
+SOURCES = ["tables.cpp", "fft.hip", "con2grid.hip", "seqsum.hip", "nodes.hip", "step.hip", "renode.hip", "congen.hip",
            "capi.hip"]
 
 FLAGS = [

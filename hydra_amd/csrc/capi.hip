@@ -18,6 +18,7 @@ int inversion(Ctx &c);
 int adapt(Ctx &c);
 int advance(Ctx &c);
 int evolution_init(Ctx &c);
+int prepare(Ctx &c);
 
 KTimer::KTimer(Ctx &ctx, const char *name, int64_t bytes) : c(ctx) {
   if (!c.timing) return;
@@ -97,6 +98,7 @@ static void free_ctx(Ctx *c) {
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->hsc) hipHostFree(c->hsc);
+  free_congen(*c);
   for (auto &r : c->krec) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
@@ -199,7 +201,7 @@ static int create(const hyd_params *p, hyd_ctx **out) {
       return HYD_ENOMEM;
     }
     c->seq_work = w;
-    c->seq_cap = seqsum_work_bytes(4 * N) / 56;
+    c->seq_cap = seqsum_work_chunks(4 * N);
   }
   if (hipHostMalloc((void **)&c->hsc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
     set_error("hipHostMalloc failed");
@@ -541,6 +543,59 @@ int hyd_evolution_init(hyd_ctx *h) {
   }
   HYD_CHECK(evolution_init(c));
   HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+int hyd_prepare(hyd_ctx *h) {
+  Ctx &c = *C(h);
+  HYD_CHECK(prepare(c));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+int hyd_recontour(hyd_ctx *h) {
+  Ctx &c = *C(h);
+  if (!c.have_field) {
+    set_error("recontour before set_field");
+    return HYD_ESTATE;
+  }
+  HYD_CHECK(recontour(c, c.qr));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+int hyd_renode(hyd_ctx *h, const double *xd, const double *yd, int32_t npd, double *xr,
+               double *yr, int32_t cap_out, int32_t *npr) {
+  Ctx &c = *C(h);
+  *npr = 0;
+  if (npd <= 0) return HYD_OK;
+  if (npd > c.cap) {
+    set_error("renode: node count exceeds capacity");
+    return HYD_ECAPACITY;
+  }
+  double *dxd, *dyd, *dxr, *dyr;
+  int64_t *doff_in, *doff_out;
+  int32_t *dnp, *dnpr;
+  HYD_CHECK(dbuf(c, "api_xd", npd, &dxd));
+  HYD_CHECK(dbuf(c, "api_yd", npd, &dyd));
+  HYD_CHECK(dbuf(c, "api_off", 2, &doff_in));
+  HYD_CHECK(dbuf(c, "api_offo", 2, &doff_out));
+  HYD_CHECK(dbuf(c, "api_np", 2, &dnp));
+  HYD_CHECK(dbuf(c, "api_npr", 2, &dnpr));
+  const int64_t cap = std::max<int32_t>(cap_out, 1);
+  HYD_CHECK(dbuf(c, "api_xr", cap, &dxr));
+  HYD_CHECK(dbuf(c, "api_yr", cap, &dyr));
+  const int64_t zero = 0;
+  HYD_HIP(hipMemcpyAsync(dxd, xd, 8 * (size_t)npd, hipMemcpyHostToDevice, c.s));
+  HYD_HIP(hipMemcpyAsync(dyd, yd, 8 * (size_t)npd, hipMemcpyHostToDevice, c.s));
+  HYD_HIP(hipMemcpyAsync(doff_in, &zero, 8, hipMemcpyHostToDevice, c.s));
+  HYD_HIP(hipMemcpyAsync(dnp, &npd, 4, hipMemcpyHostToDevice, c.s));
+  int64_t ntot = 0;
+  HYD_CHECK(renode_batch(c, dxd, dyd, doff_in, dnp, 1, npd, dxr, dyr, doff_out, dnpr, &ntot, cap));
+  HYD_HIP(hipMemcpyAsync(xr, dxr, 8 * (size_t)ntot, hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipMemcpyAsync(yr, dyr, 8 * (size_t)ntot, hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  *npr = (int32_t)ntot;
   return HYD_OK;
 }
 

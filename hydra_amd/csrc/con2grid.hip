@@ -30,6 +30,7 @@ struct C2GArgs {
   const double *xq, *yq;
   const int32_t *nextq;
   int n;
+  int ultra;         // 0: con2grid conventions, 1: con2ugrid (congen.f90:353-416)
   int nxf, nyf;
   double ellx, elly, hlx, hly, hlxi, hlyi, xmin, ymin, glxfi, glyfi, small, qjump;
   const double *xgf;
@@ -60,8 +61,10 @@ __global__ void c2g_nodes(C2GArgs a) {
   const double xi = a.xq[i], yi = a.yq[i];
   const int ia = a.nextq[i] - 1;
   const double xa = a.xq[ia], ya = a.yq[ia];
-  const int ixci = 1 + (int)(a.glxfi * (xi - a.xmin));
-  const int ixca = 1 + (int)(a.glxfi * (xa - a.xmin));
+  // fine cell of each node: contours.f90:207 (1-based) / congen.f90:354 (0-based)
+  const int base = a.ultra ? 0 : 1;
+  const int ixci = base + (int)(a.glxfi * (xi - a.xmin));
+  const int ixca = base + (int)(a.glxfi * (xa - a.xmin));
   const double xx = xa - xi;
   const double dx = xx - a.ellx * (double)(int)(xx * a.hlxi);
   const double yy = ya - yi;
@@ -83,8 +86,9 @@ __global__ void c2g_nodes(C2GArgs a) {
   }
   if (nxc != 0) {
     const int jump = nxc >= 0 ? 1 : -1;
-    const int ixbeg = ixci + (jump - 1) / 2 + a.nxf;
-    const int sgn = copysign(1.0, dx) > 0.0 ? 1 : -1;
+    // contours.f90:251-253 vs congen.f90:398-400
+    const int ixbeg = a.ultra ? ixci + (1 + jump) / 2 + a.nxf : ixci + (jump - 1) / 2 + a.nxf;
+    const int sgn = a.ultra ? jump : (copysign(1.0, dx) > 0.0 ? 1 : -1);
     for (int ncr = 0; ncr != nxc; ncr += jump) {
       const int ix = 1 + (ixbeg + ncr) % a.nxf;
       const double x2 = a.xgf[ix - 1] - xi;
@@ -126,8 +130,8 @@ __global__ void c2g_replay_collect(C2GArgs a) {
   }
   if (a.flags[0] != 0 && nxc != 0) {
     const int jump = nxc >= 0 ? 1 : -1;
-    const int ixbeg = ixci + (jump - 1) / 2 + a.nxf;
-    const int sgn = copysign(1.0, dx) > 0.0 ? 1 : -1;
+    const int ixbeg = a.ultra ? ixci + (1 + jump) / 2 + a.nxf : ixci + (jump - 1) / 2 + a.nxf;
+    const int sgn = a.ultra ? jump : (copysign(1.0, dx) > 0.0 ? 1 : -1);
     for (int ncr = 0; ncr != nxc; ncr += jump) {
       const int ix = 1 + (ixbeg + ncr) % a.nxf;
       const double x2 = a.xgf[ix - 1] - xi;
@@ -299,6 +303,63 @@ __global__ void fill_zero(double *a, int64_t n) {
   if (k < n) a[k] = 0.0;
 }
 
+// Crossing accumulation and y sweep of the fine (or ultra-fine) grid:
+// contours.f90:206-277 (ultra=0) / congen.f90:353-423 (ultra=1).  Leaves the
+// swept field in fq (column-major, nyf rows).
+int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n, int ultra,
+               int nxf, int nyf, double glxfi, double glyfi, const double *xg, int32_t *cnt,
+               int32_t *qjx, double *qbot, double *fq) {
+  const Consts &k = c.c;
+  const int64_t nfine = (int64_t)nxf * nyf;
+  HYD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * nfine, c.s));
+  HYD_HIP(hipMemsetAsync(qjx, 0, sizeof(int32_t) * nxf, c.s));
+  HYD_HIP(hipMemsetAsync(c.flags, 0, sizeof(int32_t) * 8, c.s));
+  C2GArgs a;
+  a.xq = xq;
+  a.yq = yq;
+  a.nextq = nextq;
+  a.n = n;
+  a.ultra = ultra;
+  a.nxf = nxf;
+  a.nyf = nyf;
+  a.ellx = k.ellx;
+  a.elly = k.elly;
+  a.hlx = k.hlx;
+  a.hly = k.hly;
+  a.hlxi = k.hlxi;
+  a.hlyi = k.hlyi;
+  a.xmin = k.xmin;
+  a.ymin = k.ymin;
+  a.glxfi = glxfi;
+  a.glyfi = glyfi;
+  a.small = k.small;
+  a.qjump = c.hsc->qjump;
+  a.xgf = xg;
+  a.cnt = cnt;
+  a.qjx = qjx;
+  a.flags = c.flags;
+  a.replay = c.replay;
+  a.replay_cap = c.replay_cap;
+  a.ixc = c.ixc;
+  a.nxc = c.nxc;
+  a.ndx = c.ndx;
+  a.ndy = c.ndy;
+  const int nb = (n + 255) / 256;
+  if (n > 0) {
+    hipLaunchKernelGGL(c2g_nodes, dim3(nb), dim3(256), 0, c.s, a);
+    HYD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(c2g_replay_collect, dim3(nb), dim3(256), 0, c.s, a);
+  }
+  hipLaunchKernelGGL(c2g_replay_resolve, dim3(1), dim3(64), 0, c.s, c.flags, c.replay,
+                     c.replay_val, c.replay_cap, a.qjump);
+  hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(64), 0, c.s, qjx, qbot, nxf, nfine, a.qjump,
+                     c.flags, c.replay, c.replay_val);
+  hipLaunchKernelGGL(c2g_colscan, dim3((nxf + 3) / 4), dim3(256), 0, c.s, cnt, qbot, fq, nxf,
+                     nyf, a.qjump, c.flags, c.replay, c.replay_val);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
 int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n,
              double *qc, int32_t *counts_out) {
   const Consts &k = c.c;
@@ -314,48 +375,8 @@ int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, i
   }
   const int64_t nfine = (int64_t)k.nxf * k.nyf;
   KTimer tm(c, "con2grid", (int64_t)n * 24 + 8 * N);
-  HYD_HIP(hipMemsetAsync(c.cnt, 0, sizeof(int32_t) * nfine, c.s));
-  HYD_HIP(hipMemsetAsync(c.qjx_cnt, 0, sizeof(int32_t) * k.nxf, c.s));
-  HYD_HIP(hipMemsetAsync(c.flags, 0, sizeof(int32_t) * 8, c.s));
-  C2GArgs a;
-  a.xq = xq;
-  a.yq = yq;
-  a.nextq = nextq;
-  a.n = n;
-  a.nxf = k.nxf;
-  a.nyf = k.nyf;
-  a.ellx = k.ellx;
-  a.elly = k.elly;
-  a.hlx = k.hlx;
-  a.hly = k.hly;
-  a.hlxi = k.hlxi;
-  a.hlyi = k.hlyi;
-  a.xmin = k.xmin;
-  a.ymin = k.ymin;
-  a.glxfi = k.glxfi;
-  a.glyfi = k.glyfi;
-  a.small = k.small;
-  a.qjump = c.hsc->qjump;
-  a.xgf = c.xgf;
-  a.cnt = c.cnt;
-  a.qjx = c.qjx_cnt;
-  a.flags = c.flags;
-  a.replay = c.replay;
-  a.replay_cap = c.replay_cap;
-  a.ixc = c.ixc;
-  a.nxc = c.nxc;
-  a.ndx = c.ndx;
-  a.ndy = c.ndy;
-  const int nb = (n + 255) / 256;
-  hipLaunchKernelGGL(c2g_nodes, dim3(nb), dim3(256), 0, c.s, a);
-  HYD_HIP(hipGetLastError());
-  hipLaunchKernelGGL(c2g_replay_collect, dim3(nb), dim3(256), 0, c.s, a);
-  hipLaunchKernelGGL(c2g_replay_resolve, dim3(1), dim3(64), 0, c.s, c.flags, c.replay,
-                     c.replay_val, c.replay_cap, a.qjump);
-  hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(64), 0, c.s, c.qjx_cnt, c.qbot, k.nxf, nfine,
-                     a.qjump, c.flags, c.replay, c.replay_val);
-  hipLaunchKernelGGL(c2g_colscan, dim3((k.nxf + 3) / 4), dim3(256), 0, c.s, c.cnt, c.qbot,
-                     c.fq, k.nxf, k.nyf, a.qjump, c.flags, c.replay, c.replay_val);
+  HYD_CHECK(fine_sweep(c, xq, yq, nextq, n, 0, k.nxf, k.nyf, k.glxfi, k.glyfi, c.xgf, c.cnt,
+                       c.qjx_cnt, c.qbot, c.fq));
   // 4ng -> 2ng -> ng
   const int nh1 = k.nxf / 2, nh2 = k.nx;
   hipLaunchKernelGGL(c2g_avg9, dim3((nh1 + 255) / 256, nh1), dim3(256), 0, c.s, c.fq, c.mq, nh1);
@@ -369,9 +390,7 @@ int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, i
   if (counts_out) {
     // integer level counts of the swept fine field (diagnostic path)
     std::vector<double> f(nfine);
-    int32_t fl[8];
     HYD_HIP(hipMemcpyAsync(f.data(), c.fq, sizeof(double) * nfine, hipMemcpyDeviceToHost, c.s));
-    HYD_HIP(hipMemcpyAsync(fl, c.flags, sizeof(fl), hipMemcpyDeviceToHost, c.s));
     HYD_HIP(hipStreamSynchronize(c.s));
     const double q = c.hsc->qjump;
     for (int64_t t = 0; t < nfine; t++) counts_out[t] = (int32_t)llround(f[t] / q);

@@ -12,6 +12,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
+#include <map>
 #include <string>
 #include <vector>
 
@@ -72,6 +75,12 @@ struct FftPlan {
   double fnorm, rnorm;  // 1/sqrt(n) and 2/sqrt(n)
 };
 
+// Named, grow-only device work buffers (recontouring, renode).
+struct DBuf {
+  void *p = nullptr;
+  int64_t cap = 0;  // bytes
+};
+
 // Device context.
 struct Scalars {  // lives in device memory, mirrored on the host
   double t, dt, dt2, dt3, dt6, twist, zzrms, zzmax, uumax, qjump, dtmax, dtacc;
@@ -125,6 +134,8 @@ struct Ctx {
   std::vector<double> exp_host;  // host-evaluated exp table of the step
   void *seq_work = nullptr; // chunk table of seqsum
   int64_t seq_cap = 0;      // chunks it holds
+  // work buffers of recontour / renode, allocated on first use
+  std::map<std::string, DBuf> cg_bufs;
   // timing
   bool timing = false;
   struct KRec { std::string name; hipEvent_t a, b; int64_t bytes; };
@@ -147,11 +158,36 @@ int launch_yderiv(Ctx &c, const double *var, double *der);
 
 int seqsum(Ctx &c, const double *a, int64_t n, double *dout);
 int64_t seqsum_work_bytes(int64_t n);
+int64_t seqsum_work_chunks(int64_t n);
 
 int velint(Ctx &c, const double *uu, const double *vv, const double *xq, const double *yq,
            int n, double *uq, double *vq);
 int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n,
              double *qc, int32_t *counts_out);
+
+template <class T>
+int dbuf(Ctx &c, const char *name, int64_t n, T **out) {
+  DBuf &b = c.cg_bufs[name];
+  if (b.cap < n * (int64_t)sizeof(T) || !b.p) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    const int64_t bytes = std::max<int64_t>(n * (int64_t)sizeof(T), 256);
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+      set_error(std::string("out of device memory for work buffer ") + name);
+      return HYD_ENOMEM;
+    }
+    b.cap = bytes;
+  }
+  *out = (T *)b.p;
+  return HYD_OK;
+}
+
+int recontour(Ctx &c, const double *qq);
+void free_congen(Ctx &c);
+int renode_batch(Ctx &c, const double *xd, const double *yd, const int64_t *off_in,
+                 const int32_t *npd, int ncont, int64_t nin, double *xr, double *yr,
+                 int64_t *off_out, int32_t *npr, int64_t *ntot_out, int64_t cap_out);
 
 // timing helpers
 struct KTimer {

@@ -220,8 +220,8 @@ int seqsum(Ctx &c, const double *a, int64_t n, double *dout) {
   return HYD_OK;
 }
 
-int64_t seqsum_work_bytes(int64_t n) {
-  return (int64_t)sizeof(ChunkInfo) * ((n + kChunk - 1) / kChunk + 64);
-}
+int64_t seqsum_work_chunks(int64_t n) { return (n + kChunk - 1) / kChunk + 64; }
+
+int64_t seqsum_work_bytes(int64_t n) { return (int64_t)sizeof(ChunkInfo) * seqsum_work_chunks(n); }
 
 }  // namespace hyd

@@ -362,6 +362,26 @@ int advance(Ctx &c) {
   return HYD_OK;
 }
 
+// qs = flo*(qs-qc)+qc+qd ; qd = qs-qc  (evolution.f90:155-156)
+PW_KERNEL(k_prepare, (double *__restrict__ qs, double *__restrict__ qd,
+                      const double *__restrict__ qc, const double *__restrict__ flo, int64_t n),
+          {
+            const double s = flo[k] * (qs[k] - qc[k]) + qc[k] + qd[k];
+            qs[k] = s;
+            qd[k] = s - qc[k];
+          })
+
+// evolution.f90:135-162 (prepare): qr = spctop(qs - qc) for recontouring.
+int prepare(Ctx &c) {
+  const int64_t N = c.N;
+  HYD_CHECK(con2grid(c, c.xq, c.yq, c.nextq, c.nptq, c.wa, nullptr));
+  HYD_CHECK(fft_ptospc(c, c.wa, c.qc));
+  hipLaunchKernelGGL(k_prepare, grid1(N), dim3(256), 0, c.s, c.qs, c.qd, c.qc, c.flo, N);
+  HYD_HIP(hipGetLastError());
+  HYD_CHECK(fft_spctop(c, c.qd, c.qr, PRE_NONE, 1.0, c.wc));
+  return HYD_OK;
+}
+
 // evolution.f90:96-131 (init): qc from the contours and qd = fhi*(qs-qc).
 int evolution_init(Ctx &c) {
   const int64_t N = c.N;

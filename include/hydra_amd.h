@@ -97,6 +97,17 @@ int hyd_velint(hyd_ctx *ctx, const double *uu, const double *vv, const double *x
 int hyd_con2grid(hyd_ctx *ctx, const double *xq, const double *yq, const int32_t *nextq,
                  int32_t nptq, double qjump, double *qc, int32_t *counts);
 
+/* renode(xd,yd,npd,xr,yr,npr) contours.f90:687-933.  xr/yr must hold
+   cap_out nodes; *npr = 0 means the contour is too small and is deleted. */
+int hyd_renode(hyd_ctx *ctx, const double *xd, const double *yd, int32_t npd, double *xr,
+               double *yr, int32_t cap_out, int32_t *npr);
+
+/* ---- ca/plane/qg/caps/congen.f90 ------------------------------------------ */
+/* recontour(qr) congen.f90:24-128 on the device state: the contours and the
+   residual qr (the full PV when no contours exist yet) are turned into new
+   contours on the 16ng grid; replaces the device contour store. */
+int hyd_recontour(hyd_ctx *ctx);
+
 /* ---- exact sequential reduction ------------------------------------------ */
 /* The value gfortran's SUM(a) (and the explicit loops of contours.f90:330-336)
    produce: ((a1 + a2) + a3) + ... in storage order, bit for bit.  Used for
@@ -123,6 +134,9 @@ int hyd_set_scalar(hyd_ctx *ctx, int32_t id, double v);
 
 /* init (evolution.f90:96-131): qc from the contours, qd = fhi*(qs-qc). */
 int hyd_evolution_init(hyd_ctx *ctx);
+/* prepare (evolution.f90:135-162): fold the contour PV into qs and leave the
+   residual qr = spctop(qs-qc) for recontouring. */
+int hyd_prepare(hyd_ctx *ctx);
 /* inversion (evolution.f90:494-520) on the device state. */
 int hyd_inversion(hyd_ctx *ctx);
 /* advance (evolution.f90:166-326): one RK4 step of nodes and fields. */

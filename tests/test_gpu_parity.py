@@ -128,3 +128,84 @@ def test_advance_bitwise(ng):
         assert bits_equal(cg["x"], o.d("xq", n)) and bits_equal(cg["y"], o.d("yq", n)), step
         assert h.scalar("t") == o.getd("t") and h.scalar("twist") == o.getd("twist")
     h.close()
+
+
+@pytest.mark.parametrize("ng", NGS)
+def test_renode_bitwise(ng, hyd_cache):
+    o, qq, c = oracle_state(ng)
+    h = hyd_cache(ng)
+    rng = np.random.default_rng(ng)
+    for j in list(range(4)) + list(rng.integers(0, len(c["i1q"]), 6)):
+        i1, i2 = c["i1q"][j], c["i2q"][j]
+        # perturb so that the contour is not already renoded
+        xd = c["x"][i1 - 1:i2] + 1e-3 * np.sin(7 * c["y"][i1 - 1:i2])
+        yd = c["y"][i1 - 1:i2]
+        a = h.renode(xd, yd)
+        b = o.renode(xd, yd)
+        assert bits_equal(a[0], b[0]) and bits_equal(a[1], b[1]), j
+    # a triangle: several corners
+    verts = np.array([[-0.6, -0.4], [0.7, -0.3], [0.0, 0.6]])
+    pts = np.concatenate([verts[a] + np.arange(40)[:, None] / 40.0 * (verts[(a + 1) % 3] - verts[a])
+                          for a in range(3)])
+    a = h.renode(pts[:, 0], pts[:, 1])
+    b = o.renode(pts[:, 0], pts[:, 1])
+    assert bits_equal(a[0], b[0]) and bits_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("ng", NGS)
+def test_recontour_from_field_bitwise(ng):
+    """congen from the initial field: contour count, node count, topology and
+    positions identical to the oracle."""
+    from hydra_amd import Hydra
+    from hydra_amd.ic import random_pv
+    from oracle.hyo import Oracle
+    qq = random_pv(ng, seed=12345)
+    o = Oracle(ng)
+    o.init_from_field(qq)
+    o.recontour(o.d("qr").copy())
+    c = o.contours()
+    h = Hydra(ng)
+    h.set_field(qq)
+    h.recontour()
+    g = h.contours()
+    assert len(g["x"]) == len(c["x"]) and len(g["i1q"]) == len(c["i1q"])
+    for k in ("nextq", "i1q", "i2q", "npq", "indq"):
+        assert np.array_equal(g[k], c[k]), k
+    assert bits_equal(g["x"], c["x"]) and bits_equal(g["y"], c["y"])
+    h.close()
+
+
+def test_recontour_cycle_bitwise():
+    """prepare + recontour with existing contours (the con2ugrid branch)
+    after a few steps, then evolution_init: identical to the oracle."""
+    from hydra_amd import Hydra
+    from hydra_amd.ic import random_pv
+    from oracle.hyo import Oracle
+    ng = 32
+    qq = random_pv(ng, seed=7)
+    o = Oracle(ng)
+    o.init_from_field(qq)
+    o.recontour(o.d("qr").copy())
+    c = o.contours()
+    o.evolution_init()
+    h = Hydra(ng)
+    h.set_field(qq)
+    h.set_contours(c)
+    h.evolution_init()
+    for _ in range(3):
+        o.advance()
+        h.advance()
+    o.L.hyo_prepare(o.m)
+    h.prepare()
+    assert bits_equal(h.field("qr"), o.d("qr"))
+    o.recontour(o.d("qr").copy())
+    h.recontour()
+    c2, g2 = o.contours(), h.contours()
+    for k in ("nextq", "i1q", "i2q", "npq", "indq"):
+        assert np.array_equal(g2[k], c2[k]), k
+    assert bits_equal(g2["x"], c2["x"]) and bits_equal(g2["y"], c2["y"])
+    o.evolution_init()
+    h.evolution_init()
+    for name in ("qc", "qd", "qs"):
+        assert bits_equal(h.field(name), o.d(name)), name
+    h.close()
