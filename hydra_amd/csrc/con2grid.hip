@@ -219,27 +219,46 @@ __device__ __forceinline__ double cell_value(int32_t w, int64_t key, double qjum
 
 // qbot (contours.f90:234-238): one wavefront, chain over the non-empty
 // bottom-row counters.
-__global__ void c2g_qbot(const int32_t *qjx, double *qbot, int nxf, int64_t nkey, double qjump,
-                         const int32_t *flags, const int32_t *replay, const double *vals) {
-  const int l = threadIdx.x;
+// The counters and their values are staged in LDS by the whole block; wave 0
+// runs the chain.  nxf <= kQbotMax.
+static constexpr int kQbotMax = 32768;
+__global__ void __launch_bounds__(1024) c2g_qbot(const int32_t *qjx, double *qbot, int nxf,
+                                                 int64_t nkey, double qjump, const int32_t *flags,
+                                                 const int32_t *replay, const double *vals) {
+  __shared__ double cvs[kQbotMax / 4];   // values of one quarter at a time
+  __shared__ uint8_t nz[kQbotMax / 4];
   const int nrep = flags[4];
+  const int q4 = kQbotMax / 4;
   double v = 0.0;  // qbot(1)
-  if (l == 0) qbot[0] = 0.0;
+  if (threadIdx.x == 0) qbot[0] = 0.0;
   // qbot(ix+1) = qbot(ix) + qjx(ix), ix = 1..nxf-1
-  for (int base = 0; base < nxf - 1; base += 64) {
-    const int ix0 = base + l;  // 0-based ix-1
-    const bool ok = ix0 < nxf - 1;
-    const int32_t w = ok ? qjx[ix0] : 0;
-    const double cv = w ? cell_value(w, nkey + ix0, qjump, replay, vals, nrep) : 0.0;
-    uint64_t mask = __ballot(w != 0);
-    double mine = v;
-    while (mask) {
-      const int b = __ffsll((unsigned long long)mask) - 1;
-      mask &= mask - 1;
-      v = __shfl(cv, b) + v;
-      if (l >= b) mine = v;
+  for (int part = 0; part < nxf - 1; part += q4) {
+    const int m = min(q4, nxf - 1 - part);
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+      const int32_t w = qjx[part + t];
+      nz[t] = w != 0;
+      cvs[t] = w ? cell_value(w, nkey + part + t, qjump, replay, vals, nrep) : 0.0;
     }
-    if (ok) qbot[ix0 + 1] = mine;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int l = threadIdx.x;
+      for (int base = 0; base < m; base += 64) {
+        const int t = base + l;
+        const bool ok = t < m;
+        const bool w = ok && nz[t];
+        const double cv = w ? cvs[t] : 0.0;
+        uint64_t mask = __ballot(w);
+        double mine = v;
+        while (mask) {
+          const int b = __ffsll((unsigned long long)mask) - 1;
+          mask &= mask - 1;
+          v = __shfl(cv, b) + v;
+          if (l >= b) mine = v;
+        }
+        if (ok) qbot[part + t + 1] = mine;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -255,20 +274,29 @@ __global__ void c2g_colscan(const int32_t *cnt, const double *qbot, double *fq, 
   const int32_t *cc = cnt + (int64_t)col * nyf;
   double *out = fq + (int64_t)col * nyf;
   double v = qbot[col];
-  for (int base = 0; base < nyf; base += 64) {
-    const int r = base + l;  // iy-1
-    const bool ok = r < nyf;
-    const int32_t w = (ok && r >= 1) ? cc[r] : 0;
-    const double cv = w ? cell_value(w, (int64_t)col * nyf + r, qjump, replay, vals, nrep) : 0.0;
-    uint64_t mask = __ballot(w != 0);
-    double mine = v;
-    while (mask) {
-      const int b = __ffsll((unsigned long long)mask) - 1;
-      mask &= mask - 1;
-      v = __shfl(cv, b) + v;
-      if (l >= b) mine = v;
+  constexpr int U = 8;  // segments of 64 rows loaded ahead
+  for (int base = 0; base < nyf; base += 64 * U) {
+    int32_t w[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int r = base + 64 * u + l;
+      w[u] = (r < nyf && r >= 1) ? cc[r] : 0;
     }
-    if (ok) out[r] = mine;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int r = base + 64 * u + l;  // iy-1
+      const double cv =
+          w[u] ? cell_value(w[u], (int64_t)col * nyf + r, qjump, replay, vals, nrep) : 0.0;
+      uint64_t mask = __ballot(w[u] != 0);
+      double mine = v;
+      while (mask) {
+        const int b = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1;
+        v = __shfl(cv, b) + v;
+        if (l >= b) mine = v;
+      }
+      if (r < nyf) out[r] = mine;
+    }
   }
 }
 
@@ -352,7 +380,7 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
   }
   hipLaunchKernelGGL(c2g_replay_resolve, dim3(1), dim3(64), 0, c.s, c.flags, c.replay,
                      c.replay_val, c.replay_cap, a.qjump);
-  hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(64), 0, c.s, qjx, qbot, nxf, nfine, a.qjump,
+  hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(1024), 0, c.s, qjx, qbot, nxf, nfine, a.qjump,
                      c.flags, c.replay, c.replay_val);
   hipLaunchKernelGGL(c2g_colscan, dim3((nxf + 3) / 4), dim3(256), 0, c.s, cnt, qbot, fq, nxf,
                      nyf, a.qjump, c.flags, c.replay, c.replay_val);

@@ -60,26 +60,38 @@ PW_KERNEL(k_qq_kd, (double *__restrict__ ss, const double *__restrict__ qq,
           { ss[k] = qq[k] + kdsq * pp[k]; })
 
 // zz = zz**2 and the maxima of zz**2 and uu**2+vv**2 (evolution.f90:552-556)
-__global__ void k_adapt_reduce(double *__restrict__ zz, const double *__restrict__ uu,
-                               const double *__restrict__ vv, int64_t n,
-                               unsigned long long *__restrict__ mx) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double z2 = 0.0, u2 = 0.0;
-  if (k < n) {
+__global__ void __launch_bounds__(256) k_adapt_reduce(double *__restrict__ zz,
+                                                      const double *__restrict__ uu,
+                                                      const double *__restrict__ vv, int64_t n,
+                                                      unsigned long long *__restrict__ mx) {
+  double z2m = 0.0, u2m = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x) {
     const double z = zz[k];
-    z2 = z * z;
+    const double z2 = z * z;
     zz[k] = z2;
     const double u = uu[k], v = vv[k];
-    u2 = u * u + v * v;
+    z2m = fmax(z2m, z2);
+    u2m = fmax(u2m, u * u + v * v);
   }
-  // non-negative doubles order like their bit patterns
+  // non-negative doubles order like their bit patterns: one atomic per block
   for (int off = 32; off > 0; off >>= 1) {
-    z2 = fmax(z2, __shfl_down(z2, off));
-    u2 = fmax(u2, __shfl_down(u2, off));
+    z2m = fmax(z2m, __shfl_down(z2m, off));
+    u2m = fmax(u2m, __shfl_down(u2m, off));
   }
+  __shared__ double sz[4], su[4];
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(&mx[0], (unsigned long long)__double_as_longlong(z2));
-    atomicMax(&mx[1], (unsigned long long)__double_as_longlong(u2));
+    sz[threadIdx.x >> 6] = z2m;
+    su[threadIdx.x >> 6] = u2m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      z2m = fmax(z2m, sz[w]);
+      u2m = fmax(u2m, su[w]);
+    }
+    atomicMax(&mx[0], (unsigned long long)__double_as_longlong(z2m));
+    atomicMax(&mx[1], (unsigned long long)__double_as_longlong(u2m));
   }
 }
 
@@ -259,7 +271,9 @@ int adapt(Ctx &c) {
   HYD_HIP(hipMemsetAsync(mx, 0, 2 * sizeof(unsigned long long), c.s));
   {
     KTimer t(c, "adapt_reduce", 32 * N);
-    hipLaunchKernelGGL(k_adapt_reduce, grid1(N), dim3(256), 0, c.s, zz, c.uu, c.vv, N, mx);
+    const int64_t nb = std::min<int64_t>((N + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_adapt_reduce, dim3((unsigned)nb), dim3(256), 0, c.s, zz, c.uu, c.vv, N,
+                       mx);
   }
   HYD_CHECK(seqsum(c, zz, N, c.dsum + 1));
   AdaptArgs a;
