@@ -85,10 +85,11 @@ def lib():
         L.hyd_get_scalar.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         L.hyd_set_scalar.argtypes = [C.c_void_p, C.c_int32, C.c_double]
         for fn in ("hyd_evolution_init", "hyd_inversion", "hyd_advance", "hyd_sync",
-                   "hyd_prepare", "hyd_recontour"):
+                   "hyd_prepare", "hyd_recontour", "hyd_surgery", "hyd_advect_step"):
             getattr(L, fn).argtypes = [C.c_void_p]
         L.hyd_renode.argtypes = [C.c_void_p, _D, _D, C.c_int32, _D, _D, C.c_int32,
                                  C.POINTER(C.c_int32)]
+        L.hyd_run.argtypes = [C.c_void_p, C.c_int32]
         L.hyd_set_timing.argtypes = [C.c_void_p, C.c_int32]
         L.hyd_kernel_times.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
                                        np.ctypeslib.ndpointer(dtype=np.float64),
@@ -258,7 +259,24 @@ class Hydra:
         _check(self.L.hyd_inversion(self.h))
 
     def advance(self):
+        """evolution.f90:166-326: one RK4 step."""
         _check(self.L.hyd_advance(self.h))
+
+    def surgery(self):
+        """contours.f90:350-683 on the device contour store."""
+        _check(self.L.hyd_surgery(self.h))
+
+    def advect_step(self):
+        """evolution.f90:55-80 loop body; returns 1 when the caller must
+        recontour (prepare has run), 0 after an advance."""
+        rc = self.L.hyd_advect_step(self.h)
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def run(self, nsteps):
+        """caps.f90:83-91 for nsteps advances (recontour, init, advect)."""
+        _check(self.L.hyd_run(self.h, int(nsteps)))
 
     def sync(self):
         _check(self.L.hyd_sync(self.h))

@@ -16,9 +16,6 @@ void set_error(const std::string &m) { g_err = m; }
 
 int inversion(Ctx &c);
 int adapt(Ctx &c);
-int advance(Ctx &c);
-int evolution_init(Ctx &c);
-int prepare(Ctx &c);
 
 KTimer::KTimer(Ctx &ctx, const char *name, int64_t bytes) : c(ctx) {
   if (!c.timing) return;
@@ -92,9 +89,9 @@ static void free_ctx(Ctx *c) {
                   c->qdiss_map, c->qdiss_exp, c->qs, c->qd, c->qc, c->qq, c->pp, c->uu, c->vv,
                   c->qr, c->emq, c->epq, c->qsi, c->qsf, c->sqs, c->qdi, c->qdf, c->sqd, c->wa,
                   c->wb, c->wc, c->wd, c->cnt, c->fq, c->mq, c->qjx_cnt, c->qbot, c->ixc,
-                  c->nxc, c->ndx, c->ndy, c->flags, c->replay, c->replay_val, c->xq, c->yq,
-                  c->xqi, c->yqi, c->xqf, c->yqf, c->uq, c->vq, c->nextq, c->nxt_tmp, c->dsc, c->dsum,
-                  c->seq_work};
+                  c->nxc, c->ndx, c->ndy, c->flags, c->replay, c->replay_val, c->rp_tkey,
+                  c->rp_tid, c->rp_list, c->xq, c->yq, c->xqi, c->yqi, c->xqf, c->yqf, c->uq,
+                  c->vq, c->nextq, c->nxt_tmp, c->dsc, c->dsum, c->seq_work};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->hsc) hipHostFree(c->hsc);
@@ -129,7 +126,15 @@ static int create(const hyd_params *p, hyd_ctx **out) {
   c->plan.fnorm = 1.0 / sqrt((double)n);
   c->plan.rnorm = 2.0 / sqrt((double)n);
   c->N = (int64_t)n * n;
-  c->cap = p->node_capacity > 0 ? p->node_capacity : 8 * c->N + 65536;
+  c->cap = p->node_capacity > 0 ? p->node_capacity : std::max<int64_t>(64 * c->N, 1 << 20);
+  if (c->cap > INT32_MAX / 2) {  // node numbers are int32 (nextq)
+    if (p->node_capacity > 0) {
+      set_error("node_capacity exceeds the int32 node numbering");
+      delete c;
+      return HYD_EARG;
+    }
+    c->cap = INT32_MAX / 2;
+  }
   c->dev = p->device;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
@@ -178,9 +183,20 @@ static int create(const hyd_params *p, hyd_ctx **out) {
   TRY(dalloc(&c->qjx_cnt, c->c.nxf));
   TRY(dalloc(&c->qbot, c->c.nxf));
   TRY(dalloc(&c->flags, 16));
-  c->replay_cap = 1 << 16;
-  TRY(dalloc(&c->replay, 3 * c->replay_cap));
-  TRY(dalloc(&c->replay_val, c->replay_cap));
+  c->replay_cap = 1 << 22;
+  c->rp_bcap = 1 << 20;
+  c->rp_tsize = 1 << 21;
+  TRY(dalloc(&c->replay, c->replay_cap));
+  TRY(dalloc(&c->replay_val, c->rp_bcap));
+  TRY(dalloc(&c->rp_list, 4 * c->rp_bcap));
+  TRY(dalloc(&c->rp_tid, c->rp_tsize));
+  TRY(dalloc(&c->rp_tkey, c->rp_tsize));
+  if (hipMemset(c->rp_tkey, 0xff, sizeof(int32_t) * c->rp_tsize) != hipSuccess ||
+      hipMemset(c->flags, 0, sizeof(int32_t) * 16) != hipSuccess) {
+    set_error("hipMemset failed");
+    free_ctx(c);
+    return HYD_EHIP;
+  }
   const int64_t P = c->cap;
   TRY(dalloc(&c->ixc, P));
   TRY(dalloc(&c->nxc, P));
@@ -373,13 +389,7 @@ int hyd_con2grid(hyd_ctx *h, const double *xq, const double *yq, const int32_t *
   c.hsc->qjump = saved;
   HYD_CHECK(rc);
   HYD_CHECK(d2h(c, qc, c.wa, c.N));
-  int32_t fl[8];
-  HYD_HIP(hipMemcpy(fl, c.flags, sizeof(fl), hipMemcpyDeviceToHost));
-  if (fl[3]) {
-    set_error("con2grid: too many order-dependent cells for the replay list");
-    return HYD_ECAPACITY;
-  }
-  return HYD_OK;
+  return c2g_status(c);
 }
 
 int hyd_sum_sequential(hyd_ctx *h, const double *a, int64_t n, double *sum) {
@@ -535,22 +545,32 @@ int hyd_set_scalar(hyd_ctx *h, int32_t id, double val) {
   return HYD_OK;
 }
 
+// Start of advect, evolution.f90:44-51: init, then twist = 0 and no
+// regularisations counted yet.
+static int advect_start(Ctx &c) {
+  HYD_CHECK(evolution_init(c));
+  HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  c.hsc->twist = 0.0;
+  HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
+  c.ireg = 0;
+  return HYD_OK;
+}
+
 int hyd_evolution_init(hyd_ctx *h) {
   Ctx &c = *C(h);
   if (!c.have_field) {
     set_error("evolution_init before set_field");
     return HYD_ESTATE;
   }
-  HYD_CHECK(evolution_init(c));
-  HYD_HIP(hipStreamSynchronize(c.s));
-  return HYD_OK;
+  HYD_CHECK(advect_start(c));
+  return c2g_status(c);
 }
 
 int hyd_prepare(hyd_ctx *h) {
   Ctx &c = *C(h);
   HYD_CHECK(prepare(c));
-  HYD_HIP(hipStreamSynchronize(c.s));
-  return HYD_OK;
+  return c2g_status(c);
 }
 
 int hyd_recontour(hyd_ctx *h) {
@@ -560,8 +580,7 @@ int hyd_recontour(hyd_ctx *h) {
     return HYD_ESTATE;
   }
   HYD_CHECK(recontour(c, c.qr));
-  HYD_HIP(hipStreamSynchronize(c.s));
-  return HYD_OK;
+  return c2g_status(c);
 }
 
 int hyd_renode(hyd_ctx *h, const double *xd, const double *yd, int32_t npd, double *xr,
@@ -613,14 +632,67 @@ int hyd_advance(hyd_ctx *h) {
     return HYD_ESTATE;
   }
   HYD_CHECK(advance(c));
-  int32_t fl[8];
-  HYD_HIP(hipMemcpyAsync(fl, c.flags, sizeof(fl), hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+  return c2g_status(c);
+}
+
+int hyd_surgery(hyd_ctx *h) {
+  Ctx &c = *C(h);
+  HYD_CHECK(surgery(c));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+// advect loop body, evolution.f90:55-80
+static int advect_step(Ctx &c) {
+  const double twistmax = 2.5;
+  const int nregmax = 20;
+  if (c.hsc->twist > twistmax) {
+    c.ireg = c.ireg + 1;
+    if (c.ireg == nregmax) {
+      HYD_CHECK(prepare(c));
+      HYD_HIP(hipStreamSynchronize(c.s));
+      return 1;
+    }
+    HYD_CHECK(surgery(c));
+    HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+    HYD_HIP(hipStreamSynchronize(c.s));
+    c.hsc->twist = c.hsc->twist - twistmax;
+    HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
+  }
+  return hyd_advance((hyd_ctx *)&c);
+}
+
+int hyd_advect_step(hyd_ctx *h) {
+  Ctx &c = *C(h);
+  if (!c.have_field) {
+    set_error("advect_step before set_field");
+    return HYD_ESTATE;
+  }
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
   HYD_HIP(hipStreamSynchronize(c.s));
-  if (fl[3]) {
-    set_error("con2grid: too many order-dependent cells for the replay list");
-    return HYD_ECAPACITY;
+  return advect_step(c);
+}
+
+// caps.f90:83-91 with a step count in place of the final time
+int hyd_run(hyd_ctx *h, int32_t nsteps) {
+  Ctx &c = *C(h);
+  if (!c.have_field) {
+    set_error("run before set_field");
+    return HYD_ESTATE;
   }
+  int done = 0;
+  while (done < nsteps) {
+    HYD_CHECK(recontour(c, c.qr));
+    HYD_CHECK(advect_start(c));
+    while (done < nsteps) {
+      const int r = advect_step(c);
+      if (r < 0) return r;
+      if (r == 1) break;
+      done++;
+    }
+  }
+  HYD_HIP(hipStreamSynchronize(c.s));
   return HYD_OK;
 }
 

@@ -9,7 +9,7 @@
 //     counter of its fine cell (low 16 bits: +qjump count, high 16: -qjump).
 //     A cell's sequential node-order sum depends only on the counts unless it
 //     holds both signs with at least three of one sign; those rare cells are
-//     flagged on the fly and re-summed in node order from a replay list.
+//     flagged on the fly and re-summed in node order from per-cell buckets.
 //   * the y sweep runs one wavefront per fine column over 64-row segments;
 //     only cells with crossings change the running value, so the serial
 //     chain is the column's crossings, in row order, exactly as the
@@ -26,6 +26,24 @@
 
 namespace hyd {
 
+// Order-dependent cells ("flagged": both signs, at least three of one) are
+// re-summed in node order from per-cell buckets: an open-addressing table
+// maps the cell key to a bucket; a claim pass gives every flagged cell a
+// bucket of exactly p+q slots, a fill pass drops each crossing's (node, sign)
+// into its cell's bucket, and one thread per bucket orders its few entries by
+// node and adds them as the reference does.  Nothing is sorted globally and
+// the table is left empty again at the end of the sweep.
+struct Buckets {
+  int32_t *tkey;    // table keys, -1 = empty
+  int32_t *tid;     // bucket of each table slot
+  int32_t *list;    // per bucket: table slot, first entry, size, fill count
+  int32_t *slots;   // entries: node << 1 | (sign > 0)
+  double *vals;     // node-order sum of each bucket
+  uint32_t tmask;   // table size - 1 (power of two)
+  int32_t bcap;     // buckets
+  int64_t scap;     // entries
+};
+
 struct C2GArgs {
   const double *xq, *yq;
   const int32_t *nextq;
@@ -36,9 +54,10 @@ struct C2GArgs {
   const double *xgf;
   int32_t *cnt;      // nyf*nxf packed counters
   int32_t *qjx;      // nxf packed counters (bottom row)
-  int32_t *flags;    // [0] flagged cells, [1] flagged qjx, [2] replay entries, [3] overflow
-  int32_t *replay;   // triples (key, node, sign)
-  int64_t replay_cap;
+  // [0] flagged cells, [1] flagged qjx, [2] bucket entries used, [4] buckets
+  // used, [5] dropped edge crossings; [8] sticky error (kept across sweeps)
+  int32_t *flags;
+  Buckets bk;
   int32_t *ixc, *nxc;
   double *ndx, *ndy;
 };
@@ -52,6 +71,22 @@ __device__ __forceinline__ void add_count(int32_t *slot, int sign, int32_t *flag
   const int32_t d = sign > 0 ? 1 : (1 << 16);
   const int32_t old = atomicAdd(slot, d);
   if (!is_flagged(old) && is_flagged(old + d)) atomicAdd(flag, 1);
+}
+
+__device__ __forceinline__ uint32_t bk_hash(int32_t key, uint32_t mask) {
+  return ((uint32_t)key * 2654435761u) & mask;
+}
+
+// table slot holding key, or -1
+__device__ __forceinline__ int64_t bk_find(const Buckets &b, int32_t key) {
+  uint32_t h = bk_hash(key, b.tmask);
+  for (uint32_t probe = 0; probe <= b.tmask; probe++) {
+    const int32_t k = b.tkey[h];
+    if (k == key) return h;
+    if (k == -1) return -1;
+    h = (h + 1) & b.tmask;
+  }
+  return -1;
 }
 
 // Node phase: contours.f90:206-231 and the crossing loop 249-269.
@@ -102,8 +137,57 @@ __global__ void c2g_nodes(C2GArgs a) {
   }
 }
 
-// Rare path: list the contributions to flagged cells in node order.
-__global__ void c2g_replay_collect(C2GArgs a) {
+// Claim (PASS 0) and fill (PASS 1) of the buckets of flagged cells; both
+// revisit the crossings of node i exactly as c2g_nodes found them.
+template <int PASS>
+__device__ __forceinline__ void bk_visit(const C2GArgs &a, int32_t key, int32_t w, int node,
+                                         int sgn) {
+  const Buckets &b = a.bk;
+  if (PASS == 0) {
+    uint32_t h = bk_hash(key, b.tmask);
+    for (uint32_t probe = 0;; probe++) {
+      if (probe > b.tmask) {
+        atomicOr(&a.flags[8], 1);
+        return;
+      }
+      const int32_t prev = atomicCAS(&b.tkey[h], -1, key);
+      if (prev == key) return;
+      if (prev == -1) {
+        const int sz = (w & 0xffff) + ((w >> 16) & 0xffff);
+        const int nb = atomicAdd(&a.flags[4], 1);
+        const int base = atomicAdd(&a.flags[2], sz);
+        if (nb >= b.bcap || (int64_t)base + sz > b.scap) {
+          b.tid[h] = -1;
+          atomicOr(&a.flags[8], 1);
+          return;
+        }
+        b.tid[h] = nb;
+        b.list[4 * nb] = (int32_t)h;
+        b.list[4 * nb + 1] = base;
+        b.list[4 * nb + 2] = sz;
+        b.list[4 * nb + 3] = 0;
+        return;
+      }
+      h = (h + 1) & b.tmask;
+    }
+  } else {
+    const int64_t h = bk_find(b, key);
+    const int nb = h >= 0 ? b.tid[h] : -1;
+    if (nb < 0) {
+      atomicOr(&a.flags[8], 1);
+      return;
+    }
+    const int pos = atomicAdd(&b.list[4 * nb + 3], 1);
+    if (pos >= b.list[4 * nb + 2]) {
+      atomicOr(&a.flags[8], 2);
+      return;
+    }
+    b.slots[b.list[4 * nb + 1] + pos] = (node << 1) | (sgn > 0 ? 1 : 0);
+  }
+}
+
+template <int PASS>
+__global__ void c2g_buckets(C2GArgs a) {
   if (a.flags[0] == 0 && a.flags[1] == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
@@ -117,15 +201,9 @@ __global__ void c2g_replay_collect(C2GArgs a) {
     const double cc = copysign(1.0, dy);
     const double p = -(yi - a.hly * cc) / (dy + a.small);
     const int ix = 1 + (int)(a.glxfi * (fmod(a.ellx + a.hlx + xi + p * dx, a.ellx)));
-    if (ix >= 1 && ix <= a.nxf && is_flagged(a.qjx[ix - 1])) {
-      const int64_t k = atomicAdd(&a.flags[2], 1);
-      if (k < a.replay_cap) {
-        a.replay[3 * k] = (int32_t)(nkey + ix - 1);
-        a.replay[3 * k + 1] = i;
-        a.replay[3 * k + 2] = cc > 0.0 ? -1 : 1;
-      } else {
-        a.flags[3] = 1;
-      }
+    if (ix >= 1 && ix <= a.nxf) {
+      const int32_t w = a.qjx[ix - 1];
+      if (is_flagged(w)) bk_visit<PASS>(a, (int32_t)(nkey + ix - 1), w, i, cc > 0.0 ? -1 : 1);
     }
   }
   if (a.flags[0] != 0 && nxc != 0) {
@@ -140,56 +218,48 @@ __global__ void c2g_replay_collect(C2GArgs a) {
       const int iy = 2 + (int)(a.glyfi * (y2 - a.elly * (double)(int)(y2 * a.hlyi) - a.ymin));
       if (iy < 2 || iy > a.nyf) continue;
       const int64_t cell = (int64_t)(ix - 1) * a.nyf + (iy - 1);
-      if (is_flagged(a.cnt[cell])) {
-        const int64_t k = atomicAdd(&a.flags[2], 1);
-        if (k < a.replay_cap) {
-          a.replay[3 * k] = (int32_t)cell;
-          a.replay[3 * k + 1] = i;
-          a.replay[3 * k + 2] = sgn;
-        } else {
-          a.flags[3] = 1;
-        }
-      }
+      const int32_t w = a.cnt[cell];
+      if (is_flagged(w)) bk_visit<PASS>(a, (int32_t)cell, w, i, sgn);
     }
   }
 }
 
-// Rare path: sort the replay list by (key, node) and sum each key in node
-// order.  Single thread; the list is empty in all but pathological cases.
-__global__ void c2g_replay_resolve(int32_t *flags, int32_t *replay, double *vals, int64_t cap,
-                                   double qjump) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int64_t k = flags[2];
-  if (k == 0) return;
-  if (k > cap) k = cap;
-  // insertion sort on (key, node)
-  for (int64_t a = 1; a < k; a++) {
-    const int32_t k0 = replay[3 * a], k1 = replay[3 * a + 1], k2 = replay[3 * a + 2];
-    int64_t b = a - 1;
-    while (b >= 0 && (replay[3 * b] > k0 || (replay[3 * b] == k0 && replay[3 * b + 1] > k1))) {
-      replay[3 * (b + 1)] = replay[3 * b];
-      replay[3 * (b + 1) + 1] = replay[3 * b + 1];
-      replay[3 * (b + 1) + 2] = replay[3 * b + 2];
-      b--;
+// One thread per bucket: order the entries by node and add them in that
+// order (the reference's node loop).
+__global__ void c2g_bucket_sum(Buckets b, const int32_t *flags, double qjump) {
+  const int nb_used = min(flags[4], b.bcap);
+  for (int nb = blockIdx.x * blockDim.x + threadIdx.x; nb < nb_used;
+       nb += gridDim.x * blockDim.x) {
+    const int base = b.list[4 * nb + 1], sz = b.list[4 * nb + 2];
+    int32_t *e = b.slots + base;
+    for (int t = 1; t < sz; t++) {
+      const int32_t v = e[t];
+      int u = t - 1;
+      while (u >= 0 && (e[u] >> 1) > (v >> 1)) {
+        e[u + 1] = e[u];
+        u--;
+      }
+      e[u + 1] = v;
     }
-    replay[3 * (b + 1)] = k0;
-    replay[3 * (b + 1) + 1] = k1;
-    replay[3 * (b + 1) + 2] = k2;
+    double s = 0.0;
+    for (int t = 0; t < sz; t++) s = s + ((e[t] & 1) ? qjump : -qjump);
+    b.vals[nb] = s;
   }
-  // the value of a key is stored at the position of its last entry
-  double s = 0.0;
-  for (int64_t a = 0; a < k; a++) {
-    if (a == 0 || replay[3 * a] != replay[3 * (a - 1)]) s = 0.0;
-    s = s + (replay[3 * a + 2] > 0 ? qjump : -qjump);
-    vals[a] = s;
+}
+
+// Empty the table again (after the sweep has read the bucket sums).
+__global__ void c2g_bucket_clear(Buckets b, int32_t *flags) {
+  const int nb_used = min(flags[4], b.bcap);
+  for (int nb = blockIdx.x * blockDim.x + threadIdx.x; nb < nb_used;
+       nb += gridDim.x * blockDim.x) {
+    b.tkey[b.list[4 * nb]] = -1;
+    if (b.list[4 * nb + 3] != b.list[4 * nb + 2]) atomicOr(&flags[8], 2);
   }
-  flags[4] = (int32_t)k;
 }
 
 // Value of a cell's node-order sum from its counts.
 __device__ __forceinline__ double cell_value(int32_t w, int64_t key, double qjump,
-                                             const int32_t *replay, const double *vals,
-                                             int32_t nrep) {
+                                             const Buckets &b) {
   const int p = w & 0xffff, q = (w >> 16) & 0xffff;
   if (q == 0) {
     double s = 0.0;
@@ -202,19 +272,9 @@ __device__ __forceinline__ double cell_value(int32_t w, int64_t key, double qjum
     return s;
   }
   if (p <= 2 && q <= 2) return (double)(p - q) * qjump;
-  // flagged: binary search for the last replay entry of this key
-  int lo = 0, hi = nrep - 1, last = -1;
-  while (lo <= hi) {
-    const int mid = (lo + hi) >> 1;
-    const int32_t km = replay[3 * mid];
-    if (km <= key) {
-      if (km == key) last = mid;
-      lo = mid + 1;
-    } else {
-      hi = mid - 1;
-    }
-  }
-  return last >= 0 ? vals[last] : 0.0 / 0.0;
+  const int64_t h = bk_find(b, (int32_t)key);
+  const int nb = h >= 0 ? b.tid[h] : -1;
+  return nb >= 0 ? b.vals[nb] : 0.0 / 0.0;
 }
 
 // qbot (contours.f90:234-238): one wavefront, chain over the non-empty
@@ -223,11 +283,9 @@ __device__ __forceinline__ double cell_value(int32_t w, int64_t key, double qjum
 // runs the chain.  nxf <= kQbotMax.
 static constexpr int kQbotMax = 32768;
 __global__ void __launch_bounds__(1024) c2g_qbot(const int32_t *qjx, double *qbot, int nxf,
-                                                 int64_t nkey, double qjump, const int32_t *flags,
-                                                 const int32_t *replay, const double *vals) {
+                                                 int64_t nkey, double qjump, Buckets bk) {
   __shared__ double cvs[kQbotMax / 4];   // values of one quarter at a time
   __shared__ uint8_t nz[kQbotMax / 4];
-  const int nrep = flags[4];
   const int q4 = kQbotMax / 4;
   double v = 0.0;  // qbot(1)
   if (threadIdx.x == 0) qbot[0] = 0.0;
@@ -237,7 +295,7 @@ __global__ void __launch_bounds__(1024) c2g_qbot(const int32_t *qjx, double *qbo
     for (int t = threadIdx.x; t < m; t += blockDim.x) {
       const int32_t w = qjx[part + t];
       nz[t] = w != 0;
-      cvs[t] = w ? cell_value(w, nkey + part + t, qjump, replay, vals, nrep) : 0.0;
+      cvs[t] = w ? cell_value(w, nkey + part + t, qjump, bk) : 0.0;
     }
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -264,13 +322,11 @@ __global__ void __launch_bounds__(1024) c2g_qbot(const int32_t *qjx, double *qbo
 
 // y sweep (contours.f90:272-277): one wavefront per fine column.
 __global__ void c2g_colscan(const int32_t *cnt, const double *qbot, double *fq, int nxf, int nyf,
-                            double qjump, const int32_t *flags, const int32_t *replay,
-                            const double *vals) {
+                            double qjump, Buckets bk) {
   const int wpb = blockDim.x / 64;
   const int col = blockIdx.x * wpb + threadIdx.x / 64;
   const int l = threadIdx.x & 63;
   if (col >= nxf) return;
-  const int nrep = flags[4];
   const int32_t *cc = cnt + (int64_t)col * nyf;
   double *out = fq + (int64_t)col * nyf;
   double v = qbot[col];
@@ -286,7 +342,7 @@ __global__ void c2g_colscan(const int32_t *cnt, const double *qbot, double *fq, 
     for (int u = 0; u < U; u++) {
       const int r = base + 64 * u + l;  // iy-1
       const double cv =
-          w[u] ? cell_value(w[u], (int64_t)col * nyf + r, qjump, replay, vals, nrep) : 0.0;
+          w[u] ? cell_value(w[u], (int64_t)col * nyf + r, qjump, bk) : 0.0;
       uint64_t mask = __ballot(w[u] != 0);
       double mine = v;
       while (mask) {
@@ -339,6 +395,10 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
                int32_t *qjx, double *qbot, double *fq) {
   const Consts &k = c.c;
   const int64_t nfine = (int64_t)nxf * nyf;
+  if (nfine + nxf > INT32_MAX) {
+    set_error("con2grid: fine grid too large for int32 cell keys");
+    return HYD_EARG;
+  }
   HYD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * nfine, c.s));
   HYD_HIP(hipMemsetAsync(qjx, 0, sizeof(int32_t) * nxf, c.s));
   HYD_HIP(hipMemsetAsync(c.flags, 0, sizeof(int32_t) * 8, c.s));
@@ -366,8 +426,14 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
   a.cnt = cnt;
   a.qjx = qjx;
   a.flags = c.flags;
-  a.replay = c.replay;
-  a.replay_cap = c.replay_cap;
+  a.bk.tkey = c.rp_tkey;
+  a.bk.tid = c.rp_tid;
+  a.bk.list = c.rp_list;
+  a.bk.slots = c.replay;
+  a.bk.vals = c.replay_val;
+  a.bk.tmask = (uint32_t)(c.rp_tsize - 1);
+  a.bk.bcap = (int32_t)c.rp_bcap;
+  a.bk.scap = c.replay_cap;
   a.ixc = c.ixc;
   a.nxc = c.nxc;
   a.ndx = c.ndx;
@@ -376,16 +442,31 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
   if (n > 0) {
     hipLaunchKernelGGL(c2g_nodes, dim3(nb), dim3(256), 0, c.s, a);
     HYD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(c2g_replay_collect, dim3(nb), dim3(256), 0, c.s, a);
+    hipLaunchKernelGGL(c2g_buckets<0>, dim3(nb), dim3(256), 0, c.s, a);
+    hipLaunchKernelGGL(c2g_buckets<1>, dim3(nb), dim3(256), 0, c.s, a);
   }
-  hipLaunchKernelGGL(c2g_replay_resolve, dim3(1), dim3(64), 0, c.s, c.flags, c.replay,
-                     c.replay_val, c.replay_cap, a.qjump);
-  hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(1024), 0, c.s, qjx, qbot, nxf, nfine, a.qjump,
-                     c.flags, c.replay, c.replay_val);
+  hipLaunchKernelGGL(c2g_bucket_sum, dim3(256), dim3(256), 0, c.s, a.bk, c.flags, a.qjump);
+  hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(1024), 0, c.s, qjx, qbot, nxf, nfine, a.qjump, a.bk);
   hipLaunchKernelGGL(c2g_colscan, dim3((nxf + 3) / 4), dim3(256), 0, c.s, cnt, qbot, fq, nxf,
-                     nyf, a.qjump, c.flags, c.replay, c.replay_val);
+                     nyf, a.qjump, a.bk);
+  hipLaunchKernelGGL(c2g_bucket_clear, dim3(256), dim3(256), 0, c.s, a.bk, c.flags);
   HYD_HIP(hipGetLastError());
   return HYD_OK;
+}
+
+// Reports (and clears) an overflow of the order-dependent-cell buckets of any
+// sweep since the last check; the bucket table is emptied so that the next
+// sweep starts clean.
+int c2g_status(Ctx &c) {
+  int32_t e = 0;
+  HYD_HIP(hipMemcpyAsync(&e, c.flags + 8, sizeof(int32_t), hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  if (e == 0) return HYD_OK;
+  HYD_HIP(hipMemsetAsync(c.rp_tkey, 0xff, sizeof(int32_t) * c.rp_tsize, c.s));
+  HYD_HIP(hipMemsetAsync(c.flags + 8, 0, sizeof(int32_t), c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  set_error("con2grid: too many order-dependent cells for the bucket table");
+  return HYD_ECAPACITY;
 }
 
 int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n,

@@ -114,8 +114,11 @@ struct Ctx {
   double *qbot = nullptr;      // nxf
   int32_t *ixc = nullptr, *nxc = nullptr;
   double *ndx = nullptr, *ndy = nullptr;
-  int32_t *flags = nullptr;    // small device counters
-  int32_t *replay = nullptr;   // rare order-dependent cells: (cell, node, sign)
+  int32_t *flags = nullptr;    // small device counters; [8] sticky overflow
+  // order-dependent cells: key -> bucket table, bucket list, entries, sums
+  int32_t *rp_tkey = nullptr, *rp_tid = nullptr, *rp_list = nullptr;
+  int64_t rp_tsize = 0, rp_bcap = 0;
+  int32_t *replay = nullptr;   // bucket entries (node << 1 | sign)
   double *replay_val = nullptr;
   int64_t replay_cap = 0;
   // nodes
@@ -126,6 +129,7 @@ struct Ctx {
   std::vector<int32_t> i1q, i2q, npq, indq;  // contour table (host copy)
   int nptq = 0, nq = 0;
   bool contex = false;
+  int ireg = 0;  // regularisations since the last recontouring
   bool have_field = false;
   // scalars
   Scalars *dsc = nullptr;   // device
@@ -164,6 +168,7 @@ int velint(Ctx &c, const double *uu, const double *vv, const double *xq, const d
            int n, double *uq, double *vq);
 int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n,
              double *qc, int32_t *counts_out);
+int c2g_status(Ctx &c);
 
 template <class T>
 int dbuf(Ctx &c, const char *name, int64_t n, T **out) {
@@ -184,6 +189,10 @@ int dbuf(Ctx &c, const char *name, int64_t n, T **out) {
 }
 
 int recontour(Ctx &c, const double *qq);
+int surgery(Ctx &c);
+int prepare(Ctx &c);
+int evolution_init(Ctx &c);
+int advance(Ctx &c);
 void free_congen(Ctx &c);
 int renode_batch(Ctx &c, const double *xd, const double *yd, const int64_t *off_in,
                  const int32_t *npd, int ncont, int64_t nin, double *xr, double *yr,

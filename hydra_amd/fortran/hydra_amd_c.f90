@@ -1,0 +1,237 @@
+! hydra_amd_c.f90 -- iso_c_binding interface to libhydra_amd.so.
+!
+! A Fortran host (the CAPS driver of ca/plane/qg/caps) uses this module to
+! call the MI355X step through the C ABI of include/hydra_amd.h.  Arrays are
+! passed by reference exactly as the reference keeps them: physical fields
+! (ny,nx), spectral fields (nx,ny), contour arrays 1-based.  Every function
+! returns 0 on success; hyd_check stops the program on an error, which is how
+! the reference reacts to the same conditions (stafft.f90:85-93).
+module hydra_amd_c
+use, intrinsic :: iso_c_binding
+implicit none
+
+integer(c_int), parameter :: HYD_OK = 0
+
+type, bind(c) :: hyd_params
+  integer(c_int32_t) :: ng, ncontq
+  real(c_double) :: ellx, elly, kd, beta, rtherm, rekman, cdamp, nnu
+  real(c_double) :: tgsave, tcsave, dt_fixed
+  integer(c_int64_t) :: node_capacity
+  integer(c_int32_t) :: precision, device
+end type hyd_params
+
+interface
+  function hyd_last_error() bind(c, name="hyd_last_error")
+    import :: c_ptr
+    type(c_ptr) :: hyd_last_error
+  end function
+
+  integer(c_int) function hyd_init(p, ctx) bind(c, name="hyd_init")
+    import :: c_int, c_ptr, hyd_params
+    type(hyd_params), intent(in) :: p
+    type(c_ptr), intent(out) :: ctx
+  end function
+
+  integer(c_int) function hyd_destroy(ctx) bind(c, name="hyd_destroy")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+
+  ! sta2dfft.f90:103-165
+  integer(c_int) function hyd_ptospc(ctx, rvar, svar) bind(c, name="hyd_ptospc")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: rvar(*)
+    real(c_double), intent(out) :: svar(*)
+  end function
+
+  integer(c_int) function hyd_spctop(ctx, svar, rvar) bind(c, name="hyd_spctop")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: svar(*)
+    real(c_double), intent(out) :: rvar(*)
+  end function
+
+  ! sta2dfft.f90:169-245
+  integer(c_int) function hyd_xderiv(ctx, var, der) bind(c, name="hyd_xderiv")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: var(*)
+    real(c_double), intent(out) :: der(*)
+  end function
+
+  integer(c_int) function hyd_yderiv(ctx, var, der) bind(c, name="hyd_yderiv")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: var(*)
+    real(c_double), intent(out) :: der(*)
+  end function
+
+  ! spectral.f90:304-366
+  integer(c_int) function hyd_main_invert(ctx, qq, uu, vv, pp) bind(c, name="hyd_main_invert")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: qq(*)
+    real(c_double), intent(out) :: uu(*), vv(*), pp(*)
+  end function
+
+  integer(c_int) function hyd_gradient(ctx, ff, ffx, ffy) bind(c, name="hyd_gradient")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: ff(*)
+    real(c_double), intent(out) :: ffx(*), ffy(*)
+  end function
+
+  ! contours.f90:136-169
+  integer(c_int) function hyd_velint(ctx, uu, vv, xq, yq, nptq, uq, vq) &
+      bind(c, name="hyd_velint")
+    import :: c_int, c_ptr, c_double, c_int32_t
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: uu(*), vv(*), xq(*), yq(*)
+    integer(c_int32_t), value :: nptq
+    real(c_double), intent(out) :: uq(*), vq(*)
+  end function
+
+  ! contours.f90:173-346
+  integer(c_int) function hyd_con2grid(ctx, xq, yq, nextq, nptq, qjump, qc, counts) &
+      bind(c, name="hyd_con2grid")
+    import :: c_int, c_ptr, c_double, c_int32_t
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: xq(*), yq(*)
+    integer(c_int32_t), intent(in) :: nextq(*)
+    integer(c_int32_t), value :: nptq
+    real(c_double), value :: qjump
+    real(c_double), intent(out) :: qc(*)
+    type(c_ptr), value :: counts
+  end function
+
+  ! contours.f90:687-933
+  integer(c_int) function hyd_renode(ctx, xd, yd, npd, xr, yr, cap_out, npr) &
+      bind(c, name="hyd_renode")
+    import :: c_int, c_ptr, c_double, c_int32_t
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: xd(*), yd(*)
+    integer(c_int32_t), value :: npd, cap_out
+    real(c_double), intent(out) :: xr(*), yr(*)
+    integer(c_int32_t), intent(out) :: npr
+  end function
+
+  ! device-resident state: fields, contours, scalars
+  integer(c_int) function hyd_set_field(ctx, qq, t0) bind(c, name="hyd_set_field")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: qq(*)
+    real(c_double), value :: t0
+  end function
+
+  integer(c_int) function hyd_set_contours(ctx, nptq, nq, xq, yq, nextq, i1q, i2q, npq, &
+      indq, qjump) bind(c, name="hyd_set_contours")
+    import :: c_int, c_ptr, c_double, c_int32_t
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: nptq, nq
+    real(c_double), intent(in) :: xq(*), yq(*)
+    integer(c_int32_t), intent(in) :: nextq(*), i1q(*), i2q(*), npq(*), indq(*)
+    real(c_double), value :: qjump
+  end function
+
+  integer(c_int) function hyd_get_counts(ctx, nptq, nq) bind(c, name="hyd_get_counts")
+    import :: c_int, c_ptr, c_int32_t
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), intent(out) :: nptq, nq
+  end function
+
+  integer(c_int) function hyd_get_contours(ctx, xq, yq, nextq, i1q, i2q, npq, indq) &
+      bind(c, name="hyd_get_contours")
+    import :: c_int, c_ptr, c_double, c_int32_t
+    type(c_ptr), value :: ctx
+    real(c_double), intent(out) :: xq(*), yq(*)
+    integer(c_int32_t), intent(out) :: nextq(*), i1q(*), i2q(*), npq(*), indq(*)
+  end function
+
+  integer(c_int) function hyd_get_field(ctx, id, out) bind(c, name="hyd_get_field")
+    import :: c_int, c_ptr, c_double, c_int32_t
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: id
+    real(c_double), intent(out) :: out(*)
+  end function
+
+  integer(c_int) function hyd_get_scalar(ctx, id, v) bind(c, name="hyd_get_scalar")
+    import :: c_int, c_ptr, c_double, c_int32_t
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: id
+    real(c_double), intent(out) :: v
+  end function
+
+  ! contours.f90:350-683, congen.f90:24-128, evolution.f90:28-326
+  integer(c_int) function hyd_surgery(ctx) bind(c, name="hyd_surgery")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+
+  integer(c_int) function hyd_recontour(ctx) bind(c, name="hyd_recontour")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+
+  integer(c_int) function hyd_evolution_init(ctx) bind(c, name="hyd_evolution_init")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+
+  integer(c_int) function hyd_prepare(ctx) bind(c, name="hyd_prepare")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+
+  integer(c_int) function hyd_inversion(ctx) bind(c, name="hyd_inversion")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+
+  integer(c_int) function hyd_advance(ctx) bind(c, name="hyd_advance")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+
+  integer(c_int) function hyd_advect_step(ctx) bind(c, name="hyd_advect_step")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+
+  integer(c_int) function hyd_run(ctx, nsteps) bind(c, name="hyd_run")
+    import :: c_int, c_ptr, c_int32_t
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: nsteps
+  end function
+end interface
+
+contains
+
+! Text of the last error as a Fortran string.
+function hyd_error_text() result(msg)
+  character(len=:), allocatable :: msg
+  character(kind=c_char), pointer :: s(:)
+  type(c_ptr) :: p
+  integer :: n
+  p = hyd_last_error()
+  call c_f_pointer(p, s, [4096])
+  n = 0
+  do while (n < 4096)
+    if (s(n + 1) == c_null_char) exit
+    n = n + 1
+  enddo
+  allocate(character(len=n) :: msg)
+  msg = transfer(s(1:n), msg)
+end function
+
+! Stop on a non-zero status, as the reference stops on its errors.
+subroutine hyd_check(rc, what)
+  integer(c_int), intent(in) :: rc
+  character(len=*), intent(in) :: what
+  if (rc /= HYD_OK) then
+    write(*, '(a,a,a,i0,a,a)') ' hydra_amd: ', what, ' failed (', rc, '): ', hyd_error_text()
+    stop 1
+  endif
+end subroutine
+
+end module hydra_amd_c

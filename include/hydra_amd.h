@@ -52,7 +52,7 @@ typedef struct hyd_params {
   double cdamp, nnu;     /* parameters.f90:32                              */
   double tgsave, tcsave; /* parameters.f90:23 (only dtmax depends on them) */
   double dt_fixed;       /* > 0: fixed time step instead of adapt's choice */
-  int64_t node_capacity; /* device node capacity (0: 64*ng*ng)             */
+  int64_t node_capacity; /* device node capacity (0: max(64*ng*ng, 2^20))   */
   int32_t precision;     /* 0: plain FP64, operation-order faithful        */
   int32_t device;        /* HIP device ordinal                             */
 } hyd_params;
@@ -102,6 +102,12 @@ int hyd_con2grid(hyd_ctx *ctx, const double *xq, const double *yq, const int32_t
 int hyd_renode(hyd_ctx *ctx, const double *xd, const double *yd, int32_t npd, double *xr,
                double *yr, int32_t cap_out, int32_t *npr);
 
+/* surgery contours.f90:350-683 on the device contour store: nodes closer
+   than dm to a segment of the same PV level are reconnected (in the
+   reference's node order), contours are rebuilt, renoded, and contours of
+   three or fewer nodes are removed. */
+int hyd_surgery(hyd_ctx *ctx);
+
 /* ---- ca/plane/qg/caps/congen.f90 ------------------------------------------ */
 /* recontour(qr) congen.f90:24-128 on the device state: the contours and the
    residual qr (the full PV when no contours exist yet) are turned into new
@@ -132,7 +138,9 @@ int hyd_put_field(hyd_ctx *ctx, int32_t id, const double *in);
 int hyd_get_scalar(hyd_ctx *ctx, int32_t id, double *out);
 int hyd_set_scalar(hyd_ctx *ctx, int32_t id, double v);
 
-/* init (evolution.f90:96-131): qc from the contours, qd = fhi*(qs-qc). */
+/* init (evolution.f90:96-131): qc from the contours, qd = fhi*(qs-qc); then
+   the start of advect (evolution.f90:44-51): twist = 0, regularisation
+   count 0.  Call after every recontouring. */
 int hyd_evolution_init(hyd_ctx *ctx);
 /* prepare (evolution.f90:135-162): fold the contour PV into qs and leave the
    residual qr = spctop(qs-qc) for recontouring. */
@@ -141,6 +149,15 @@ int hyd_prepare(hyd_ctx *ctx);
 int hyd_inversion(hyd_ctx *ctx);
 /* advance (evolution.f90:166-326): one RK4 step of nodes and fields. */
 int hyd_advance(hyd_ctx *ctx);
+/* One pass of the advect loop body, evolution.f90:55-80: when twist exceeds
+   2.5, count a regularisation; at the 20th, prepare and return 1 (the caller
+   recontours, caps.f90:83-91), otherwise surgery and twist -= 2.5.  Then one
+   advance.  Returns 0 after an advance, 1 after prepare, < 0 on error. */
+int hyd_advect_step(hyd_ctx *ctx);
+/* caps.f90:83-91 driven by a step count: recontour, init, then advect
+   steps, recontouring whenever the regularisation count reaches 20, until
+   nsteps advances have been taken. */
+int hyd_run(hyd_ctx *ctx, int32_t nsteps);
 
 /* Synchronise the context's stream (timing helpers). */
 int hyd_sync(hyd_ctx *ctx);

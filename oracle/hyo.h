@@ -112,6 +112,8 @@ void hyo_con2grid(const hyo_model *m, double *qc);
    (round(qa/qjump), rows 1..nyf) -- the exact integer contract. */
 void hyo_con2grid_counts(const hyo_model *m, double *qc, int32_t *counts);
 int hyo_surgery(hyo_model *m);
+int hyo_renode_cap(const hyo_model *m, const double *xd, const double *yd, int npd,
+                   double *xr, double *yr, int64_t cap, int *npr);
 int hyo_renode(const hyo_model *m, const double *xd, const double *yd, int npd,
                double *xr, double *yr, int *npr);
 

@@ -88,9 +88,9 @@ def lib():
     return _lib
 
 
-def default_params(ng, kd=0.0, ncontq=80, dt_fixed=0.0, npm=0, tsave=1000.0):
+def default_params(ng, kd=0.0, ncontq=80, dt_fixed=0.0, npm=0, tsave=1000.0, cdamp=2.0):
     return HyoParams(nx=ng, ny=ng, ncontq=ncontq, ellx=2 * np.pi, elly=2 * np.pi, kd=kd,
-                     beta=0.0, rtherm=0.0, rekman=0.0, cdamp=2.0, nnu=3.0, tgsave=tsave,
+                     beta=0.0, rtherm=0.0, rekman=0.0, cdamp=cdamp, nnu=3.0, tgsave=tsave,
                      tcsave=tsave, dt_fixed=dt_fixed, npm=npm)
 
 

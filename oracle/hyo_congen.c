@@ -219,13 +219,13 @@ static int ugrid2con(hyo_model *m, const double *qa, work_store *w) {
       }
       int keep = 0, npa = 0;
       double *xo = w->xa + (ibeg - 1), *yo = w->ya + (ibeg - 1);
-      if ((int64_t)ibeg - 1 + 2 * (int64_t)npd + 16 > w->npm) { status = HYO_ERR_CAPACITY; goto out; }
+      const int64_t capo = w->npm - ((int64_t)ibeg - 1);
       do {
-        if ((status = hyo_renode(m, xd, yd, npd, xo, yo, &npa))) goto out;
+        if ((status = hyo_renode_cap(m, xd, yd, npd, xo, yo, capo, &npa))) goto out;
         if (npa == 0) break;
-        if ((status = hyo_renode(m, xo, yo, npa, xd, yd, &npd))) goto out;
+        if ((status = hyo_renode_cap(m, xo, yo, npa, xd, yd, nprm, &npd))) goto out;
         if (npd == 0) break;
-        if ((status = hyo_renode(m, xd, yd, npd, xo, yo, &npa))) goto out;
+        if ((status = hyo_renode_cap(m, xd, yd, npd, xo, yo, capo, &npa))) goto out;
         if (npa == 0) break;
         keep = 1;
       } while (0);

@@ -18,6 +18,7 @@
  *                oracle/_ref.
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -177,10 +178,11 @@ void hyo_con2grid_counts(const hyo_model *m, double *qc, int32_t *counts) {
 
 void hyo_con2grid(const hyo_model *m, double *qc) { hyo_con2grid_counts(m, qc, NULL); }
 
-/* contours.f90:687-933.  xd,yd have npd entries; xr,yr must hold the result
-   (at most nint(sum e)+1 nodes).  *npr = 0 means "delete". */
-int hyo_renode(const hyo_model *m, const double *xd0, const double *yd0, int npd, double *xr0,
-               double *yr0, int *npr_out) {
+/* contours.f90:687-933.  xd,yd have npd entries; xr,yr hold at most cap
+   nodes (HYO_ERR_CAPACITY if the result would not fit; the reference writes
+   into fixed arrays without a check).  *npr = 0 means "delete". */
+int hyo_renode_cap(const hyo_model *m, const double *xd0, const double *yd0, int npd,
+                   double *xr0, double *yr0, int64_t cap, int *npr_out) {
   double ellx = m->p.ellx, elly = m->p.elly, hlxi = m->hlxi, hlyi = m->hlyi;
   double dmsq = m->dmsq, elf = m->elf, dmi = m->dmi, densf = m->densf;
   const double f12 = 0.5, f13 = 1.0 / 3.0, f16 = 1.0 / 6.0;
@@ -261,6 +263,11 @@ int hyo_renode(const hyo_model *m, const double *xd0, const double *yd0, int npd
     npr = 0;
     goto done;
   }
+  if (ncorn <= 1 && npr > cap) {
+    npr = 0;
+    status = HYO_ERR_CAPACITY;
+    goto done;
+  }
 #define PLACE(im)                                                        \
   do {                                                                   \
     double p = 1.0 - acc / e[i];                                         \
@@ -316,6 +323,11 @@ int hyo_renode(const hyo_model *m, const double *xd0, const double *yd0, int npd
       }
       if (sm > small) {
         int npseg = (int)lround(sm) + 1;
+        if ((int64_t)npr + npseg > cap) {
+          npr = 0;
+          status = HYO_ERR_CAPACITY;
+          goto done;
+        }
         double fac = (double)npseg / sm;
         i = ibeg;
         while (i != iend) {
@@ -349,6 +361,11 @@ done:
   free(next);
   free(corner);
   return status;
+}
+
+int hyo_renode(const hyo_model *m, const double *xd, const double *yd, int npd, double *xr,
+               double *yr, int *npr) {
+  return hyo_renode_cap(m, xd, yd, npd, xr, yr, INT64_MAX, npr);
 }
 
 /* contours.f90:350-683 */
@@ -560,7 +577,7 @@ int hyo_surgery(hyo_model *m) {
       }
       int npnew = npd;
       if (npd > 3) {
-        int rs = hyo_renode(m, xd, yd, npd, &xa[nptq + 1], &ya[nptq + 1], &npnew);
+        int rs = hyo_renode_cap(m, xd, yd, npd, &xa[nptq + 1], &ya[nptq + 1], npm - nptq, &npnew);
         if (rs) { status = rs; goto out; }
       }
       npq[nq] = npnew;

@@ -14,10 +14,14 @@
  *   - stochastic and narrow-band forcing and thermal damping are not
  *     supported (ivor=0, fbase=0, rtherm=0 in every configuration).
  *
- * Pinning: a steady shear psi=cos(y) advects nodes along the closed-form
- * trajectory; energy/enstrophy drift stays within the dissipation bound;
- * one advance() is compared bit-for-bit with the same sequence of calls
- * into the amdflang build of the reference modules (oracle/_ref).
+ * Pinning (evolution.f90 itself does not compile here, so these routines
+ * cannot be run against the reference build): adapt, source and energy
+ * against closed forms on one- and two-mode fields; advance against
+ * textbook RK4 (no damping) and Lawson's integrating-factor RK4 driven
+ * stage by stage through the oracle's own right-hand side; init, prepare
+ * and inversion as compositions of the pinned con2grid, ptospc, spctop and
+ * main_invert.  The RHS pieces those use (con2grid, velint, FFTs,
+ * main_invert, gradient) are compared with the reference build.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -253,8 +253,9 @@ hyo_model *hyo_create(const hyo_params *p) {
   m->xtrig = (double *)xcalloc(2 * (size_t)nx, sizeof(double));
   m->ytrig = (double *)xcalloc(2 * (size_t)ny, sizeof(double));
   /* capacities: constants.f90:95-101 scaled to a run-time default */
-  m->npm = p->npm > 0 ? p->npm : (int64_t)64 * nx * ny + 4096;
-  m->nm = m->npm / 20 + m->npm / 200 + 16;
+  /* constants.f90:35-37 */
+  m->npm = p->npm > 0 ? p->npm : (int64_t)500 * nx * ny;
+  m->nm = m->npm / 20 + m->npm / 200;
   m->xq = (double *)xcalloc(m->npm, sizeof(double));
   m->yq = (double *)xcalloc(m->npm, sizeof(double));
   m->nextq = (int *)xcalloc(m->npm, sizeof(int));

@@ -4,7 +4,7 @@ the container that holds /root/reference)."""
 import numpy as np
 import pytest
 
-from tests.helpers import bits_equal
+from tests.helpers import TANGLES, bits_equal, tangle_contours
 
 NG = 32
 
@@ -54,6 +54,21 @@ def test_recontour_con2grid_surgery(pair, seed):
     x = (1 - 1e-12) * (x - L * np.trunc(x / np.pi))
     y = (1 - 1e-12) * (y - L * np.trunc(y / np.pi))
     c = dict(a, x=x, y=y, qjump=qj)
+    o.set_contours(c)
+    r.set_contours(c)
+    o.surgery()
+    r.surgery()
+    a, b = o.contours(), r.contours()
+    for k in ("x", "y", "nextq", "i1q", "i2q", "npq", "indq"):
+        assert bits_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("seed,ncirc,rmin,rmax", TANGLES)
+def test_surgery_tangles(pair, seed, ncirc, rmin, rmax):
+    """Surgery on tangles of crossing circles (order-dependent reconnections
+    and deletions) against the reference build."""
+    o, r = pair
+    c = tangle_contours(o.getd("dm"), np.random.default_rng(seed), ncirc, (-2, 1, 3), rmin, rmax)
     o.set_contours(c)
     r.set_contours(c)
     o.surgery()
