@@ -36,24 +36,75 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(name, N, P):
-    """Algorithmic HBM bytes per launch of a kernel group (logical fields read
-    and written once, at their storage precision; fine-grid and FFT-internal
-    passes excluded)."""
-    s = 8
-    return {
-        # one 1D batch pass of a 2D transform: read and write one field
-        "fft_ptospc": 2 * 2 * s * N,          # two passes per ptospc call
-        "fft_spctop": 2 * 2 * s * N,
-        # con2grid: read node x,y,next (20 B/node), write qc
-        "con2grid": 20 * P + s * N,
-        "velint": 32 * P,
-        "combine_qq": 5 * s * N,
-        "advect_prod": 5 * s * N,
-        "source_finish": 5 * s * N,
-        "seqsum": s * N,
-        "adapt_reduce": 4 * s * N,
-    }.get(name)
+# HIP-event record name -> kernel symbol as rocprofv3 reports it
+KERNEL_SYMBOL = {"k:fft_fwd": "fft_kernel<N, true>", "k:fft_rev": "fft_kernel<N, false>"}
+
+
+def traffic_from_profile(kernel, ng):
+    """HBM bytes per launch of `kernel` at this ng, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel
+    of known size; see profiles/), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    rec = d.get(f"{kernel}@ng{ng}")
+    return None if rec is None else rec["traffic_bytes"]
+
+
+COPY_BYTES = 256 << 20  # each of read and write of the calibration copy
+
+
+def copy_calibration():
+    """A plain device copy of known size (read COPY_BYTES, write COPY_BYTES):
+    the reference point that scales the FETCH_SIZE/WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+
+
+def pmc_summary(fetch_dir, write_dir, ng, out_json):
+    """Per-launch HBM traffic of the FFT kernels from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the same
+    counter on the calibration copy."""
+    import csv
+    import glob
+
+    def per_kernel(d, counter):
+        vals = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    k = row["Kernel_Name"]
+                    key = ("copy" if "elementwise" in k or "mul" in k.lower() else
+                           "fft_kernel<N, true>" if "fft_kernel<" in k and "true>" in k else
+                           "fft_kernel<N, false>" if "fft_kernel<" in k and "false>" in k else None)
+                    if key:
+                        vals.setdefault(key, []).append(float(row["Counter_Value"]))
+        return {k: float(np.mean(v)) for k, v in vals.items()}
+
+    fe = per_kernel(fetch_dir, "FETCH_SIZE")
+    wr = per_kernel(write_dir, "WRITE_SIZE")
+    kib = COPY_BYTES / 1024.0
+    ff = kib / fe["copy"]
+    fw = kib / wr["copy"]
+    out = {"calibration": {"copy_bytes": COPY_BYTES, "fetch_kib": fe["copy"], "write_kib": wr["copy"],
+                           "fetch_factor": ff, "write_factor": fw}}
+    N = ng * ng
+    for k in ("fft_kernel<N, true>", "fft_kernel<N, false>"):
+        if k in fe and k in wr:
+            t = (fe[k] * ff + wr[k] * fw) * 1024.0
+            out[f"{k}@ng{ng}"] = {"fetch_kib": fe[k], "write_kib": wr[k], "traffic_bytes": round(t),
+                                  "algorithmic_bytes": 16 * N}
+    with open(out_json, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
 
 
 def main():
@@ -65,7 +116,14 @@ def main():
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run the calibration copy (for rocprofv3 --pmc runs)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc runs into profiles/pmc_traffic.json")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary, args.ng, os.path.join(ROOT, "profiles", "pmc_traffic.json"))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs one flow on one GPU; multi-GPU runs are not part of it "
               "(use --gpus 1)", file=sys.stderr)
@@ -124,19 +182,25 @@ def main():
     groups = sorted(tot.items(), key=lambda kv: -kv[1])
     for name, ms in groups:
         log(f"  {name:16s} {ms / args.steps:8.3f} ms/step  {cnt[name] // args.steps} calls/step")
-    dom = None
-    for name, ms in groups:
-        if algorithmic_bytes(name, N, P) is not None:
-            dom = name
-            break
+    # roofline: the single kernel ("k:" records, one record per launch) with
+    # the largest share of the timed region; its algorithmic bytes per launch
+    # are recorded with each launch
+    kbytes = {}
+    for name, ms, by in recs:
+        if name.startswith("k:"):
+            kbytes.setdefault(name, []).append(by)
+    dom = max(kbytes, key=lambda k: tot[k])
     avg_ms = tot[dom] / cnt[dom]
-    alg = algorithmic_bytes(dom, N, P)
+    alg = float(np.mean(kbytes[dom]))
     achieved = alg / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+    roofline = {"bound": "hbm", "kernel": KERNEL_SYMBOL.get(dom, dom), "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_launch_ms": round(avg_ms, 4),
+                "traffic": traffic_from_profile(KERNEL_SYMBOL.get(dom, dom), ng),
+                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": cnt[dom] / args.steps,
                 "algorithmic_bytes_per_launch": int(alg)}
 
+    if args.copy_calib:
+        copy_calibration()
     cpu = None
     if not args.no_cpu:
         try:

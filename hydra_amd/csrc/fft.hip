@@ -31,7 +31,7 @@ struct FftArgs {
   const double *dk;     // filtered wavenumbers for a fused derivative
   const double *mulk;   // optional per-element multiplier applied before the
                         // derivative (unused: kept null)
-  int n, m, V;
+  int n, m, V, lgV;
   int in_rows, out_rows;
   int pre;              // 0 none, 1 derivative across vectors, 2 along elements
   int n2, n4;
@@ -252,28 +252,39 @@ __device__ __forceinline__ void rev_r2(const double *__restrict__ a, double *__r
 
 // ---------------------------------------------------------------------------
 
-template <bool FWD>
+// N (the transform length) is a compile-time constant so that the stage
+// loops unroll and every index division becomes a shift or a multiply; the
+// batch V (a power of two) is a run-time shift.  Blocks are mapped so that
+// the blocks one XCD runs hold neighbouring vectors (shared L2 lines on the
+// strided side).
+template <int N, bool FWD>
 __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
   extern __shared__ double lds[];
-  const int n = g.n, m = g.m, V = g.V;
-  const int v0 = blockIdx.x * V;
-  double *bufA = lds, *bufB = lds + V * n;
-  const int tot = V * n;
-  const int nhalf = n / 2;
+  constexpr int L2N = __builtin_ctz(N);
+  constexpr int n2 = L2N & 1, n4 = L2N >> 1;
+  const int m = g.m, V = g.V, lv = g.lgV;
+  const int nb = gridDim.x;
+  const int bid = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
+  const int v0 = bid * V;
+  double *bufA = lds, *bufB = lds + V * N;
+  double *tcs = lds + 2 * V * N, *tsn = tcs + N;
+  const int tot = V * N;
+  constexpr int nhalf = N / 2;
+  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) tcs[i] = g.trig[i];
   // ---- load
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     int v, e;
     if (g.in_rows) {
-      v = idx / n;
-      e = idx - v * n;
+      v = idx / N;
+      e = idx - v * N;
     } else {
-      e = idx / V;
-      v = idx - e * V;
+      e = idx >> lv;
+      v = idx & (V - 1);
     }
     const int vg = v0 + v;
     double x;
     if (FWD) {
-      x = g.in_rows ? g.in[(int64_t)vg * n + e] : g.in[vg + (int64_t)m * e];
+      x = g.in_rows ? g.in[(int64_t)vg * N + e] : g.in[vg + (int64_t)m * e];
     } else {
       // optional fused derivative (sta2dfft.f90:169-245)
       if (g.pre == 2) {
@@ -281,8 +292,8 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
         if (e == 0 || e == nhalf) {
           x = 0.0;
         } else {
-          const int em = n - e;
-          const double var = g.in_rows ? g.in[(int64_t)vg * n + em] : g.in[vg + (int64_t)m * em];
+          const int em = N - e;
+          const double var = g.in_rows ? g.in[(int64_t)vg * N + em] : g.in[vg + (int64_t)m * em];
           x = (e < nhalf) ? -g.dk[2 * e - 1] * var : g.dk[2 * em - 1] * var;
         }
       } else if (g.pre == 1) {
@@ -292,54 +303,57 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
           x = 0.0;
         } else {
           const int vm = m - vg;
-          const double var = g.in_rows ? g.in[(int64_t)vm * n + e] : g.in[vm + (int64_t)m * e];
+          const double var = g.in_rows ? g.in[(int64_t)vm * N + e] : g.in[vm + (int64_t)m * e];
           x = (vg < mh) ? -g.dk[2 * vg - 1] * var : g.dk[2 * vm - 1] * var;
         }
       } else {
-        x = g.in_rows ? g.in[(int64_t)vg * n + e] : g.in[vg + (int64_t)m * e];
+        x = g.in_rows ? g.in[(int64_t)vg * N + e] : g.in[vg + (int64_t)m * e];
       }
       // revfft pre-processing, stafft.f90:311-325
       if (e >= nhalf + 1) x = -x;
       if (e == 0 || e == nhalf) x = 0.5 * x;
     }
-    bufA[v * n + e] = x;
+    bufA[v * N + e] = x;
   }
   __syncthreads();
   // ---- stages
   double *a = bufA, *b = bufB;
-  const double *trig = g.trig;
   if (FWD) {
-    int rem = n, cum = 1;
-    for (int s = 0; s < g.n2; s++) {
+    int rem = N, cum = 1;
+#pragma unroll
+    for (int s = 0; s < n2; s++) {
       rem /= 2;
       const int iloc = (rem - 1) * 2 * cum;
-      fwd_r2(a, b, n, V, rem, cum, trig + iloc, trig + n + iloc);
+      fwd_r2(a, b, N, V, rem, cum, tcs + iloc, tsn + iloc);
       __syncthreads();
       double *t = a; a = b; b = t;
       cum *= 2;
     }
-    for (int s = 0; s < g.n4; s++) {
+#pragma unroll
+    for (int s = 0; s < n4; s++) {
       rem /= 4;
       const int iloc = (rem - 1) * 4 * cum;
-      fwd_r4(a, b, n, V, rem, cum, trig + iloc, trig + n + iloc);
+      fwd_r4(a, b, N, V, rem, cum, tcs + iloc, tsn + iloc);
       __syncthreads();
       double *t = a; a = b; b = t;
       cum *= 4;
     }
   } else {
-    int cum = 1, rem = n;
-    for (int s = 0; s < g.n4; s++) {
+    int cum = 1, rem = N;
+#pragma unroll
+    for (int s = 0; s < n4; s++) {
       rem /= 4;
       const int iloc = (cum - 1) * 4 * rem;
-      rev_r4(a, b, n, V, cum, rem, trig + iloc, trig + n + iloc);
+      rev_r4(a, b, N, V, cum, rem, tcs + iloc, tsn + iloc);
       __syncthreads();
       double *t = a; a = b; b = t;
       cum *= 4;
     }
-    for (int s = 0; s < g.n2; s++) {
+#pragma unroll
+    for (int s = 0; s < n2; s++) {
       rem /= 2;
       const int iloc = (cum - 1) * 2 * rem;
-      rev_r2(a, b, n, V, cum, rem, trig + iloc, trig + n + iloc);
+      rev_r2(a, b, N, V, cum, rem, tcs + iloc, tsn + iloc);
       __syncthreads();
       double *t = a; a = b; b = t;
       cum *= 2;
@@ -350,15 +364,33 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     int v, e;
     if (g.out_rows) {
-      v = idx / n;
-      e = idx - v * n;
-      g.out[(int64_t)(v0 + v) * n + e] = a[v * n + e] * post;
+      v = idx / N;
+      e = idx - v * N;
+      g.out[(int64_t)(v0 + v) * N + e] = a[v * N + e] * post;
     } else {
-      e = idx / V;
-      v = idx - e * V;
-      g.out[(v0 + v) + (int64_t)m * e] = a[v * n + e] * post;
+      e = idx >> lv;
+      v = idx & (V - 1);
+      g.out[(v0 + v) + (int64_t)m * e] = a[v * N + e] * post;
     }
   }
+}
+
+template <int N>
+static int launch_n(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftArgs &g) {
+  // more than 64 KiB of LDS per workgroup must be requested (gfx950: 160 KiB)
+  static bool raised = false;
+  if (!raised && shm > 65536) {
+    HYD_HIP(hipFuncSetAttribute((const void *)fft_kernel<N, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    HYD_HIP(hipFuncSetAttribute((const void *)fft_kernel<N, false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    raised = true;
+  }
+  if (fwd)
+    hipLaunchKernelGGL((fft_kernel<N, true>), grid, dim3(256), shm, s, g);
+  else
+    hipLaunchKernelGGL((fft_kernel<N, false>), grid, dim3(256), shm, s, g);
+  return HYD_OK;
 }
 
 static int fft_launch(Ctx &c, bool fwd, const double *in, double *out, int n, int m,
@@ -377,18 +409,32 @@ static int fft_launch(Ctx &c, bool fwd, const double *in, double *out, int n, in
   if (V > m) V = m;
   while (m % V) V /= 2;
   g.V = V;
+  g.lgV = __builtin_ctz(V);
   g.in_rows = in_rows;
   g.out_rows = out_rows;
   g.pre = pre;
   g.n2 = c.plan.n2;
   g.n4 = c.plan.n4;
   g.post = post;
-  const size_t shm = sizeof(double) * 2 * (size_t)V * n;
-  const int grid = m / V;
-  if (fwd)
-    hipLaunchKernelGGL(fft_kernel<true>, dim3(grid), dim3(256), shm, c.s, g);
-  else
-    hipLaunchKernelGGL(fft_kernel<false>, dim3(grid), dim3(256), shm, c.s, g);
+  const size_t shm = sizeof(double) * (2 * (size_t)V * n + 2 * (size_t)n);
+  const dim3 grid(m / V);
+  // one launch reads and writes m vectors of n doubles
+  KTimer tk(c, fwd ? "k:fft_fwd" : "k:fft_rev", 16 * (int64_t)n * m);
+  switch (n) {
+    case 8: HYD_CHECK(launch_n<8>(fwd, grid, shm, c.s, g)); break;
+    case 16: HYD_CHECK(launch_n<16>(fwd, grid, shm, c.s, g)); break;
+    case 32: HYD_CHECK(launch_n<32>(fwd, grid, shm, c.s, g)); break;
+    case 64: HYD_CHECK(launch_n<64>(fwd, grid, shm, c.s, g)); break;
+    case 128: HYD_CHECK(launch_n<128>(fwd, grid, shm, c.s, g)); break;
+    case 256: HYD_CHECK(launch_n<256>(fwd, grid, shm, c.s, g)); break;
+    case 512: HYD_CHECK(launch_n<512>(fwd, grid, shm, c.s, g)); break;
+    case 1024: HYD_CHECK(launch_n<1024>(fwd, grid, shm, c.s, g)); break;
+    case 2048: HYD_CHECK(launch_n<2048>(fwd, grid, shm, c.s, g)); break;
+    case 4096: HYD_CHECK(launch_n<4096>(fwd, grid, shm, c.s, g)); break;
+    default:
+      set_error("fft: transform length must be a power of two in [8, 4096]");
+      return HYD_EFACTOR;
+  }
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }

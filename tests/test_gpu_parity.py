@@ -89,13 +89,42 @@ def test_sum_sequential_bitwise(hyd_cache):
     cases = [rng.standard_normal(4096 * 4) * 10 + 3, rng.standard_normal(4096 * 4),
              np.round(rng.standard_normal(12345) * 64) / 64 + 2.0 ** -30,
              rng.integers(-5, 6, 9999) * 0.1, rng.standard_normal(777) ** 2,
-             np.full(5000, 0.1), np.array([1.0, 1e-16, 1e-16, 1e-16, -1.0])]
+             np.full(5000, 0.1), np.array([1.0, 1e-16, 1e-16, 1e-16, -1.0]),
+             rng.standard_normal(5000) * 1e-300,
+             np.array([1e16 if i % 2 == 0 else -1e16 + 1.5 for i in range(3001)]),
+             np.r_[rng.standard_normal(8000) * 1e8, -rng.standard_normal(8000) * 1e8],
+             np.r_[[2.0 ** 40], np.tile([0.5 ** 12 * 3, -0.5 ** 12], 6000)],
+             np.r_[[-0.0, 0.0, -0.0], rng.standard_normal(600)], np.array([-0.0]),
+             np.array([0.0, -0.0, -0.0]), np.zeros(0)]
     for a in cases:
         ref = 0.0
         for x in a:
             ref = ref + float(x)
         got = h.sum_sequential(a)
         assert np.float64(got).view(np.int64) == np.float64(ref).view(np.int64), (len(a), got, ref)
+
+
+def test_sum_sequential_long_bitwise():
+    """Long sums (up to the 4 ng^2 work space at ng=256) of zero-mean,
+    tie-rich data shaped like the con2grid field: many binade changes,
+    exact ties, chunk counts that are not multiples of 64."""
+    from hydra_amd import Hydra
+    h = Hydra(256)
+    rng = np.random.default_rng(5)
+    ng = 256
+    x = np.linspace(-np.pi, np.pi, ng, endpoint=False)
+    f = np.add.outer(np.sin(3 * x), np.cos(2 * x)) * 5 + rng.standard_normal((ng, ng)) * 0.1
+    qj = 0.37
+    lev = np.round(f / qj) * qj  # level steps: many exact ties
+    cases = [lev.ravel(), (lev - lev.mean()).ravel(), f.ravel(),
+             np.tile(lev.ravel(), 4)[: 4 * ng * ng - 77], rng.standard_normal(4 * ng * ng)]
+    for a in cases:
+        ref = 0.0
+        for v in a.tolist():
+            ref = ref + v
+        got = h.sum_sequential(a)
+        assert np.float64(got).view(np.int64) == np.float64(ref).view(np.int64), (len(a), got, ref)
+    h.close()
 
 
 @pytest.mark.parametrize("ng", [32, 64])
