@@ -90,6 +90,7 @@ def lib():
         L.hyd_renode.argtypes = [C.c_void_p, _D, _D, C.c_int32, _D, _D, C.c_int32,
                                  C.POINTER(C.c_int32)]
         L.hyd_run.argtypes = [C.c_void_p, C.c_int32]
+        L.hyd_get_tuning.argtypes = [C.c_void_p, _I]
         L.hyd_set_timing.argtypes = [C.c_void_p, C.c_int32]
         L.hyd_kernel_times.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
                                        np.ctypeslib.ndpointer(dtype=np.float64),
@@ -278,12 +279,19 @@ class Hydra:
         """caps.f90:83-91 for nsteps advances (recontour, init, advect)."""
         _check(self.L.hyd_run(self.h, int(nsteps)))
 
+    def tuning(self):
+        """FFT batch per layout class picked by the autotuner at init."""
+        v = np.zeros(4, dtype=np.int32)
+        _check(self.L.hyd_get_tuning(self.h, v))
+        return v.tolist()
+
     def sync(self):
         _check(self.L.hyd_sync(self.h))
 
     # -- timing -----------------------------------------------------------------
-    def set_timing(self, on=True):
-        _check(self.L.hyd_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, level=2):
+        """0 off, 1 single-kernel records only, 2 all records."""
+        _check(self.L.hyd_set_timing(self.h, int(level)))
 
     def kernel_times(self, max_records=1 << 16):
         names = C.create_string_buffer(32 * max_records)

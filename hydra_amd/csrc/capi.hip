@@ -15,10 +15,9 @@ static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
 
 int inversion(Ctx &c);
-int adapt(Ctx &c);
 
 KTimer::KTimer(Ctx &ctx, const char *name, int64_t bytes) : c(ctx) {
-  if (!c.timing) return;
+  if (!c.timing || (c.timing == 1 && strncmp(name, "k:", 2) != 0)) return;
   Ctx::KRec r;
   r.name = name;
   r.bytes = bytes;
@@ -95,6 +94,8 @@ static void free_ctx(Ctx *c) {
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->hsc) hipHostFree(c->hsc);
+  if (c->exp_host) hipHostFree(c->exp_host);
+  if (c->ev_sc) hipEventDestroy(c->ev_sc);
   free_congen(*c);
   for (auto &r : c->krec) {
     hipEventDestroy(r.a);
@@ -225,6 +226,13 @@ static int create(const hyd_params *p, hyd_ctx **out) {
     return HYD_ENOMEM;
   }
   memset(c->hsc, 0, sizeof(Scalars));
+  if (hipHostMalloc((void **)&c->exp_host, sizeof(double) * std::max<size_t>(c->ht.qdiss_u.size(), 1),
+                    hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_sc, hipEventDisableTiming) != hipSuccess) {
+    set_error("hipHostMalloc / hipEventCreate failed");
+    free_ctx(c);
+    return HYD_ENOMEM;
+  }
   TRY(upload(c->green, c->ht.green.data(), N, c->s));
   TRY(upload(c->flo, c->ht.flo.data(), N, c->s));
   TRY(upload(c->fhi, c->ht.fhi.data(), N, c->s));
@@ -243,6 +251,7 @@ static int create(const hyd_params *p, hyd_ctx **out) {
       return HYD_EHIP;
     }
   }
+  TRY(fft_autotune(*c));
 #undef TRY
   *out = (hyd_ctx *)c;
   return HYD_OK;
@@ -696,6 +705,12 @@ int hyd_run(hyd_ctx *h, int32_t nsteps) {
   return HYD_OK;
 }
 
+int hyd_get_tuning(hyd_ctx *h, int32_t *v) {
+  Ctx &c = *C(h);
+  for (int k = 0; k < 4; k++) v[k] = c.fft_v[k];
+  return HYD_OK;
+}
+
 int hyd_sync(hyd_ctx *h) {
   HYD_HIP(hipStreamSynchronize(C(h)->s));
   return HYD_OK;
@@ -709,7 +724,7 @@ int hyd_set_timing(hyd_ctx *h, int32_t on) {
     hipEventDestroy(r.b);
   }
   c.krec.clear();
-  c.timing = on != 0;
+  c.timing = on < 0 ? 0 : (on > 2 ? 2 : on);
   return HYD_OK;
 }
 

@@ -404,10 +404,16 @@ static int fft_launch(Ctx &c, bool fwd, const double *in, double *out, int n, in
   g.mulk = nullptr;
   g.n = n;
   g.m = m;
-  int V = 4096 / n;
-  if (V < 1) V = 1;
-  if (V > m) V = m;
-  while (m % V) V /= 2;
+  // batch per workgroup: the autotuned value of this layout class when it
+  // divides m, else 4096/n
+  const int cls = fwd ? (out_rows ? 0 : 1) : (in_rows ? 3 : 2);
+  int V = c.fft_v[cls];
+  if (V < 1 || V > m || m % V) {
+    V = 4096 / n;
+    if (V < 1) V = 1;
+    if (V > m) V = m;
+    while (m % V) V /= 2;
+  }
   g.V = V;
   g.lgV = __builtin_ctz(V);
   g.in_rows = in_rows;
@@ -437,6 +443,50 @@ static int fft_launch(Ctx &c, bool fwd, const double *in, double *out, int n, in
   }
   HYD_HIP(hipGetLastError());
   return HYD_OK;
+}
+
+// Autotuner: for each layout class (forward strided->rows, forward
+// strided->strided, reverse strided->strided, reverse rows->strided) time
+// the candidate batches V on scratch fields and keep the fastest.  Every V
+// gives bit-identical results (each vector's arithmetic is the same).
+int fft_autotune(Ctx &c) {
+  const int n = c.c.nx;
+  hipEvent_t e0, e1;
+  HYD_HIP(hipEventCreate(&e0));
+  HYD_HIP(hipEventCreate(&e1));
+  const bool fw[4] = {true, true, false, false};
+  const int inr[4] = {0, 0, 0, 1}, outr[4] = {1, 0, 0, 0};
+  int rc = HYD_OK;
+  for (int cls = 0; cls < 4 && rc == HYD_OK; cls++) {
+    float best = 1e30f;
+    int bestV = 0;
+    for (int V = 1; V <= 32 && V <= n; V *= 2) {
+      if ((2 * (size_t)V * n + 2 * (size_t)n) * sizeof(double) > 160 * 1024) break;
+      c.fft_v[cls] = V;
+      const double *trig = fw[cls] ? c.xtrig : c.ytrig;
+      rc = fft_launch(c, fw[cls], c.wa, c.wb, n, n, inr[cls], outr[cls], 0, nullptr, 1.0, trig);
+      if (rc != HYD_OK) break;
+      if (hipEventRecord(e0, c.s) != hipSuccess) { rc = HYD_EHIP; break; }
+      for (int r = 0; r < 5 && rc == HYD_OK; r++)
+        rc = fft_launch(c, fw[cls], c.wa, c.wb, n, n, inr[cls], outr[cls], 0, nullptr, 1.0, trig);
+      if (rc != HYD_OK) break;
+      float ms = 0.0f;
+      if (hipEventRecord(e1, c.s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = HYD_EHIP;
+        break;
+      }
+      if (ms < best) {
+        best = ms;
+        bestV = V;
+      }
+    }
+    c.fft_v[cls] = bestV;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != HYD_OK) set_error("fft autotuning failed");
+  return rc;
 }
 
 // ptospc, sta2dfft.f90:103-132: x-transform (ny strided vectors) written as

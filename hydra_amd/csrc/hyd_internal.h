@@ -91,6 +91,7 @@ struct Ctx {
   Consts c;
   HostTables ht;
   FftPlan plan;
+  int fft_v[4] = {0, 0, 0, 0};  // autotuned FFT batch per layout class
   hipStream_t s = nullptr;
   int dev = 0;
   int64_t N = 0;        // ng*ng
@@ -135,13 +136,14 @@ struct Ctx {
   Scalars *dsc = nullptr;   // device
   Scalars *hsc = nullptr;   // pinned host mirror
   double *dsum = nullptr;   // device scratch for sequential sums (8 slots)
-  std::vector<double> exp_host;  // host-evaluated exp table of the step
+  double *exp_host = nullptr;    // pinned: host-evaluated exp table of the step
+  hipEvent_t ev_sc = nullptr;    // adapt's scalars have reached the host mirror
   void *seq_work = nullptr; // chunk table of seqsum
   int64_t seq_cap = 0;      // chunks it holds
   // work buffers of recontour / renode, allocated on first use
   std::map<std::string, DBuf> cg_bufs;
   // timing
-  bool timing = false;
+  int timing = 0;  // 0 off, 1 single-kernel ("k:") records only, 2 all records
   struct KRec { std::string name; hipEvent_t a, b; int64_t bytes; };
   std::vector<KRec> krec;
 };
@@ -153,6 +155,7 @@ int fft_ptospc(Ctx &c, const double *rvar, double *svar);
 int fft_spctop(Ctx &c, const double *svar, double *rvar, int pre, double post_sign,
                double *tmp);
 int fft_forward_1d(Ctx &c, double *x, int m);   // forfft(m, ng, x)
+int fft_autotune(Ctx &c);
 int fft_reverse_1d(Ctx &c, double *x, int m);   // revfft(m, ng, x)
 
 int main_invert_dev(Ctx &c, const double *qq, double *uu, double *vv, double *pp, double *tmp);

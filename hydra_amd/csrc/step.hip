@@ -286,14 +286,23 @@ int adapt(Ctx &c) {
   a.tcsave = c.p.tcsave;
   hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, mx, a);
   HYD_HIP(hipGetLastError());
-  // integrating factors, evolution.f90:603-605
+  // the integrating factors need dt and zzrms on the host: start the copy
+  // now, evaluate the exponentials in adapt_finish while the GPU computes
+  // the sources of stage 1 (which do not use them)
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
-  HYD_HIP(hipStreamSynchronize(c.s));
+  HYD_HIP(hipEventRecord(c.ev_sc, c.s));
+  return HYD_OK;
+}
+
+// integrating factors, evolution.f90:603-605: epq = exp(dt/2*zzrms*qdiss)
+// evaluated on the host (the reference's libm exp) over the distinct qdiss
+// values, expanded on the device; emq = 1/epq.
+int adapt_finish(Ctx &c) {
+  const int64_t N = c.N;
+  HYD_HIP(hipEventSynchronize(c.ev_sc));
   const double dfac = c.hsc->dt2 * c.hsc->zzrms;
-  std::vector<double> &tab = c.exp_host;
-  tab.resize(c.ht.qdiss_u.size());
-  host_exp_table(c.ht.qdiss_u, dfac, tab.data());
-  HYD_HIP(hipMemcpyAsync(c.qdiss_exp, tab.data(), sizeof(double) * tab.size(),
+  host_exp_table(c.ht.qdiss_u, dfac, c.exp_host);
+  HYD_HIP(hipMemcpyAsync(c.qdiss_exp, c.exp_host, sizeof(double) * c.ht.qdiss_u.size(),
                          hipMemcpyHostToDevice, c.s));
   hipLaunchKernelGGL(k_expand_epq, grid1(N), dim3(256), 0, c.s, c.epq, c.emq, c.qdiss_map,
                      c.qdiss_exp, N);
@@ -354,6 +363,7 @@ int advance(Ctx &c) {
   HYD_CHECK(adapt(c));
   HYD_CHECK(source(c, 0));
   HYD_CHECK(rk_nodes(c, 1));
+  HYD_CHECK(adapt_finish(c));
   HYD_CHECK(rk_fields(c, 1));
   // stage 2
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);

@@ -163,8 +163,17 @@ int hyd_run(hyd_ctx *ctx, int32_t nsteps);
 int hyd_sync(hyd_ctx *ctx);
 /* Kernel-time accounting of the dominant kernels, measured with HIP events
    on the context's stream: returns the number of records written. */
+/* No reference counterpart: the FFT batch (vectors per workgroup) that the
+   autotuner picked at hyd_init for the layout classes forward
+   strided->rows, forward strided->strided, reverse strided->strided and
+   reverse rows->strided (v[0..3]). */
+int hyd_get_tuning(hyd_ctx *ctx, int32_t *v);
+/* No reference counterpart: per-launch timing with HIP events on the
+   library's stream.  Level 0 off, 1 the single kernels whose records start
+   with "k:" (one record per launch), 2 every timed group as well.
+   hyd_kernel_times returns the records (names: max*32 bytes). */
 int hyd_kernel_times(hyd_ctx *ctx, int32_t max, char *names, double *ms, int64_t *bytes);
-int hyd_set_timing(hyd_ctx *ctx, int32_t on);
+int hyd_set_timing(hyd_ctx *ctx, int32_t level);
 
 #ifdef __cplusplus
 }
