@@ -25,6 +25,8 @@
 
 #include <algorithm>
 
+#include <hipcub/hipcub.hpp>
+
 #include "hyd_internal.h"
 
 namespace hyd {
@@ -328,52 +330,208 @@ __global__ void cg_successor(const int64_t *kib, const int32_t *lev_of_gid, int6
   atomicAdd(err, 1);
 }
 
-// congen.f90:258-293 for one level per thread.  Writes the crossing ids of
-// each closed contour consecutively into order[], and (start, length) of
-// each contour into cyc[].
-__global__ void cg_walk(const int64_t *succ, const int64_t *sad2, int8_t *used, int32_t *sadcnt,
-                        const int64_t *lev_base, const int64_t *lev_n, int nlev, int64_t *order,
-                        int64_t *cyc_start, int32_t *cyc_len, int64_t *cyc_base,
-                        int32_t *cyc_count, int32_t *err) {
+// congen.f90:258-293, the walk.  A box with one exit per level fixes the
+// successor of the crossing that enters it, so the crossings form
+// deterministic chains that end where a crossing enters a box with two
+// exits (a saddle, whose exit depends on the order of visits: icrtab is
+// last-in-first-out and noctab counts down, not decremented when the walk
+// closes, congen.f90:283).  Pointer jumping finds every chain, its length and
+// each crossing's position; a thread per level then walks the chains in the
+// reference's order -- start at the lowest unused crossing, which is the
+// lowest crossing of the first unused chain -- taking only the saddle
+// decisions sequentially; a last pass writes every crossing to its place in
+// the walk.  Chains without a saddle are closed loops on their own.
+
+// chain end (the saddle entry) and distance to it
+__global__ void cw_init(const int64_t *succ, const int64_t *sad2, int64_t n, int32_t *P,
+                        int32_t *E, int32_t *D) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  if (sad2[c] >= 0) {
+    P[c] = -1;
+    E[c] = (int32_t)c;
+    D[c] = 0;
+  } else {
+    P[c] = (int32_t)succ[c];
+    E[c] = -1;
+    D[c] = 1;
+  }
+}
+__global__ void cw_jump(const int32_t *P, const int32_t *E, const int32_t *D, int64_t n,
+                        int32_t *P2, int32_t *E2, int32_t *D2) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int32_t q = P[c];
+  if (q < 0) {
+    P2[c] = -1;
+    E2[c] = E[c];
+    D2[c] = D[c];
+  } else {
+    P2[c] = P[q];
+    E2[c] = E[q];
+    D2[c] = D[c] + D[q];
+  }
+}
+// closed loops without a saddle: minimum crossing by pointer jumping
+__global__ void cw_cyc_init(const int64_t *succ, const int32_t *E, int64_t n, int32_t *M,
+                            int32_t *Q) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  M[c] = (int32_t)c;
+  Q[c] = E[c] < 0 ? (int32_t)succ[c] : (int32_t)c;
+}
+__global__ void cw_cyc_min(const int32_t *M, const int32_t *Q, int64_t n, int32_t *M2,
+                           int32_t *Q2) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int32_t q = Q[c];
+  M2[c] = min(M[c], M[q]);
+  Q2[c] = Q[q];
+}
+// distance to the loop's last crossing (the one before its minimum)
+__global__ void cw_cyc_rank_init(const int64_t *succ, const int32_t *E, const int32_t *M,
+                                 int64_t n, int32_t *R, int32_t *Q) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  if (E[c] >= 0) {
+    R[c] = 0;
+    Q[c] = -1;
+    return;
+  }
+  const int32_t s = (int32_t)succ[c];
+  const bool last = s == M[c];
+  R[c] = last ? 0 : 1;
+  Q[c] = last ? -1 : s;
+}
+__global__ void cw_rank_jump(const int32_t *R, const int32_t *Q, int64_t n, int32_t *R2,
+                             int32_t *Q2) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int32_t q = Q[c];
+  if (q < 0) {
+    R2[c] = R[c];
+    Q2[c] = -1;
+  } else {
+    R2[c] = R[c] + R[q];
+    Q2[c] = Q[q];
+  }
+}
+// chain key (its end crossing, or a loop's minimum) and per-key tables
+struct ChainTab {
+  int32_t *key, *pos;       // per crossing
+  int32_t *hd, *len, *cmin;  // per key: head distance, length, lowest crossing
+  int8_t *loop, *used;       // per key
+  int64_t *aoff, *boff;      // per key: output offsets of the (up to two) pieces
+  int32_t *afrom;            // per key: first position of piece a
+};
+__global__ void cw_keys(const int32_t *E, const int32_t *D, const int32_t *M, const int32_t *R,
+                        int64_t n, ChainTab t) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const bool lp = E[c] < 0;
+  const int32_t k = lp ? M[c] : E[c];
+  t.key[c] = k;
+  if (lp) {
+    if (k == c) {
+      t.len[k] = R[c] + 1;
+      t.loop[k] = 1;
+    }
+  } else {
+    atomicMax(&t.hd[k], D[c]);
+    if (k == c) t.loop[k] = 0;
+  }
+  atomicMin(&t.cmin[k], (int32_t)c);
+}
+__global__ void cw_pos(const int32_t *D, const int32_t *R, int64_t n, ChainTab t, int8_t *start,
+                       const int32_t *lev_of_gid, int32_t *levcnt) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int32_t k = t.key[c];
+  if (t.loop[k]) {
+    t.pos[c] = t.len[k] - 1 - R[c];
+  } else {
+    t.pos[c] = t.hd[k] - D[c];
+    if (k == c) t.len[k] = t.hd[k] + 1;
+  }
+  const bool st = t.cmin[k] == (int32_t)c;
+  start[c] = st ? 1 : 0;
+  if (st) atomicAdd(&levcnt[lev_of_gid[c]], 1);
+}
+__global__ void cw_fill_tab(ChainTab t, int64_t n) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  t.hd[c] = 0;
+  t.len[c] = 0;
+  t.cmin[c] = 0x7fffffff;
+  t.loop[c] = 0;
+  t.used[c] = 0;
+}
+// starts (chain minima) in crossing order; soff[lv] = first start of level lv
+__global__ void cw_walk(const int32_t *starts, const int64_t *soff, const int32_t *scnt,
+                        const int64_t *succ, const int64_t *sad2, int32_t *sadcnt,
+                        const int64_t *lev_base, const int64_t *lev_n, int nlev, ChainTab t,
+                        int64_t *cyc_start, int32_t *cyc_len, int32_t *cyc_count, int32_t *err) {
   const int lv = blockIdx.x * blockDim.x + threadIdx.x;
   if (lv >= nlev) return;
-  const int64_t b = lev_base[lv], n = lev_n[lv];
-  int64_t w = b;                 // write cursor in order[]
-  int64_t cb = cyc_base[lv];     // contour slots of this level
+  const int64_t b = lev_base[lv], nn = lev_n[lv];
+  int64_t w = b;
   int32_t nc = 0;
-  for (int64_t icr = b; icr < b + n; icr++) {
-    if (used[icr]) continue;
+  for (int32_t i = 0; i < scnt[lv]; i++) {
+    const int32_t s = starts[soff[lv] + i];
+    const int32_t h = t.key[s];
+    if (t.used[h]) continue;
+    t.used[h] = 1;
     const int64_t start = w;
-    order[w++] = icr;
-    int64_t cur = icr;
-    int64_t guard = 0;
-    for (;;) {
-      int64_t nxt;
-      const int64_t e2 = sad2[cur];
-      if (e2 < 0) {
-        nxt = succ[cur];
-      } else {
-        // two exits: noctab counts down from 2 (icrtab last-in-first-out);
-        // the count is not decremented when the walk closes (congen.f90:283)
+    if (t.loop[h]) {
+      t.afrom[h] = 0;
+      t.aoff[h] = w;
+      w += t.len[h];
+    } else {
+      const int32_t ps = t.pos[s];
+      t.afrom[h] = ps;
+      t.aoff[h] = w;
+      w += t.len[h] - ps;
+      int32_t cur = h;
+      for (int64_t guard = 0;; guard++) {
+        const int64_t e2 = sad2[cur];
         const int32_t noc = sadcnt[e2];
-        nxt = noc == 2 ? e2 : succ[cur];
-        if (nxt != icr) sadcnt[e2] = noc - 1;
+        const int64_t nxt = noc == 2 ? e2 : succ[cur];
+        if (nxt == s) break;
+        sadcnt[e2] = noc - 1;
+        if (nxt < b || nxt >= b + nn || guard > nn || t.pos[nxt] != 0) {
+          atomicAdd(err, 1);
+          break;
+        }
+        const int32_t kn = t.key[nxt];
+        if (kn == h) {
+          t.boff[h] = w;
+          w += ps;
+          break;
+        }
+        if (t.used[kn] || t.loop[kn]) {
+          atomicAdd(err, 1);
+          break;
+        }
+        t.used[kn] = 1;
+        t.afrom[kn] = 0;
+        t.aoff[kn] = w;
+        w += t.len[kn];
+        cur = kn;
       }
-      if (nxt == icr) break;
-      if (nxt < b || nxt >= b + n || used[nxt] || ++guard > n) {
-        atomicAdd(err, 1);
-        break;
-      }
-      used[nxt] = 1;
-      order[w++] = nxt;
-      cur = nxt;
     }
-    used[icr] = 1;
-    cyc_start[cb + nc] = start;
-    cyc_len[cb + nc] = (int32_t)(w - start);
+    cyc_start[b + nc] = start;
+    cyc_len[b + nc] = (int32_t)(w - start);
     nc++;
   }
+  if (w != b + nn) atomicAdd(err, 1);
   cyc_count[lv] = nc;
+}
+__global__ void cw_expand(ChainTab t, int64_t n, const int32_t *err, int64_t *order) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n || *err) return;  // a failed walk leaves offsets unset
+  const int32_t k = t.key[c], p = t.pos[c];
+  const int64_t o = p >= t.afrom[k] ? t.aoff[k] + (p - t.afrom[k]) : t.boff[k] + p;
+  if (o >= 0 && o < n) order[o] = c;
 }
 
 __global__ void cg_gather(const int64_t *order, const double *xcr, const double *ycr, int64_t n,
@@ -677,15 +835,12 @@ int recontour(Ctx &c, const double *qq) {
   hipLaunchKernelGGL(cg_fill_i64, dim3((hcap + 255) / 256), dim3(256), 0, c.s, h.v1, hcap, -1);
   hipLaunchKernelGGL(cg_fill_i64, dim3((hcap + 255) / 256), dim3(256), 0, c.s, h.v2, hcap, -1);
   int64_t *succ, *sad2;
-  int8_t *used;
   int32_t *sadcnt, *errs;
   HYD_CHECK(dbuf(c, "succ", nalloc, &succ));
   HYD_CHECK(dbuf(c, "sad2", nalloc, &sad2));
-  HYD_CHECK(dbuf(c, "used", nalloc, &used));
   HYD_CHECK(dbuf(c, "sadcnt", nalloc, &sadcnt));
   HYD_CHECK(dbuf(c, "errs", 4, &errs));
   HYD_HIP(hipMemsetAsync(errs, 0, 16, c.s));
-  HYD_HIP(hipMemsetAsync(used, 0, nalloc, c.s));
   if (ntot > 0) {
     hipLaunchKernelGGL(cg_fill_i32, gb, dim3(256), 0, c.s, sadcnt, ntot, 2);
     hipLaunchKernelGGL(cg_hash_insert, gb, dim3(256), 0, c.s, kob, lev_of_gid, ntot, h, errs);
@@ -693,7 +848,8 @@ int recontour(Ctx &c, const double *qq) {
                        errs + 1);
   }
   HYD_HIP(hipGetLastError());
-  // ---- 5. walk
+  // ---- 5. walk: chains between saddles by pointer jumping, then the
+  // saddle decisions in the reference's order (one thread per level)
   int64_t *order, *cyc_start, *d_cyc_base;
   int32_t *cyc_len, *cyc_count;
   HYD_CHECK(dbuf(c, "order", nalloc, &order));
@@ -701,12 +857,96 @@ int recontour(Ctx &c, const double *qq) {
   HYD_CHECK(dbuf(c, "cyc_len", nalloc, &cyc_len));
   HYD_CHECK(dbuf(c, "cyc_base", nlev, &d_cyc_base));
   HYD_CHECK(dbuf(c, "cyc_count", nlev, &cyc_count));
-  // contour slots: a level has at most n/2 contours (each has >= 2 crossings)
   HYD_HIP(hipMemcpyAsync(d_cyc_base, lev_base.data(), 8 * nlev, hipMemcpyHostToDevice, c.s));
-  hipLaunchKernelGGL(cg_walk, dim3((nlev + 63) / 64), dim3(64), 0, c.s, succ, sad2, used, sadcnt,
-                     d_lev_base, d_lev_n, nlev, order, cyc_start, cyc_len, d_cyc_base, cyc_count,
-                     errs + 2);
-  HYD_HIP(hipGetLastError());
+  {
+    int32_t *P1, *E1, *D1, *P2, *E2, *D2, *M1, *Q1, *M2, *Q2, *R1, *R2;
+    HYD_CHECK(dbuf(c, "cw_p1", nalloc, &P1));
+    HYD_CHECK(dbuf(c, "cw_e1", nalloc, &E1));
+    HYD_CHECK(dbuf(c, "cw_d1", nalloc, &D1));
+    HYD_CHECK(dbuf(c, "cw_p2", nalloc, &P2));
+    HYD_CHECK(dbuf(c, "cw_e2", nalloc, &E2));
+    HYD_CHECK(dbuf(c, "cw_d2", nalloc, &D2));
+    HYD_CHECK(dbuf(c, "cw_m1", nalloc, &M1));
+    HYD_CHECK(dbuf(c, "cw_q1", nalloc, &Q1));
+    HYD_CHECK(dbuf(c, "cw_m2", nalloc, &M2));
+    HYD_CHECK(dbuf(c, "cw_q2", nalloc, &Q2));
+    HYD_CHECK(dbuf(c, "cw_r1", nalloc, &R1));
+    HYD_CHECK(dbuf(c, "cw_r2", nalloc, &R2));
+    ChainTab tb;
+    HYD_CHECK(dbuf(c, "cw_key", nalloc, &tb.key));
+    HYD_CHECK(dbuf(c, "cw_pos", nalloc, &tb.pos));
+    HYD_CHECK(dbuf(c, "cw_hd", nalloc, &tb.hd));
+    HYD_CHECK(dbuf(c, "cw_len", nalloc, &tb.len));
+    HYD_CHECK(dbuf(c, "cw_cmin", nalloc, &tb.cmin));
+    HYD_CHECK(dbuf(c, "cw_loop", nalloc, &tb.loop));
+    HYD_CHECK(dbuf(c, "cw_used", nalloc, &tb.used));
+    HYD_CHECK(dbuf(c, "cw_aoff", nalloc, &tb.aoff));
+    HYD_CHECK(dbuf(c, "cw_boff", nalloc, &tb.boff));
+    HYD_CHECK(dbuf(c, "cw_afrom", nalloc, &tb.afrom));
+    int8_t *sflag;
+    int32_t *levcnt, *starts, *nsel;
+    int64_t *d_soff;
+    HYD_CHECK(dbuf(c, "cw_sflag", nalloc, &sflag));
+    HYD_CHECK(dbuf(c, "cw_levcnt", nlev, &levcnt));
+    HYD_CHECK(dbuf(c, "cw_starts", nalloc, &starts));
+    HYD_CHECK(dbuf(c, "cw_nsel", 1, &nsel));
+    HYD_CHECK(dbuf(c, "cw_soff", nlev, &d_soff));
+    int64_t maxn = 1;
+    for (int lv = 0; lv < nlev; lv++) maxn = std::max(maxn, lev_n[lv]);
+    int rounds = 1;
+    while (((int64_t)1 << rounds) < maxn + 1) rounds++;
+    HYD_HIP(hipMemsetAsync(levcnt, 0, 4 * nlev, c.s));
+    if (ntot > 0) {
+      hipLaunchKernelGGL(cw_init, gb, dim3(256), 0, c.s, succ, sad2, ntot, P1, E1, D1);
+      for (int r = 0; r < rounds; r++) {
+        hipLaunchKernelGGL(cw_jump, gb, dim3(256), 0, c.s, P1, E1, D1, ntot, P2, E2, D2);
+        std::swap(P1, P2);
+        std::swap(E1, E2);
+        std::swap(D1, D2);
+      }
+      hipLaunchKernelGGL(cw_cyc_init, gb, dim3(256), 0, c.s, succ, E1, ntot, M1, Q1);
+      for (int r = 0; r < rounds; r++) {
+        hipLaunchKernelGGL(cw_cyc_min, gb, dim3(256), 0, c.s, M1, Q1, ntot, M2, Q2);
+        std::swap(M1, M2);
+        std::swap(Q1, Q2);
+      }
+      hipLaunchKernelGGL(cw_cyc_rank_init, gb, dim3(256), 0, c.s, succ, E1, M1, ntot, R1, Q1);
+      for (int r = 0; r < rounds; r++) {
+        hipLaunchKernelGGL(cw_rank_jump, gb, dim3(256), 0, c.s, R1, Q1, ntot, R2, Q2);
+        std::swap(R1, R2);
+        std::swap(Q1, Q2);
+      }
+      hipLaunchKernelGGL(cw_fill_tab, gb, dim3(256), 0, c.s, tb, ntot);
+      hipLaunchKernelGGL(cw_keys, gb, dim3(256), 0, c.s, E1, D1, M1, R1, ntot, tb);
+      hipLaunchKernelGGL(cw_pos, gb, dim3(256), 0, c.s, D1, R1, ntot, tb, sflag, lev_of_gid,
+                         levcnt);
+      HYD_HIP(hipGetLastError());
+      // starts (chain minima) in crossing order
+      size_t tbytes = 0;
+      hipcub::CountingInputIterator<int32_t> ids(0);
+      HYD_HIP(hipcub::DeviceSelect::Flagged(nullptr, tbytes, ids, sflag, starts, nsel, (int)ntot,
+                                            c.s));
+      char *tmp;
+      HYD_CHECK(dbuf(c, "cw_seltmp", (int64_t)tbytes, &tmp));
+      HYD_HIP(hipcub::DeviceSelect::Flagged(tmp, tbytes, ids, sflag, starts, nsel, (int)ntot,
+                                            c.s));
+    }
+    std::vector<int32_t> hlc(nlev);
+    HYD_HIP(hipMemcpyAsync(hlc.data(), levcnt, 4 * nlev, hipMemcpyDeviceToHost, c.s));
+    HYD_HIP(hipStreamSynchronize(c.s));
+    std::vector<int64_t> hso(nlev);
+    int64_t so = 0;
+    for (int lv = 0; lv < nlev; lv++) {
+      hso[lv] = so;
+      so += hlc[lv];
+    }
+    HYD_HIP(hipMemcpyAsync(d_soff, hso.data(), 8 * nlev, hipMemcpyHostToDevice, c.s));
+    hipLaunchKernelGGL(cw_walk, dim3((nlev + 63) / 64), dim3(64), 0, c.s, starts, d_soff, levcnt,
+                       succ, sad2, sadcnt, d_lev_base, d_lev_n, nlev, tb, cyc_start, cyc_len,
+                       cyc_count, errs + 2);
+    if (ntot > 0) hipLaunchKernelGGL(cw_expand, gb, dim3(256), 0, c.s, tb, ntot, errs + 2, order);
+    HYD_HIP(hipGetLastError());
+  }
   std::vector<int32_t> hcc(nlev), herr(4);
   HYD_HIP(hipMemcpyAsync(hcc.data(), cyc_count, 4 * nlev, hipMemcpyDeviceToHost, c.s));
   HYD_HIP(hipMemcpyAsync(herr.data(), errs, 16, hipMemcpyDeviceToHost, c.s));
