@@ -110,7 +110,7 @@ def pmc_summary(fetch_dir, write_dir, ng, out_json):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ng", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=12345)

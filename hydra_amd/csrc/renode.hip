@@ -1,15 +1,16 @@
 // renode.hip -- node redistribution, contours.f90:687-933, for a batch of
-// closed contours, one wavefront per contour.
+// closed contours, one workgroup per contour.
 //
 // The per-node geometry (increments, cubic coefficients, corner test,
 // curvature-based density) is evaluated lane-parallel in the reference's
 // expression order.  The two sequential pieces -- the sum of the fractional
 // node counts (contours.f90:799-802, whose nint decides how many nodes the
 // contour keeps) and the accumulator walk that places the new nodes
-// (813-930) -- run on lane 0 exactly as written, so node counts and
-// positions are identical to the reference.  The walk only records (old
-// segment, accumulator) per new node; the positions are then evaluated
-// lane-parallel.
+// (813-930) -- run exactly as written as wavefront-uniform loops: the
+// wavefront loads 64 consecutive values at a time and the loop reads them
+// with readlane, so the only serial chain is the additions themselves.
+// The walk only records (old segment, accumulator) per new node; the
+// positions are then evaluated lane-parallel.
 //
 // Two launches: RN_COUNT computes npr per contour (0 = delete); after an
 // exclusive scan of npr, RN_PLACE writes the nodes.
@@ -45,9 +46,9 @@ __device__ __forceinline__ double wrapc(double v, double L, double hli) {
 
 // Shared geometry of contours.f90:717-796 for contour c (all lanes).
 __device__ void renode_geometry(const RenodeArgs &g, int64_t o, int n) {
-  const int l = threadIdx.x;
+  const int l = threadIdx.x, nt = blockDim.x;
   // increments, v, e
-  for (int i = l; i < n; i += 64) {
+  for (int i = l; i < n; i += nt) {
     const int ia = i + 1 < n ? i + 1 : 0;
     const double xx = g.xd[o + ia] - g.xd[o + i];
     const double dx = wrapc(xx, g.ellx, g.hlxi);
@@ -61,7 +62,7 @@ __device__ void renode_geometry(const RenodeArgs &g, int64_t o, int n) {
   }
   __syncthreads();
   // u(i)=v(ib), a(i)=-dx(ib), c(i)=-dy(ib); corner test; b(i)
-  for (int i = l; i < n; i += 64) {
+  for (int i = l; i < n; i += nt) {
     const int ib = i > 0 ? i - 1 : n - 1;
     const double u = g.sv[o + ib], a = -g.sdx[o + ib], c = -g.sdy[o + ib];
     const double dx = g.sdx[o + i], dy = g.sdy[o + i], v = g.sv[o + i];
@@ -79,7 +80,7 @@ __device__ void renode_geometry(const RenodeArgs &g, int64_t o, int n) {
   }
   __syncthreads();
   // u=e*(b(ia)+b), c=e*(b(ia)-b); a,b,c cubic coefficients; curvature terms
-  for (int i = l; i < n; i += 64) {
+  for (int i = l; i < n; i += nt) {
     const int ia = i + 1 < n ? i + 1 : 0;
     const double e = g.se[o + i], bi = g.sb[o + i], ba = g.sb[o + ia];
     double u = e * (ba + bi);
@@ -97,16 +98,16 @@ __device__ void renode_geometry(const RenodeArgs &g, int64_t o, int n) {
     g.stm[o + i] = bnew;
   }
   __syncthreads();
-  for (int i = l; i < n; i += 64) g.sb[o + i] = g.stm[o + i];
+  for (int i = l; i < n; i += nt) g.sb[o + i] = g.stm[o + i];
   __syncthreads();
   // d(i) = (u(ib)+u(i))/(v(ib)+v(i)+small), kept in stm
-  for (int i = l; i < n; i += 64) {
+  for (int i = l; i < n; i += nt) {
     const int ib = i > 0 ? i - 1 : n - 1;
     g.stm[o + i] = (g.su[o + ib] + g.su[o + i]) / (g.sv[o + ib] + g.sv[o + i] + g.small);
   }
   __syncthreads();
   // u(i) = f12*(d(i)+d(ia)); e(i) = e(i)*min(dmi, densf*sqrt(u)+u)
-  for (int i = l; i < n; i += 64) {
+  for (int i = l; i < n; i += nt) {
     const int ia = i + 1 < n ? i + 1 : 0;
     const double u = F12 * (g.stm[o + i] + g.stm[o + ia]);
     const double t = g.densf * sqrt(u) + u;
@@ -115,10 +116,44 @@ __device__ void renode_geometry(const RenodeArgs &g, int64_t o, int n) {
   __syncthreads();
 }
 
-// Sequential sum of e over [b, e) in node order (lane 0).
+__device__ __forceinline__ double rl_d(double x, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+  return __hiloint2double(hi, lo);
+}
+
+// Wavefront-uniform reader of se[o + i], 64 values per load.
+struct ERead {
+  const double *se;
+  int64_t o;
+  int n, base;
+  double blk;
+  __device__ void init(const double *s_, int64_t o_, int n_) {
+    se = s_;
+    o = o_;
+    n = n_;
+    base = -64;
+    blk = 0.0;
+  }
+  __device__ __forceinline__ double get(int i) {
+    if (i < base || i >= base + 64) {
+      base = i & ~63;
+      const int j = base + (int)(threadIdx.x & 63);
+      blk = j < n ? se[o + j] : 0.0;
+    }
+    return rl_d(blk, i - base);
+  }
+};
+
+// Sequential sum of e over [b, e) in node order (every lane, same value).
 __device__ __forceinline__ double seq_e(const double *se, int64_t o, int b, int e) {
+  const int l = threadIdx.x & 63;
   double s = 0.0;
-  for (int i = b; i < e; i++) s = s + se[o + i];
+  for (int b0 = b; b0 < e; b0 += 64) {
+    const double v = b0 + l < e ? se[o + b0 + l] : 0.0;
+    const int m = e - b0 < 64 ? e - b0 : 64;
+    for (int j = 0; j < m; j++) s = s + rl_d(v, j);
+  }
   return s;
 }
 
@@ -137,72 +172,83 @@ __device__ __forceinline__ void place(const RenodeArgs &g, int64_t o, int i, dou
   yo = g.oms * wrapc(yy, g.elly, g.hlyi);
 }
 
+// One workgroup of 256 threads per contour: the lane-parallel phases use
+// all of them, the sequential sums and walks run on wavefront 0.
 template <bool PLACE>
-__global__ void __launch_bounds__(64) renode_kernel(RenodeArgs g) {
+__global__ void __launch_bounds__(256) renode_kernel(RenodeArgs g) {
   const int c = blockIdx.x;
   if (c >= g.ncont) return;
   const int n = g.npd[c];
   const int64_t o = g.off_in[c];
-  const int l = threadIdx.x;
+  const int t = threadIdx.x, l = t & 63, nt = blockDim.x;
+  const bool w0 = t < 64;
   if (n <= 0) {
-    if (!PLACE && l == 0) g.npr[c] = 0;
+    if (!PLACE && t == 0) g.npr[c] = 0;
     return;
   }
   renode_geometry(g, o, n);
-  // corners (d holds the flag); count them and find the first
+  // corners (d holds the flag): count, first; the total of e
+  // (contours.f90:799-809)
   __shared__ int s_ncorn, s_first;
-  if (l == 0) {
-    int nc = 0, first = -1;
-    for (int i = 0; i < n; i++)
-      if (g.sd[o + i] != 0.0) {
-        if (first < 0) first = i;
-        nc++;
-      }
-    s_ncorn = nc;
-    s_first = first;
+  __shared__ double s_sum;
+  if (w0) {
+    int ncorn = 0, first = -1;
+    for (int b0 = 0; b0 < n; b0 += 64) {
+      const uint64_t m = __ballot(b0 + l < n && g.sd[o + b0 + l] != 0.0);
+      if (m && first < 0) first = b0 + __ffsll((unsigned long long)m) - 1;
+      ncorn += __popcll(m);
+    }
+    const double sum = seq_e(g.se, o, 0, n);
+    if (l == 0) {
+      s_ncorn = ncorn;
+      s_first = first;
+      s_sum = sum;
+    }
   }
   __syncthreads();
-  const int ncorn = s_ncorn;
-  // total and node count (contours.f90:799-809)
-  __shared__ double s_sum;
-  if (l == 0) s_sum = seq_e(g.se, o, 0, n);
-  __syncthreads();
+  const int ncorn = s_ncorn, s_first_ = s_first;
   const double sum = s_sum;
   int npr = nint_d(sum) + 1;
   if (npr < 3) {
-    if (!PLACE && l == 0) g.npr[c] = 0;
+    if (!PLACE && t == 0) g.npr[c] = 0;
     return;
   }
   if (ncorn <= 1) {
     if (!PLACE) {
-      if (l == 0) g.npr[c] = npr;
+      if (t == 0) g.npr[c] = npr;
       return;
     }
     const double fac = (double)npr / sum;
-    for (int i = l; i < n; i += 64) g.se[o + i] = fac * g.se[o + i];
+    for (int i = t; i < n; i += nt) g.se[o + i] = fac * g.se[o + i];
     __syncthreads();
     const int64_t oo = g.off_out[c];
-    const int start = ncorn == 0 ? 0 : s_first;
-    if (l == 0) {
-      g.xr[oo] = g.xd[o + start];
-      g.yr[oo] = g.yd[o + start];
+    const int start = ncorn == 0 ? 0 : s_first_;
+    if (w0) {
+      if (l == 0) {
+        g.xr[oo] = g.xd[o + start];
+        g.yr[oo] = g.yd[o + start];
+      }
       // accumulator walk; ncorn=0 walks i = 1,2,.. ; ncorn=1 starts after
       // the corner with next() wrapping (contours.f90:826-868)
+      ERead er;
+      er.init(g.se, o, n);
       double acc = 0.0;
       int i = ncorn == 0 ? -1 : start - 1;
       for (int im = 1; im < npr; im++) {
         while (acc < 1.0) {
           i = (ncorn == 0) ? i + 1 : (i + 1 < n ? i + 1 : 0);
           if (ncorn == 0 && i >= n) { i = n - 1; break; }  // unreachable: sum(e)=npr
-          acc = acc + g.se[o + i];
+          acc = acc + er.get(i);
         }
         acc = acc - 1.0;
-        g.wi[oo + im] = i;
-        g.wacc[oo + im] = acc;
+        if (l == 0) {
+          g.wi[oo + im] = i;
+          g.wacc[oo + im] = acc;
+        }
       }
     }
     __syncthreads();
-    for (int im = 1 + l; im < npr; im += 64) {
+    for (int im = 1 + t; im < npr; im += nt) {
       double xo, yo;
       place(g, o, g.wi[oo + im], g.wacc[oo + im], xo, yo);
       g.xr[oo + im] = xo;
@@ -211,54 +257,70 @@ __global__ void __launch_bounds__(64) renode_kernel(RenodeArgs g) {
     return;
   }
   // several corners: renode each stretch between consecutive corners
-  // (contours.f90:873-929).  Lane 0 does the sequential sums and walks.
-  if (l == 0) {
+  // (contours.f90:873-929); wavefront 0 does the sums, scalings and walks.
+  __shared__ int s_tot;
+  if (w0) {
     const int64_t oo = PLACE ? g.off_out[c] : 0;
     int tot = 0;
-    // list corners in order
-    int k = 0;
-    int ibeg = s_first;
-    int corner_idx = s_first;
+    int ibeg = s_first_;
+    int corner_idx = s_first_;
+    ERead er;
+    er.init(g.se, o, n);
     for (int kk = 0; kk < ncorn; kk++) {
       // next corner after corner_idx (wrapping to the first one at the end)
       int iend;
       if (kk == ncorn - 1) {
-        iend = s_first;
+        iend = s_first_;
       } else {
         int j = corner_idx + 1;
-        while (g.sd[o + j] == 0.0) j++;
+        for (;;) {
+          const int jb = j & ~63;
+          const uint64_t m = __ballot(jb + l >= j && jb + l < n && g.sd[o + jb + l] != 0.0);
+          if (m) {
+            j = jb + __ffsll((unsigned long long)m) - 1;
+            break;
+          }
+          j = jb + 64;
+        }
         iend = j;
       }
       // sum over the stretch ibeg .. iend (exclusive), with next() wrapping
       double sm = 0.0;
       int i = ibeg;
       do {
-        sm = sm + g.se[o + i];
+        sm = sm + er.get(i);
         i = i + 1 < n ? i + 1 : 0;
       } while (i != iend);
       if (sm > g.small) {
         const int npseg = nint_d(sm) + 1;
         if (PLACE) {
           const double fac = (double)npseg / sm;
-          i = ibeg;
-          do {
-            g.se[o + i] = fac * g.se[o + i];
-            i = i + 1 < n ? i + 1 : 0;
-          } while (i != iend);
-          g.xr[oo + tot] = g.xd[o + ibeg];
-          g.yr[oo + tot] = g.yd[o + ibeg];
-          g.wi[oo + tot] = -1;
+          // scale the stretch (lane-parallel over its wrapped index range)
+          const int len = iend > ibeg ? iend - ibeg : iend + n - ibeg;
+          for (int q0 = l; q0 < len; q0 += 64) {
+            const int q = ibeg + q0 < n ? ibeg + q0 : ibeg + q0 - n;
+            g.se[o + q] = fac * g.se[o + q];
+          }
+          __threadfence();  // other lanes' stores visible to the reload below
+          er.base = -64;    // the scaled values replace the cached block
+          if (l == 0) {
+            g.xr[oo + tot] = g.xd[o + ibeg];
+            g.yr[oo + tot] = g.yd[o + ibeg];
+            g.wi[oo + tot] = -1;
+          }
           if (npseg > 1) {
             double acc = 0.0;
             i = ibeg - 1;
             for (int im = tot + 1; im < tot + npseg; im++) {
               while (acc < 1.0) {
                 i = i + 1 < n ? i + 1 : 0;
-                acc = acc + g.se[o + i];
+                acc = acc + er.get(i);
               }
               acc = acc - 1.0;
-              g.wi[oo + im] = i;
-              g.wacc[oo + im] = acc;
+              if (l == 0) {
+                g.wi[oo + im] = i;
+                g.wacc[oo + im] = acc;
+              }
             }
           }
         }
@@ -266,16 +328,17 @@ __global__ void __launch_bounds__(64) renode_kernel(RenodeArgs g) {
       }
       ibeg = iend;
       corner_idx = iend;
-      k++;
     }
-    (void)k;
-    if (!PLACE) g.npr[c] = tot;
+    if (l == 0) s_tot = tot;
   }
-  if (!PLACE) return;
   __syncthreads();
+  if (!PLACE) {
+    if (t == 0) g.npr[c] = s_tot;
+    return;
+  }
   const int64_t oo = g.off_out[c];
-  const int tot = g.npr[c];
-  for (int im = l; im < tot; im += 64) {
+  const int tot = s_tot;
+  for (int im = t; im < tot; im += nt) {
     const int i = g.wi[oo + im];
     if (i < 0) continue;  // corner copied above
     double xo, yo;
@@ -326,7 +389,7 @@ int renode_batch(Ctx &c, const double *xd, const double *yd, const int64_t *off_
   g.densf = k.densf;
   g.wi = nullptr;
   g.wacc = nullptr;
-  hipLaunchKernelGGL(renode_kernel<false>, dim3(ncont), dim3(64), 0, c.s, g);
+  hipLaunchKernelGGL(renode_kernel<false>, dim3(ncont), dim3(256), 0, c.s, g);
   HYD_HIP(hipGetLastError());
   std::vector<int32_t> hn(ncont);
   HYD_HIP(hipMemcpyAsync(hn.data(), npr, 4 * ncont, hipMemcpyDeviceToHost, c.s));
@@ -345,7 +408,7 @@ int renode_batch(Ctx &c, const double *xd, const double *yd, const int64_t *off_
   HYD_HIP(hipMemcpyAsync(off_out, ho.data(), 8 * ncont, hipMemcpyHostToDevice, c.s));
   HYD_CHECK(dbuf(c, "rn_wi", std::max<int64_t>(tot, 1), &g.wi));
   HYD_CHECK(dbuf(c, "rn_wacc", std::max<int64_t>(tot, 1), &g.wacc));
-  hipLaunchKernelGGL(renode_kernel<true>, dim3(ncont), dim3(64), 0, c.s, g);
+  hipLaunchKernelGGL(renode_kernel<true>, dim3(ncont), dim3(256), 0, c.s, g);
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }
