@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", type=int, default=0,
+                    help="0 plain FP64 (bit-exact), 1 mixed FP32 grid / FP64 state")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run the calibration copy (for rocprofv3 --pmc runs)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -134,7 +136,7 @@ def main():
 
     ng = args.ng
     qq = random_pv(ng, seed=args.seed)
-    h = Hydra(ng, ncontq=80)
+    h = Hydra(ng, ncontq=80, precision=args.precision)
     t0 = time.perf_counter()
     h.set_field(qq)
     h.recontour()
@@ -230,12 +232,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == 0 else "f32/f64",
         "data": "synthetic",
         "config": {"workload": f"plain FP64 CAPS RK4 step, 2D Euler, ng={ng}, 80 PV levels, "
                                f"seeded k^3 exp(-2(k/6)^2) field (rms 4pi)",
                    "ng": ng, "nodes": P, "nodes_end": P_end, "contours": nq,
-                   "precision": "plain-fp64", "fft_batch_autotuned": tuning,
+                   "precision": "plain-fp64" if args.precision == 0 else "mixed-fp32-grid",
+                   "fft_batch_autotuned": tuning,
                    "dt": "adaptive (reference adapt)", "seed": args.seed},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -116,13 +116,14 @@ class Hydra:
 
     def __init__(self, ng, ncontq=80, ellx=2 * np.pi, elly=None, kd=0.0, beta=0.0, rekman=0.0,
                  cdamp=2.0, nnu=3.0, tgsave=1000.0, tcsave=1000.0, dt_fixed=0.0,
-                 node_capacity=0, device=0):
+                 node_capacity=0, device=0, precision=0):
         self.L = lib()
         self.ng = ng
         self.params = HydParams(ng=ng, ncontq=ncontq, ellx=ellx, elly=ellx if elly is None else elly,
                                 kd=kd, beta=beta, rtherm=0.0, rekman=rekman, cdamp=cdamp, nnu=nnu,
                                 tgsave=tgsave, tcsave=tcsave, dt_fixed=dt_fixed,
-                                node_capacity=node_capacity, precision=0, device=device)
+                                node_capacity=node_capacity, precision=precision,
+                                device=device)
         h = C.c_void_p()
         _check(self.L.hyd_init(C.byref(self.params), C.byref(h)))
         self.h = h

@@ -90,9 +90,17 @@ static void free_ctx(Ctx *c) {
                   c->wb, c->wc, c->wd, c->cnt, c->fq, c->mq, c->qjx_cnt, c->qbot, c->ixc,
                   c->nxc, c->ndx, c->ndy, c->flags, c->replay, c->replay_val, c->rp_tkey,
                   c->rp_tid, c->rp_list, c->xq, c->yq, c->xqi, c->yqi, c->xqf, c->yqf, c->uq,
-                  c->vq, c->nextq, c->nxt_tmp, c->dsc, c->dsum, c->seq_work};
+                  c->vq, c->nextq, c->nxt_tmp, c->dsc, c->dsum, c->seq_work, c->uu32,
+                  c->vv32, c->fa32, c->fb32, c->fc32, c->tmp32, c->fq32, c->mq32, c->xtrig32,
+                  c->ytrig32, c->qb32, c->qdiss_u_dev, c->part};
   for (void *p : ptrs)
     if (p) hipFree(p);
+  for (int k = 0; k < 4; k++) {
+    if (c->t4[k]) hipFree(c->t4[k]);
+    if (c->t32[k]) hipFree(c->t32[k]);
+  }
+  if (c->fd32) hipFree(c->fd32);
+  if (c->fe32) hipFree(c->fe32);
   if (c->hsc) hipHostFree(c->hsc);
   if (c->exp_host) hipHostFree(c->exp_host);
   if (c->ev_sc) hipEventDestroy(c->ev_sc);
@@ -136,6 +144,11 @@ static int create(const hyd_params *p, hyd_ctx **out) {
     }
     c->cap = INT32_MAX / 2;
   }
+  if (p->precision != 0 && p->precision != 1) {
+    set_error("precision must be 0 (plain FP64) or 1 (mixed FP32 grid / FP64 state)");
+    delete c;
+    return HYD_EARG;
+  }
   c->dev = p->device;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
@@ -166,6 +179,7 @@ static int create(const hyd_params *p, hyd_ctx **out) {
                        &c->qr,  &c->emq, &c->epq, &c->qsi, &c->qsf, &c->sqs, &c->qdi,
                        &c->qdf, &c->sqd, &c->wa,  &c->wb,  &c->wc,  &c->wd};
   for (double **f : fields) TRY(dalloc(f, N));
+  for (int k = 0; k < 4; k++) TRY(dalloc(&c->t4[k], N));
   TRY(dalloc(&c->green, N));
   TRY(dalloc(&c->flo, N));
   TRY(dalloc(&c->fhi, N));
@@ -245,6 +259,31 @@ static int create(const hyd_params *p, hyd_ctx **out) {
     std::vector<double> xgf(c->c.nxf);
     for (int ix = 1; ix <= c->c.nxf; ix++) xgf[ix - 1] = c->c.xmin + c->c.glxf * (double)(ix - 1);
     TRY(upload(c->xgf, xgf.data(), c->c.nxf, c->s));
+    if (hipStreamSynchronize(c->s) != hipSuccess) {
+      set_error("stream sync failed");
+      free_ctx(c);
+      return HYD_EHIP;
+    }
+  }
+  if (p->precision == 1) {
+    float **f32[] = {&c->uu32, &c->vv32, &c->fa32,   &c->fb32,   &c->fc32,  &c->tmp32,
+                     &c->fd32, &c->fe32, &c->t32[0], &c->t32[1], &c->t32[2], &c->t32[3]};
+    for (float **f : f32) TRY(dalloc(f, N));
+    TRY(dalloc(&c->fq32, nfine));
+    TRY(dalloc(&c->mq32, nfine / 4));
+    TRY(dalloc(&c->qb32, c->c.nxf));
+    TRY(dalloc(&c->xtrig32, 2 * n));
+    TRY(dalloc(&c->ytrig32, 2 * n));
+    TRY(dalloc(&c->qdiss_u_dev, c->nqu));
+    TRY(dalloc(&c->part, 1024));
+    std::vector<float> tx(2 * n), ty(2 * n);
+    for (int i = 0; i < 2 * n; i++) {
+      tx[i] = (float)c->ht.xtrig[i];
+      ty[i] = (float)c->ht.ytrig[i];
+    }
+    TRY(upload(c->xtrig32, tx.data(), 2 * n, c->s));
+    TRY(upload(c->ytrig32, ty.data(), 2 * n, c->s));
+    TRY(upload(c->qdiss_u_dev, c->ht.qdiss_u.data(), c->nqu, c->s));
     if (hipStreamSynchronize(c->s) != hipSuccess) {
       set_error("stream sync failed");
       free_ctx(c);
@@ -640,7 +679,7 @@ int hyd_advance(hyd_ctx *h) {
     set_error("advance before set_field");
     return HYD_ESTATE;
   }
-  HYD_CHECK(advance(c));
+  HYD_CHECK(c.p.precision == 1 ? advance_mixed(c) : advance(c));
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
   return c2g_status(c);
 }

@@ -358,20 +358,21 @@ __global__ void c2g_colscan(const int32_t *cnt, const double *qbot, double *fq, 
 
 // Nine-point average of an (nh2 x nh2) column-major field f to (nh x nh),
 // contours.f90:298-326; out layout column-major with leading dim nh.
-__global__ void c2g_avg9(const double *__restrict__ f, double *__restrict__ out, int nh) {
+template <class T>
+__global__ void c2g_avg9(const T *__restrict__ f, T *__restrict__ out, int nh) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int ix = blockIdx.y + 1;
   if (iy > nh) return;
   const int n2 = 2 * nh;
 #define F(r, c) f[(int64_t)((c) == 0 ? n2 - 1 : (c)-1) * n2 + ((r) == 0 ? n2 - 1 : (r)-1)]
   const int r1 = 2 * iy, r0 = 2 * iy - 2, c1 = 2 * ix, c0 = 2 * ix - 2;
-  const double qev0_a = F(r1, c1), qev0_b = F(r0, c1), qev2_a = F(r1, c0), qev2_b = F(r0, c0);
-  const double qev1_a = F(r1, c1 - 1), qev1_b = F(r0, c1 - 1);
-  const double qod0 = F(r1 - 1, c1), qod2 = F(r1 - 1, c0), qod1 = F(r1 - 1, c1 - 1);
+  const T qev0_a = F(r1, c1), qev0_b = F(r0, c1), qev2_a = F(r1, c0), qev2_b = F(r0, c0);
+  const T qev1_a = F(r1, c1 - 1), qev1_b = F(r0, c1 - 1);
+  const T qod0 = F(r1 - 1, c1), qod2 = F(r1 - 1, c0), qod1 = F(r1 - 1, c1 - 1);
 #undef F
   out[(int64_t)(ix - 1) * nh + (iy - 1)] =
-      0.0625 * (qev0_a + qev0_b + qev2_a + qev2_b) +
-      0.125 * (qev1_a + qev1_b + qod0 + qod2) + 0.25 * qod1;
+      (T)0.0625 * (qev0_a + qev0_b + qev2_a + qev2_b) +
+      (T)0.125 * (qev1_a + qev1_b + qod0 + qod2) + (T)0.25 * qod1;
 }
 
 __global__ void c2g_subtract(const double *qa, double *qc, int64_t n, const double *qsum,
@@ -488,8 +489,8 @@ int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, i
                        c.qjx_cnt, c.qbot, c.fq));
   // 4ng -> 2ng -> ng
   const int nh1 = k.nxf / 2, nh2 = k.nx;
-  hipLaunchKernelGGL(c2g_avg9, dim3((nh1 + 255) / 256, nh1), dim3(256), 0, c.s, c.fq, c.mq, nh1);
-  hipLaunchKernelGGL(c2g_avg9, dim3((nh2 + 255) / 256, nh2), dim3(256), 0, c.s, c.mq, c.wd, nh2);
+  hipLaunchKernelGGL(c2g_avg9<double>, dim3((nh1 + 255) / 256, nh1), dim3(256), 0, c.s, c.fq, c.mq, nh1);
+  hipLaunchKernelGGL(c2g_avg9<double>, dim3((nh2 + 255) / 256, nh2), dim3(256), 0, c.s, c.mq, c.wd, nh2);
   HYD_HIP(hipGetLastError());
   // sequential mean, contours.f90:330-336
   HYD_CHECK(seqsum(c, c.wd, N, c.dsum));
@@ -504,6 +505,147 @@ int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, i
     const double q = c.hsc->qjump;
     for (int64_t t = 0; t < nfine; t++) counts_out[t] = (int32_t)llround(f[t] / q);
   }
+  return HYD_OK;
+}
+
+// ---- mixed precision ------------------------------------------------------
+// The fine field is an integer PV level count times qjump: the bottom row
+// and every column are integer prefix sums of the crossing counts (order
+// free), converted once; averages in FP32; the mean by a parallel
+// reduction.  No order-dependent cell needs special treatment.
+__device__ __forceinline__ int level_jump(int32_t w) { return (w & 0xffff) - ((w >> 16) & 0xffff); }
+
+__global__ void c2g_fill_i32(int32_t *a, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) a[k] = 0;
+}
+
+// qb[ix] = sum_{j < ix} jump(qjx[j]) (one workgroup)
+__global__ void __launch_bounds__(1024) c2g_qbot_int(const int32_t *qjx, int32_t *qb, int nxf) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (nxf + 1023) / 1024;
+  const int b = min(nxf, per * t), e = min(nxf, b + per);
+  int32_t s = 0;
+  for (int j = b; j < e; j++) s += level_jump(qjx[j]);
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int32_t v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int32_t acc = part[t] - s;
+  for (int j = b; j < e; j++) {
+    qb[j] = acc;
+    acc += level_jump(qjx[j]);
+  }
+}
+
+// y sweep as integer prefix sums, one wavefront per fine column
+__global__ void c2g_colscan_int(const int32_t *cnt, const int32_t *qb, float *fq, int nxf,
+                                int nyf, double qjump) {
+  const int wpb = blockDim.x / 64;
+  const int col = blockIdx.x * wpb + threadIdx.x / 64;
+  const int l = threadIdx.x & 63;
+  if (col >= nxf) return;
+  const int32_t *cc = cnt + (int64_t)col * nyf;
+  float *out = fq + (int64_t)col * nyf;
+  int32_t run = qb[col];
+  for (int base = 0; base < nyf; base += 64) {
+    const int r = base + l;
+    int32_t v = (r < nyf && r >= 1) ? level_jump(cc[r]) : 0;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t u = __shfl_up(v, o);
+      if (l >= o) v += u;
+    }
+    if (r < nyf) out[r] = (float)((double)(run + v) * qjump);
+    run += __shfl(v, 63);
+  }
+}
+
+__global__ void c2g_sum_partial(const float *a, int64_t n, double *part) {
+  double s = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    s += (double)a[k];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+  __shared__ double w[4];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+}
+
+__global__ void c2g_subtract_mean(const float *qa, float *qc, int64_t n, const double *part,
+                                  int np, double ndiv) {
+  __shared__ double mean;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < np; i++) s += part[i];
+    mean = s / ndiv;
+  }
+  __syncthreads();
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) qc[k] = (float)((double)qa[k] - mean);
+}
+
+int con2grid_mixed(Ctx &c, float *qc) {
+  const Consts &k = c.c;
+  const int64_t N = c.N;
+  const int n = c.nptq;
+  const int nxf = k.nxf, nyf = k.nyf;
+  const int64_t nfine = (int64_t)nxf * nyf;
+  KTimer tm(c, "con2grid32", (int64_t)n * 24 + 4 * N);
+  hipLaunchKernelGGL(c2g_fill_i32, dim3((unsigned)((nfine + 255) / 256)), dim3(256), 0, c.s,
+                     c.cnt, nfine);
+  hipLaunchKernelGGL(c2g_fill_i32, dim3((nxf + 255) / 256), dim3(256), 0, c.s, c.qjx_cnt,
+                     (int64_t)nxf);
+  hipLaunchKernelGGL(c2g_fill_i32, dim3(1), dim3(64), 0, c.s, c.flags, (int64_t)8);
+  if (n > 0) {
+    C2GArgs a;
+    a.xq = c.xq;
+    a.yq = c.yq;
+    a.nextq = c.nextq;
+    a.n = n;
+    a.ultra = 0;
+    a.nxf = nxf;
+    a.nyf = nyf;
+    a.ellx = k.ellx;
+    a.elly = k.elly;
+    a.hlx = k.hlx;
+    a.hly = k.hly;
+    a.hlxi = k.hlxi;
+    a.hlyi = k.hlyi;
+    a.xmin = k.xmin;
+    a.ymin = k.ymin;
+    a.glxfi = k.glxfi;
+    a.glyfi = k.glyfi;
+    a.small = k.small;
+    a.qjump = 0.0;
+    a.xgf = c.xgf;
+    a.cnt = c.cnt;
+    a.qjx = c.qjx_cnt;
+    a.flags = c.flags;
+    a.ixc = c.ixc;
+    a.nxc = c.nxc;
+    a.ndx = c.ndx;
+    a.ndy = c.ndy;
+    hipLaunchKernelGGL(c2g_nodes, dim3((n + 255) / 256), dim3(256), 0, c.s, a);
+  }
+  const double qjump = c.hsc->qjump;  // fixed between recontourings
+  hipLaunchKernelGGL(c2g_qbot_int, dim3(1), dim3(1024), 0, c.s, c.qjx_cnt, c.qb32, nxf);
+  hipLaunchKernelGGL(c2g_colscan_int, dim3((nxf + 3) / 4), dim3(256), 0, c.s, c.cnt, c.qb32,
+                     c.fq32, nxf, nyf, qjump);
+  const int nh1 = nxf / 2, nh2 = k.nx;
+  hipLaunchKernelGGL(c2g_avg9<float>, dim3((nh1 + 255) / 256, nh1), dim3(256), 0, c.s, c.fq32,
+                     c.mq32, nh1);
+  hipLaunchKernelGGL(c2g_avg9<float>, dim3((nh2 + 255) / 256, nh2), dim3(256), 0, c.s, c.mq32,
+                     c.fc32, nh2);
+  hipLaunchKernelGGL(c2g_sum_partial, dim3(1024), dim3(256), 0, c.s, c.fc32, N, c.part);
+  hipLaunchKernelGGL(c2g_subtract_mean, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.s,
+                     c.fc32, qc, N, c.part, 1024, (double)N);
+  HYD_HIP(hipGetLastError());
   return HYD_OK;
 }
 

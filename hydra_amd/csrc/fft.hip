@@ -24,47 +24,50 @@
 
 namespace hyd {
 
+// Up to kMaxF fields of the same shape and direction per launch
+// (blockIdx.y selects the field); each field has its own input, output,
+// fused derivative and scaling.
+static constexpr int kMaxF = 4;
 struct FftArgs {
-  const double *in;
-  double *out;
-  const double *trig;   // 2n: cos then -sin (stafft.f90:103-122)
-  const double *dk;     // filtered wavenumbers for a fused derivative
-  const double *mulk;   // optional per-element multiplier applied before the
-                        // derivative (unused: kept null)
+  const void *in[kMaxF];    // TI
+  void *out[kMaxF];         // TO
+  const double *dk[kMaxF];  // filtered wavenumbers for a fused derivative
+  int pre[kMaxF];           // 0 none, 1 derivative across vectors, 2 along elements
+  double post[kMaxF];       // normalisation (and sign) folded into the store
+  const void *trig;         // T, 2n: cos then -sin (stafft.f90:103-122)
   int n, m, V, lgV;
   int in_rows, out_rows;
-  int pre;              // 0 none, 1 derivative across vectors, 2 along elements
   int n2, n4;
-  double post;          // normalisation (and sign) folded into the store
 };
 
 // ---------------------------------------------------------------------------
 // forward stages (decimation in time)
 
-__device__ __forceinline__ void fwd_r2(const double *__restrict__ a, double *__restrict__ b,
+template <class T>
+__device__ __forceinline__ void fwd_r2(const T *__restrict__ a, T *__restrict__ b,
                                        int n, int V, int rem, int lv,
-                                       const double *__restrict__ cs,
-                                       const double *__restrict__ sn) {
+                                       const T *__restrict__ cs,
+                                       const T *__restrict__ sn) {
   const int nh = (lv - 1) / 2;
   const int nk = 1 + nh;
   const int per = rem * nk;
   for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
     const int v = it / per, r = it - v * per;
     const int kk = r / rem, irem = r - kk * rem;
-    const double *av = a + v * n;
-    double *bv = b + v * n;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
 #define A(j, k) av[irem + rem * ((j) + 2 * (k))]
 #define B(k, j) bv[irem + rem * ((k) + lv * (j))]
     if (kk == 0) {
-      const double a00 = A(0, 0), a10 = A(1, 0);
+      const T a00 = A(0, 0), a10 = A(1, 0);
       B(0, 0) = a00 + a10;
       B(0, 1) = a00 - a10;
     } else {
       const int k = kk, kc = lv - k;
-      const double c1k = cs[k], s1k = sn[k];
-      const double a1k = A(1, k), a1kc = A(1, kc), a0k = A(0, k), a0kc = A(0, kc);
-      const double x1 = c1k * a1k - s1k * a1kc;
-      const double y1 = c1k * a1kc + s1k * a1k;
+      const T c1k = cs[k], s1k = sn[k];
+      const T a1k = A(1, k), a1kc = A(1, kc), a0k = A(0, k), a0kc = A(0, kc);
+      const T x1 = c1k * a1k - s1k * a1kc;
+      const T y1 = c1k * a1kc + s1k * a1k;
       B(k, 0) = a0k + x1;
       B(kc, 0) = a0k - x1;
       B(k, 1) = y1 - a0kc;
@@ -75,11 +78,12 @@ __device__ __forceinline__ void fwd_r2(const double *__restrict__ a, double *__r
   }
 }
 
-__device__ __forceinline__ void fwd_r4(const double *__restrict__ a, double *__restrict__ b,
+template <class T>
+__device__ __forceinline__ void fwd_r4(const T *__restrict__ a, T *__restrict__ b,
                                        int n, int V, int rem, int lv,
-                                       const double *__restrict__ cs,
-                                       const double *__restrict__ sn) {
-  const double rtf12 = 0.7071067811865475244008443621048490392848359376884740365883398689953662;
+                                       const T *__restrict__ cs,
+                                       const T *__restrict__ sn) {
+  const T rtf12 = (T)0.7071067811865475244008443621048490392848359376884740365883398689953662;
   const int nh = (lv - 1) / 2;
   const int even = (lv & 1) == 0;
   const int nk = 1 + nh + even;
@@ -87,39 +91,39 @@ __device__ __forceinline__ void fwd_r4(const double *__restrict__ a, double *__r
   for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
     const int v = it / per, r = it - v * per;
     const int kk = r / rem, irem = r - kk * rem;
-    const double *av = a + v * n;
-    double *bv = b + v * n;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
 #define A(j, k) av[irem + rem * ((j) + 4 * (k))]
 #define B(k, j) bv[irem + rem * ((k) + lv * (j))]
     if (kk == 0) {
-      const double a0 = A(0, 0), a1 = A(1, 0), a2 = A(2, 0), a3 = A(3, 0);
-      const double t1r = a0 + a2;
-      const double t2r = a1 + a3;
+      const T a0 = A(0, 0), a1 = A(1, 0), a2 = A(2, 0), a3 = A(3, 0);
+      const T t1r = a0 + a2;
+      const T t2r = a1 + a3;
       B(0, 0) = t1r + t2r;
       B(0, 1) = a0 - a2;
       B(0, 2) = t1r - t2r;
       B(0, 3) = a3 - a1;
     } else if (kk <= nh) {
       const int k = kk, kc = lv - k;
-      const double c1k = cs[k], s1k = sn[k];
-      const double c2k = cs[k + lv], s2k = sn[k + lv];
-      const double c3k = cs[k + 2 * lv], s3k = sn[k + 2 * lv];
-      const double a0k = A(0, k), a0kc = A(0, kc), a1k = A(1, k), a1kc = A(1, kc);
-      const double a2k = A(2, k), a2kc = A(2, kc), a3k = A(3, k), a3kc = A(3, kc);
-      const double x1p = c1k * a1k - s1k * a1kc;
-      const double y1p = c1k * a1kc + s1k * a1k;
-      const double x2p = c2k * a2k - s2k * a2kc;
-      const double y2p = c2k * a2kc + s2k * a2k;
-      const double x3p = c3k * a3k - s3k * a3kc;
-      const double y3p = c3k * a3kc + s3k * a3k;
-      const double t1r = a0k + x2p;
-      const double t1i = a0kc + y2p;
-      const double t2r = x1p + x3p;
-      const double t2i = y1p + y3p;
-      const double t3r = a0k - x2p;
-      const double t3i = a0kc - y2p;
-      const double t4r = x3p - x1p;
-      const double t4i = y1p - y3p;
+      const T c1k = cs[k], s1k = sn[k];
+      const T c2k = cs[k + lv], s2k = sn[k + lv];
+      const T c3k = cs[k + 2 * lv], s3k = sn[k + 2 * lv];
+      const T a0k = A(0, k), a0kc = A(0, kc), a1k = A(1, k), a1kc = A(1, kc);
+      const T a2k = A(2, k), a2kc = A(2, kc), a3k = A(3, k), a3kc = A(3, kc);
+      const T x1p = c1k * a1k - s1k * a1kc;
+      const T y1p = c1k * a1kc + s1k * a1k;
+      const T x2p = c2k * a2k - s2k * a2kc;
+      const T y2p = c2k * a2kc + s2k * a2k;
+      const T x3p = c3k * a3k - s3k * a3kc;
+      const T y3p = c3k * a3kc + s3k * a3k;
+      const T t1r = a0k + x2p;
+      const T t1i = a0kc + y2p;
+      const T t2r = x1p + x3p;
+      const T t2i = y1p + y3p;
+      const T t3r = a0k - x2p;
+      const T t3i = a0kc - y2p;
+      const T t4r = x3p - x1p;
+      const T t4i = y1p - y3p;
       B(k, 0) = t1r + t2r;
       B(kc, 0) = t3r - t4i;
       B(k, 1) = t3r + t4i;
@@ -130,9 +134,9 @@ __device__ __forceinline__ void fwd_r4(const double *__restrict__ a, double *__r
       B(kc, 3) = t1i + t2i;
     } else {
       const int l2 = lv / 2;
-      const double a0 = A(0, l2), a1 = A(1, l2), a2 = A(2, l2), a3 = A(3, l2);
-      const double q1 = rtf12 * (a1 - a3);
-      const double q2 = rtf12 * (a1 + a3);
+      const T a0 = A(0, l2), a1 = A(1, l2), a2 = A(2, l2), a3 = A(3, l2);
+      const T q1 = rtf12 * (a1 - a3);
+      const T q2 = rtf12 * (a1 + a3);
       B(l2, 0) = a0 + q1;
       B(l2, 1) = a0 - q1;
       B(l2, 2) = a2 - q2;
@@ -146,11 +150,12 @@ __device__ __forceinline__ void fwd_r4(const double *__restrict__ a, double *__r
 // ---------------------------------------------------------------------------
 // reverse stages (decimation in frequency); nv = cum, lv = rem
 
-__device__ __forceinline__ void rev_r4(const double *__restrict__ a, double *__restrict__ b,
+template <class T>
+__device__ __forceinline__ void rev_r4(const T *__restrict__ a, T *__restrict__ b,
                                        int n, int V, int cum, int lv,
-                                       const double *__restrict__ cs,
-                                       const double *__restrict__ sn) {
-  const double rtf12 = 0.7071067811865475244008443621048490392848359376884740365883398689953662;
+                                       const T *__restrict__ cs,
+                                       const T *__restrict__ sn) {
+  const T rtf12 = (T)0.7071067811865475244008443621048490392848359376884740365883398689953662;
   const int nh = (lv - 1) / 2;
   const int even = (lv & 1) == 0;
   const int nk = 1 + nh + even;
@@ -158,41 +163,41 @@ __device__ __forceinline__ void rev_r4(const double *__restrict__ a, double *__r
   for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
     const int v = it / per, r = it - v * per;
     const int kk = r / cum, ic = r - kk * cum;
-    const double *av = a + v * n;
-    double *bv = b + v * n;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
 #define A(k, j) av[ic + cum * ((k) + lv * (j))]
 #define B(j, k) bv[ic + cum * ((j) + 4 * (k))]
     if (kk == 0) {
-      const double a0 = A(0, 0), a1 = A(0, 1), a2 = A(0, 2), a3 = A(0, 3);
-      const double t1r = a0 + a2;
-      const double t2r = a1;
-      const double t3r = a0 - a2;
-      const double t4r = a3;
+      const T a0 = A(0, 0), a1 = A(0, 1), a2 = A(0, 2), a3 = A(0, 3);
+      const T t1r = a0 + a2;
+      const T t2r = a1;
+      const T t3r = a0 - a2;
+      const T t4r = a3;
       B(0, 0) = t1r + t2r;
       B(1, 0) = t3r + t4r;
       B(2, 0) = t1r - t2r;
       B(3, 0) = t3r - t4r;
     } else if (kk <= nh) {
       const int k = kk, kc = lv - k;
-      const double c1k = cs[k], s1k = sn[k];
-      const double c2k = cs[k + lv], s2k = sn[k + lv];
-      const double c3k = cs[k + 2 * lv], s3k = sn[k + 2 * lv];
-      const double ak0 = A(k, 0), ak1 = A(k, 1), ak2 = A(k, 2), ak3 = A(k, 3);
-      const double akc0 = A(kc, 0), akc1 = A(kc, 1), akc2 = A(kc, 2), akc3 = A(kc, 3);
-      const double t1r = ak0 + akc1;
-      const double t1i = akc3 - ak2;
-      const double t2r = ak1 + akc0;
-      const double t2i = akc2 - ak3;
-      const double t3r = ak0 - akc1;
-      const double t3i = akc3 + ak2;
-      const double t4r = ak1 - akc0;
-      const double t4i = akc2 + ak3;
-      const double x1p = t3r + t4i;
-      const double y1p = t3i - t4r;
-      const double x2p = t1r - t2r;
-      const double y2p = t1i - t2i;
-      const double x3p = t3r - t4i;
-      const double y3p = t3i + t4r;
+      const T c1k = cs[k], s1k = sn[k];
+      const T c2k = cs[k + lv], s2k = sn[k + lv];
+      const T c3k = cs[k + 2 * lv], s3k = sn[k + 2 * lv];
+      const T ak0 = A(k, 0), ak1 = A(k, 1), ak2 = A(k, 2), ak3 = A(k, 3);
+      const T akc0 = A(kc, 0), akc1 = A(kc, 1), akc2 = A(kc, 2), akc3 = A(kc, 3);
+      const T t1r = ak0 + akc1;
+      const T t1i = akc3 - ak2;
+      const T t2r = ak1 + akc0;
+      const T t2i = akc2 - ak3;
+      const T t3r = ak0 - akc1;
+      const T t3i = akc3 + ak2;
+      const T t4r = ak1 - akc0;
+      const T t4i = akc2 + ak3;
+      const T x1p = t3r + t4i;
+      const T y1p = t3i - t4r;
+      const T x2p = t1r - t2r;
+      const T y2p = t1i - t2i;
+      const T x3p = t3r - t4i;
+      const T y3p = t3i + t4r;
       B(0, k) = t1r + t2r;
       B(0, kc) = t1i + t2i;
       B(1, k) = c1k * x1p - s1k * y1p;
@@ -203,11 +208,11 @@ __device__ __forceinline__ void rev_r4(const double *__restrict__ a, double *__r
       B(3, kc) = c3k * y3p + s3k * x3p;
     } else {
       const int l2 = lv / 2;
-      const double a0 = A(l2, 0), a1 = A(l2, 1), a2 = A(l2, 2), a3 = A(l2, 3);
+      const T a0 = A(l2, 0), a1 = A(l2, 1), a2 = A(l2, 2), a3 = A(l2, 3);
       B(0, l2) = a0 + a1;
       B(2, l2) = a3 - a2;
-      const double t3r = a0 - a1;
-      const double t4r = a3 + a2;
+      const T t3r = a0 - a1;
+      const T t4r = a3 + a2;
       B(1, l2) = rtf12 * (t3r + t4r);
       B(3, l2) = rtf12 * (t4r - t3r);
     }
@@ -216,30 +221,31 @@ __device__ __forceinline__ void rev_r4(const double *__restrict__ a, double *__r
   }
 }
 
-__device__ __forceinline__ void rev_r2(const double *__restrict__ a, double *__restrict__ b,
+template <class T>
+__device__ __forceinline__ void rev_r2(const T *__restrict__ a, T *__restrict__ b,
                                        int n, int V, int cum, int lv,
-                                       const double *__restrict__ cs,
-                                       const double *__restrict__ sn) {
+                                       const T *__restrict__ cs,
+                                       const T *__restrict__ sn) {
   const int nh = (lv - 1) / 2;
   const int nk = 1 + nh;
   const int per = cum * nk;
   for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
     const int v = it / per, r = it - v * per;
     const int kk = r / cum, ic = r - kk * cum;
-    const double *av = a + v * n;
-    double *bv = b + v * n;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
 #define A(k, j) av[ic + cum * ((k) + lv * (j))]
 #define B(j, k) bv[ic + cum * ((j) + 2 * (k))]
     if (kk == 0) {
-      const double a0 = A(0, 0), a1 = A(0, 1);
+      const T a0 = A(0, 0), a1 = A(0, 1);
       B(0, 0) = a0 + a1;
       B(1, 0) = a0 - a1;
     } else {
       const int k = kk, kc = lv - k;
-      const double c1k = cs[k], s1k = sn[k];
-      const double ak0 = A(k, 0), akc0 = A(kc, 0), ak1 = A(k, 1), akc1 = A(kc, 1);
-      const double x1p = ak0 - akc0;
-      const double y1p = akc1 + ak1;
+      const T c1k = cs[k], s1k = sn[k];
+      const T ak0 = A(k, 0), akc0 = A(kc, 0), ak1 = A(k, 1), akc1 = A(kc, 1);
+      const T x1p = ak0 - akc0;
+      const T y1p = akc1 + ak1;
       B(0, k) = ak0 + akc0;
       B(0, kc) = akc1 - ak1;
       B(1, k) = c1k * x1p - s1k * y1p;
@@ -257,20 +263,27 @@ __device__ __forceinline__ void rev_r2(const double *__restrict__ a, double *__r
 // batch V (a power of two) is a run-time shift.  Blocks are mapped so that
 // the blocks one XCD runs hold neighbouring vectors (shared L2 lines on the
 // strided side).
-template <int N, bool FWD>
+template <int N, bool FWD, class TI, class T, class TO>
 __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
-  extern __shared__ double lds[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *lds = (T *)lds_raw;
+  const int f = blockIdx.y;
+  const TI *gin = (const TI *)g.in[f];
+  TO *gout = (TO *)g.out[f];
+  const double *gdk = g.dk[f];
+  const int gpre = g.pre[f];
+  const T *gtrig = (const T *)g.trig;
   constexpr int L2N = __builtin_ctz(N);
   constexpr int n2 = L2N & 1, n4 = L2N >> 1;
   const int m = g.m, V = g.V, lv = g.lgV;
   const int nb = gridDim.x;
   const int bid = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
   const int v0 = bid * V;
-  double *bufA = lds, *bufB = lds + V * N;
-  double *tcs = lds + 2 * V * N, *tsn = tcs + N;
+  T *bufA = lds, *bufB = lds + V * N;
+  T *tcs = lds + 2 * V * N, *tsn = tcs + N;
   const int tot = V * N;
   constexpr int nhalf = N / 2;
-  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) tcs[i] = g.trig[i];
+  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) tcs[i] = gtrig[i];
   // ---- load
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     int v, e;
@@ -282,42 +295,42 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
       v = idx & (V - 1);
     }
     const int vg = v0 + v;
-    double x;
+    T x;
     if (FWD) {
-      x = g.in_rows ? g.in[(int64_t)vg * N + e] : g.in[vg + (int64_t)m * e];
+      x = (T)(g.in_rows ? gin[(int64_t)vg * N + e] : gin[vg + (int64_t)m * e]);
     } else {
       // optional fused derivative (sta2dfft.f90:169-245)
-      if (g.pre == 2) {
+      if (gpre == 2) {
         // along elements: der(e) from var(n-e)
         if (e == 0 || e == nhalf) {
           x = 0.0;
         } else {
           const int em = N - e;
-          const double var = g.in_rows ? g.in[(int64_t)vg * N + em] : g.in[vg + (int64_t)m * em];
-          x = (e < nhalf) ? -g.dk[2 * e - 1] * var : g.dk[2 * em - 1] * var;
+          const T var = (T)g.in_rows ? gin[(int64_t)vg * N + em] : gin[vg + (int64_t)m * em];
+          x = (e < nhalf) ? -(T)gdk[2 * e - 1] * var : (T)gdk[2 * em - 1] * var;
         }
-      } else if (g.pre == 1) {
+      } else if (gpre == 1) {
         // across vectors (vector index is the derivative's wavenumber, m = its length)
         const int mh = m / 2;
         if (vg == 0 || vg == mh) {
           x = 0.0;
         } else {
           const int vm = m - vg;
-          const double var = g.in_rows ? g.in[(int64_t)vm * N + e] : g.in[vm + (int64_t)m * e];
-          x = (vg < mh) ? -g.dk[2 * vg - 1] * var : g.dk[2 * vm - 1] * var;
+          const T var = (T)g.in_rows ? gin[(int64_t)vm * N + e] : gin[vm + (int64_t)m * e];
+          x = (vg < mh) ? -(T)gdk[2 * vg - 1] * var : (T)gdk[2 * vm - 1] * var;
         }
       } else {
-        x = g.in_rows ? g.in[(int64_t)vg * N + e] : g.in[vg + (int64_t)m * e];
+        x = (T)(g.in_rows ? gin[(int64_t)vg * N + e] : gin[vg + (int64_t)m * e]);
       }
       // revfft pre-processing, stafft.f90:311-325
       if (e >= nhalf + 1) x = -x;
-      if (e == 0 || e == nhalf) x = 0.5 * x;
+      if (e == 0 || e == nhalf) x = (T)0.5 * x;
     }
     bufA[v * N + e] = x;
   }
   __syncthreads();
   // ---- stages
-  double *a = bufA, *b = bufB;
+  T *a = bufA, *b = bufB;
   if (FWD) {
     int rem = N, cum = 1;
 #pragma unroll
@@ -326,7 +339,7 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
       const int iloc = (rem - 1) * 2 * cum;
       fwd_r2(a, b, N, V, rem, cum, tcs + iloc, tsn + iloc);
       __syncthreads();
-      double *t = a; a = b; b = t;
+      T *t = a; a = b; b = t;
       cum *= 2;
     }
 #pragma unroll
@@ -335,7 +348,7 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
       const int iloc = (rem - 1) * 4 * cum;
       fwd_r4(a, b, N, V, rem, cum, tcs + iloc, tsn + iloc);
       __syncthreads();
-      double *t = a; a = b; b = t;
+      T *t = a; a = b; b = t;
       cum *= 4;
     }
   } else {
@@ -346,7 +359,7 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
       const int iloc = (cum - 1) * 4 * rem;
       rev_r4(a, b, N, V, cum, rem, tcs + iloc, tsn + iloc);
       __syncthreads();
-      double *t = a; a = b; b = t;
+      T *t = a; a = b; b = t;
       cum *= 4;
     }
 #pragma unroll
@@ -355,53 +368,62 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
       const int iloc = (cum - 1) * 2 * rem;
       rev_r2(a, b, N, V, cum, rem, tcs + iloc, tsn + iloc);
       __syncthreads();
-      double *t = a; a = b; b = t;
+      T *t = a; a = b; b = t;
       cum *= 2;
     }
   }
   // ---- store (normalisation folded in, stafft.f90:275-284 / 394-403)
-  const double post = g.post;
+  const T post = (T)g.post[f];
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     int v, e;
     if (g.out_rows) {
       v = idx / N;
       e = idx - v * N;
-      g.out[(int64_t)(v0 + v) * N + e] = a[v * N + e] * post;
+      gout[(int64_t)(v0 + v) * N + e] = (TO)(a[v * N + e] * post);
     } else {
       e = idx >> lv;
       v = idx & (V - 1);
-      g.out[(v0 + v) + (int64_t)m * e] = a[v * N + e] * post;
+      gout[(v0 + v) + (int64_t)m * e] = (TO)(a[v * N + e] * post);
     }
   }
 }
 
-template <int N>
+template <int N, class TI, class T, class TO>
 static int launch_n(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftArgs &g) {
   // more than 64 KiB of LDS per workgroup must be requested (gfx950: 160 KiB)
   static bool raised = false;
   if (!raised && shm > 65536) {
-    HYD_HIP(hipFuncSetAttribute((const void *)fft_kernel<N, true>,
+    HYD_HIP(hipFuncSetAttribute((const void *)fft_kernel<N, true, TI, T, TO>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HYD_HIP(hipFuncSetAttribute((const void *)fft_kernel<N, false>,
+    HYD_HIP(hipFuncSetAttribute((const void *)fft_kernel<N, false, TI, T, TO>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     raised = true;
   }
   if (fwd)
-    hipLaunchKernelGGL((fft_kernel<N, true>), grid, dim3(256), shm, s, g);
+    hipLaunchKernelGGL((fft_kernel<N, true, TI, T, TO>), grid, dim3(256), shm, s, g);
   else
-    hipLaunchKernelGGL((fft_kernel<N, false>), grid, dim3(256), shm, s, g);
+    hipLaunchKernelGGL((fft_kernel<N, false, TI, T, TO>), grid, dim3(256), shm, s, g);
   return HYD_OK;
 }
 
-static int fft_launch(Ctx &c, bool fwd, const double *in, double *out, int n, int m,
-                      int in_rows, int out_rows, int pre, const double *dk, double post,
-                      const double *trig) {
+template <class TI, class T, class TO>
+static int fft_launch_t(Ctx &c, bool fwd, int F, const TI *const *in, TO *const *out, int n,
+                        int m, int in_rows, int out_rows, const int *pre,
+                        const double *const *dk, const double *post, const T *trig) {
+  if (F < 1 || F > kMaxF) {
+    set_error("fft: batch of 1..4 fields");
+    return HYD_EARG;
+  }
   FftArgs g;
-  g.in = in;
-  g.out = out;
+  for (int f = 0; f < kMaxF; f++) {
+    const int q = f < F ? f : 0;
+    g.in[f] = in[q];
+    g.out[f] = out[q];
+    g.dk[f] = dk ? dk[q] : nullptr;
+    g.pre[f] = pre ? pre[q] : 0;
+    g.post[f] = post[q];
+  }
   g.trig = trig;
-  g.dk = dk;
-  g.mulk = nullptr;
   g.n = n;
   g.m = m;
   // batch per workgroup: the autotuned value of this layout class when it
@@ -418,31 +440,44 @@ static int fft_launch(Ctx &c, bool fwd, const double *in, double *out, int n, in
   g.lgV = __builtin_ctz(V);
   g.in_rows = in_rows;
   g.out_rows = out_rows;
-  g.pre = pre;
   g.n2 = c.plan.n2;
   g.n4 = c.plan.n4;
-  g.post = post;
-  const size_t shm = sizeof(double) * (2 * (size_t)V * n + 2 * (size_t)n);
-  const dim3 grid(m / V);
-  // one launch reads and writes m vectors of n doubles
-  KTimer tk(c, fwd ? "k:fft_fwd" : "k:fft_rev", 16 * (int64_t)n * m);
+  const size_t shm = sizeof(T) * (2 * (size_t)V * n + 2 * (size_t)n);
+  const dim3 grid(m / V, F);
+  // one launch reads and writes F x m vectors of n values
+  KTimer tk(c, sizeof(T) == 8 ? (fwd ? "k:fft_fwd" : "k:fft_rev") : (fwd ? "k:fft32_fwd" : "k:fft32_rev"),
+            (int64_t)(sizeof(TI) + sizeof(TO)) * n * m * F);
   switch (n) {
-    case 8: HYD_CHECK(launch_n<8>(fwd, grid, shm, c.s, g)); break;
-    case 16: HYD_CHECK(launch_n<16>(fwd, grid, shm, c.s, g)); break;
-    case 32: HYD_CHECK(launch_n<32>(fwd, grid, shm, c.s, g)); break;
-    case 64: HYD_CHECK(launch_n<64>(fwd, grid, shm, c.s, g)); break;
-    case 128: HYD_CHECK(launch_n<128>(fwd, grid, shm, c.s, g)); break;
-    case 256: HYD_CHECK(launch_n<256>(fwd, grid, shm, c.s, g)); break;
-    case 512: HYD_CHECK(launch_n<512>(fwd, grid, shm, c.s, g)); break;
-    case 1024: HYD_CHECK(launch_n<1024>(fwd, grid, shm, c.s, g)); break;
-    case 2048: HYD_CHECK(launch_n<2048>(fwd, grid, shm, c.s, g)); break;
-    case 4096: HYD_CHECK(launch_n<4096>(fwd, grid, shm, c.s, g)); break;
+    case 8: HYD_CHECK((launch_n<8, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
+    case 16: HYD_CHECK((launch_n<16, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
+    case 32: HYD_CHECK((launch_n<32, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
+    case 64: HYD_CHECK((launch_n<64, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
+    case 128: HYD_CHECK((launch_n<128, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
+    case 256: HYD_CHECK((launch_n<256, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
+    case 512: HYD_CHECK((launch_n<512, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
+    case 1024: HYD_CHECK((launch_n<1024, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
+    case 2048: HYD_CHECK((launch_n<2048, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
+    case 4096: HYD_CHECK((launch_n<4096, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
     default:
       set_error("fft: transform length must be a power of two in [8, 4096]");
       return HYD_EFACTOR;
   }
   HYD_HIP(hipGetLastError());
   return HYD_OK;
+}
+
+static int fft_launch(Ctx &c, bool fwd, const double *in, double *out, int n, int m,
+                      int in_rows, int out_rows, int pre, const double *dk, double post,
+                      const double *trig) {
+  return fft_launch_t<double, double, double>(c, fwd, 1, &in, &out, n, m, in_rows, out_rows,
+                                              &pre, &dk, &post, trig);
+}
+
+template <class TI, class T, class TO>
+static int fft_launch1(Ctx &c, bool fwd, const TI *in, TO *out, int n, int m, int in_rows,
+                       int out_rows, int pre, const double *dk, double post, const T *trig) {
+  return fft_launch_t<TI, T, TO>(c, fwd, 1, &in, &out, n, m, in_rows, out_rows, &pre, &dk, &post,
+                                 trig);
 }
 
 // Autotuner: for each layout class (forward strided->rows, forward
@@ -515,6 +550,88 @@ int fft_spctop(Ctx &c, const double *svar, double *rvar, int pre, double post_si
 }
 
 // forfft(m, ng, x) / revfft(m, ng, x) on a strided (m, ng) array in place.
+// Mixed-precision transforms (FP32 arithmetic, FP64 spectral state):
+// ptospc from an FP32 physical field to FP64 spectral coefficients, spctop
+// from FP64 coefficients (optionally differentiated) to an FP32 field.
+int fft_ptospc32(Ctx &c, const float *rvar, double *svar) {
+  const int n = c.c.nx;
+  HYD_CHECK((fft_launch1<float, float, float>(c, true, rvar, c.tmp32, n, n, 0, 1, 0, nullptr,
+                                                c.plan.fnorm, c.xtrig32)));
+  HYD_CHECK((fft_launch1<float, float, double>(c, true, c.tmp32, svar, n, n, 0, 0, 0, nullptr,
+                                                 c.plan.fnorm, c.ytrig32)));
+  return HYD_OK;
+}
+
+int fft_spctop32(Ctx &c, const double *svar, float *rvar, int pre, double post_sign) {
+  const int n = c.c.nx;
+  const double *dk = pre == PRE_XDERIV ? c.hrkx : (pre == PRE_YDERIV ? c.hrky : nullptr);
+  HYD_CHECK((fft_launch1<double, float, float>(c, false, svar, c.tmp32, n, n, 0, 0, pre, dk,
+                                                 c.plan.rnorm, c.ytrig32)));
+  HYD_CHECK((fft_launch1<float, float, float>(c, false, c.tmp32, rvar, n, n, 1, 0, 0, nullptr,
+                                                c.plan.rnorm * post_sign, c.xtrig32)));
+  return HYD_OK;
+}
+
+// Batches of F <= 4 fields: two launches whatever F is.
+int fft_spctop_n(Ctx &c, int F, const double *const *svar, double *const *rvar, const int *pre,
+                 const double *post_sign, double *const *tmp) {
+  const int n = c.c.nx;
+  KTimer t(c, "fft_spctop", 4 * 8 * c.N * F);
+  const double *dk[kMaxF];
+  double p1[kMaxF], p2[kMaxF];
+  for (int f = 0; f < F; f++) {
+    dk[f] = pre[f] == PRE_XDERIV ? c.hrkx : (pre[f] == PRE_YDERIV ? c.hrky : nullptr);
+    p1[f] = c.plan.rnorm;
+    p2[f] = c.plan.rnorm * post_sign[f];
+  }
+  const int zero[kMaxF] = {0, 0, 0, 0};
+  HYD_CHECK((fft_launch_t<double, double, double>(c, false, F, svar, tmp, n, n, 0, 0, pre, dk, p1,
+                                                   c.ytrig)));
+  HYD_CHECK((fft_launch_t<double, double, double>(c, false, F, (const double *const *)tmp, rvar, n,
+                                                   n, 1, 0, zero, nullptr, p2, c.xtrig)));
+  return HYD_OK;
+}
+
+int fft_ptospc_n(Ctx &c, int F, const double *const *rvar, double *const *svar) {
+  const int n = c.c.nx;
+  KTimer t(c, "fft_ptospc", 4 * 8 * c.N * F);
+  const double p[kMaxF] = {c.plan.fnorm, c.plan.fnorm, c.plan.fnorm, c.plan.fnorm};
+  HYD_CHECK((fft_launch_t<double, double, double>(c, true, F, rvar, svar, n, n, 0, 1, nullptr,
+                                                   nullptr, p, c.xtrig)));
+  HYD_CHECK((fft_launch_t<double, double, double>(c, true, F, (const double *const *)svar, svar, n,
+                                                   n, 0, 0, nullptr, nullptr, p, c.ytrig)));
+  return HYD_OK;
+}
+
+int fft_spctop32_n(Ctx &c, int F, const double *const *svar, float *const *rvar, const int *pre,
+                   const double *post_sign, float *const *tmp) {
+  const int n = c.c.nx;
+  const double *dk[kMaxF];
+  double p1[kMaxF], p2[kMaxF];
+  for (int f = 0; f < F; f++) {
+    dk[f] = pre[f] == PRE_XDERIV ? c.hrkx : (pre[f] == PRE_YDERIV ? c.hrky : nullptr);
+    p1[f] = c.plan.rnorm;
+    p2[f] = c.plan.rnorm * post_sign[f];
+  }
+  const int zero[kMaxF] = {0, 0, 0, 0};
+  HYD_CHECK((fft_launch_t<double, float, float>(c, false, F, svar, tmp, n, n, 0, 0, pre, dk, p1,
+                                                 c.ytrig32)));
+  HYD_CHECK((fft_launch_t<float, float, float>(c, false, F, (const float *const *)tmp, rvar, n, n,
+                                                1, 0, zero, nullptr, p2, c.xtrig32)));
+  return HYD_OK;
+}
+
+int fft_ptospc32_n(Ctx &c, int F, const float *const *rvar, double *const *svar,
+                   float *const *tmp) {
+  const int n = c.c.nx;
+  const double p[kMaxF] = {c.plan.fnorm, c.plan.fnorm, c.plan.fnorm, c.plan.fnorm};
+  HYD_CHECK((fft_launch_t<float, float, float>(c, true, F, rvar, tmp, n, n, 0, 1, nullptr, nullptr,
+                                                p, c.xtrig32)));
+  HYD_CHECK((fft_launch_t<float, float, double>(c, true, F, (const float *const *)tmp, svar, n, n,
+                                                 0, 0, nullptr, nullptr, p, c.ytrig32)));
+  return HYD_OK;
+}
+
 int fft_forward_1d(Ctx &c, double *x, int m) {
   return fft_launch(c, true, x, x, c.c.nx, m, 0, 0, 0, nullptr, c.plan.fnorm, c.xtrig);
 }

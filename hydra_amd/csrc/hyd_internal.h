@@ -122,6 +122,15 @@ struct Ctx {
   int32_t *replay = nullptr;   // bucket entries (node << 1 | sign)
   double *replay_val = nullptr;
   int64_t replay_cap = 0;
+  // mixed precision (precision == 1): FP32 physical fields and twiddles
+  float *uu32 = nullptr, *vv32 = nullptr, *fa32 = nullptr, *fb32 = nullptr, *fc32 = nullptr;
+  float *tmp32 = nullptr, *fq32 = nullptr, *mq32 = nullptr;
+  float *fd32 = nullptr, *fe32 = nullptr, *t32[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *t4[4] = {nullptr, nullptr, nullptr, nullptr};  // FFT temporaries of a batch
+  float *xtrig32 = nullptr, *ytrig32 = nullptr;
+  int32_t *qb32 = nullptr;        // bottom-row level counts
+  double *qdiss_u_dev = nullptr;  // distinct qdiss values (device exp)
+  double *part = nullptr;         // per-block partial sums (1024)
   // nodes
   double *xq = nullptr, *yq = nullptr, *xqi = nullptr, *yqi = nullptr, *xqf = nullptr;
   double *yqf = nullptr, *uq = nullptr, *vq = nullptr;
@@ -155,6 +164,17 @@ int fft_ptospc(Ctx &c, const double *rvar, double *svar);
 int fft_spctop(Ctx &c, const double *svar, double *rvar, int pre, double post_sign,
                double *tmp);
 int fft_forward_1d(Ctx &c, double *x, int m);   // forfft(m, ng, x)
+int fft_ptospc32(Ctx &c, const float *rvar, double *svar);
+int fft_spctop_n(Ctx &c, int F, const double *const *svar, double *const *rvar, const int *pre,
+                 const double *post_sign, double *const *tmp);
+int fft_ptospc_n(Ctx &c, int F, const double *const *rvar, double *const *svar);
+int fft_spctop32_n(Ctx &c, int F, const double *const *svar, float *const *rvar, const int *pre,
+                   const double *post_sign, float *const *tmp);
+int fft_ptospc32_n(Ctx &c, int F, const float *const *rvar, double *const *svar,
+                   float *const *tmp);
+int fft_spctop32(Ctx &c, const double *svar, float *rvar, int pre, double post_sign);
+int con2grid_mixed(Ctx &c, float *qc);
+int advance_mixed(Ctx &c);
 int fft_autotune(Ctx &c);
 int fft_reverse_1d(Ctx &c, double *x, int m);   // revfft(m, ng, x)
 

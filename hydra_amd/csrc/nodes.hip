@@ -22,9 +22,10 @@ __device__ __forceinline__ double wrapy(double yy, const NodeGeom &g) {
 
 // Bilinear interpolation of (uu,vv) at node i; ixp/iyp of contours.f90:58-66
 // are the wrap ix0 == nx -> 1.
-__device__ __forceinline__ void interp(const double *__restrict__ uu,
-                                       const double *__restrict__ vv, double x, double y,
-                                       const NodeGeom &g, double &uo, double &vo) {
+template <class G>
+__device__ __forceinline__ void interp(const G *__restrict__ uu, const G *__restrict__ vv,
+                                       double x, double y, const NodeGeom &g, double &uo,
+                                       double &vo) {
   // positions are kept in [-L/2, L/2) by the oms wrap, so 1 <= ix0 <= nx;
   // the clamps only keep a corrupted input inside the arrays
   const double xx = g.glxi * (x - g.xmin);
@@ -41,8 +42,10 @@ __device__ __forceinline__ void interp(const double *__restrict__ uu,
   const int64_t c01 = (int64_t)(ix1 - 1) * g.ny + (iy0 - 1);
   const int64_t c10 = (int64_t)(ix0 - 1) * g.ny + (iy1 - 1);
   const int64_t c11 = (int64_t)(ix1 - 1) * g.ny + (iy1 - 1);
-  uo = pyc * (pxc * uu[c00] + px * uu[c01]) + py * (pxc * uu[c10] + px * uu[c11]);
-  vo = pyc * (pxc * vv[c00] + px * vv[c01]) + py * (pxc * vv[c10] + px * vv[c11]);
+  uo = pyc * (pxc * (double)uu[c00] + px * (double)uu[c01]) +
+       py * (pxc * (double)uu[c10] + px * (double)uu[c11]);
+  vo = pyc * (pxc * (double)vv[c00] + px * (double)vv[c01]) +
+       py * (pxc * (double)vv[c10] + px * (double)vv[c11]);
 }
 
 __global__ void velint_kernel(const double *__restrict__ uu, const double *__restrict__ vv,
@@ -57,8 +60,10 @@ __global__ void velint_kernel(const double *__restrict__ uu, const double *__res
   vq[i] = v;
 }
 
-// Fused velint + RK stage update.  stage 1..4 as in evolution.f90.
-__global__ void rk_nodes_kernel(const double *__restrict__ uu, const double *__restrict__ vv,
+// Fused velint + RK stage update.  stage 1..4 as in evolution.f90.  G is the
+// grid type (double in the plain step, float in the mixed-precision step).
+template <class G>
+__global__ void rk_nodes_kernel(const G *__restrict__ uu, const G *__restrict__ vv,
                                 double *__restrict__ xq, double *__restrict__ yq,
                                 double *__restrict__ xqi, double *__restrict__ yqi,
                                 double *__restrict__ xqf, double *__restrict__ yqf, int n,
@@ -123,8 +128,12 @@ int rk_nodes(Ctx &c, int stage) {
   if (n <= 0 || !c.contex) return HYD_OK;
   static const int64_t words[5] = {0, 8, 10, 10, 6};
   KTimer t(c, "rk_nodes", (int64_t)n * 8 * words[stage] + 16 * c.N);
-  hipLaunchKernelGGL(rk_nodes_kernel, dim3((n + 255) / 256), dim3(256), 0, c.s, c.uu, c.vv, c.xq,
-                     c.yq, c.xqi, c.yqi, c.xqf, c.yqf, n, stage, c.dsc, geom(c));
+  if (c.p.precision == 1)
+    hipLaunchKernelGGL(rk_nodes_kernel<float>, dim3((n + 255) / 256), dim3(256), 0, c.s, c.uu32,
+                       c.vv32, c.xq, c.yq, c.xqi, c.yqi, c.xqf, c.yqf, n, stage, c.dsc, geom(c));
+  else
+    hipLaunchKernelGGL(rk_nodes_kernel<double>, dim3((n + 255) / 256), dim3(256), 0, c.s, c.uu,
+                       c.vv, c.xq, c.yq, c.xqi, c.yqi, c.xqf, c.yqf, n, stage, c.dsc, geom(c));
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }

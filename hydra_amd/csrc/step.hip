@@ -213,9 +213,14 @@ int inversion(Ctx &c) {
     hipLaunchKernelGGL(k_green, grid1(N), dim3(256), 0, c.s, c.pp, c.green, c.qq, N);
   }
   HYD_HIP(hipGetLastError());
-  // main_invert: vv = spctop(xderiv(pp)), uu = -spctop(yderiv(pp))
-  HYD_CHECK(fft_spctop(c, c.pp, c.vv, PRE_XDERIV, 1.0, c.wc));
-  HYD_CHECK(fft_spctop(c, c.pp, c.uu, PRE_YDERIV, -1.0, c.wc));
+  // main_invert: vv = spctop(xderiv(pp)), uu = -spctop(yderiv(pp)), one batch
+  {
+    const double *in[2] = {c.pp, c.pp};
+    double *out[2] = {c.vv, c.uu};
+    const int pre[2] = {PRE_XDERIV, PRE_YDERIV};
+    const double sg[2] = {1.0, -1.0};
+    HYD_CHECK(fft_spctop_n(c, 2, in, out, pre, sg, c.t4));
+  }
   return HYD_OK;
 }
 
@@ -313,24 +318,26 @@ int adapt_finish(Ctx &c) {
 int source(Ctx &c, int lev) {
   const int64_t N = c.N;
   const int wb = c.c.beffect ? 1 : 0;
-  // qd source
-  HYD_CHECK(fft_spctop(c, c.qd, c.wa, PRE_XDERIV, 1.0, c.wc));
-  HYD_CHECK(fft_spctop(c, c.qd, c.wb, PRE_YDERIV, 1.0, c.wc));
+  // the four gradients (qd and qs, x and y) in one batch
   {
-    KTimer t(c, "advect_prod", 40 * N);
+    const double *in[4] = {c.qd, c.qd, c.qs, c.qs};
+    double *out[4] = {c.wa, c.wb, c.wc, c.wd};
+    const int pre[4] = {PRE_XDERIV, PRE_YDERIV, PRE_XDERIV, PRE_YDERIV};
+    const double sg[4] = {1.0, 1.0, 1.0, 1.0};
+    HYD_CHECK(fft_spctop_n(c, 4, in, out, pre, sg, c.t4));
+  }
+  {
+    KTimer t(c, "advect_prod", 80 * N);
     hipLaunchKernelGGL(k_advect_prod, grid1(N), dim3(256), 0, c.s, c.wa, c.wb, c.uu, c.vv,
                        c.p.beta, 0, N);
-  }
-  HYD_CHECK(fft_ptospc(c, c.wa, c.sqd));
-  // qs source
-  HYD_CHECK(fft_spctop(c, c.qs, c.wa, PRE_XDERIV, 1.0, c.wc));
-  HYD_CHECK(fft_spctop(c, c.qs, c.wb, PRE_YDERIV, 1.0, c.wc));
-  {
-    KTimer t(c, "advect_prod", 40 * N);
-    hipLaunchKernelGGL(k_advect_prod, grid1(N), dim3(256), 0, c.s, c.wa, c.wb, c.uu, c.vv,
+    hipLaunchKernelGGL(k_advect_prod, grid1(N), dim3(256), 0, c.s, c.wc, c.wd, c.uu, c.vv,
                        c.p.beta, wb, N);
   }
-  HYD_CHECK(fft_ptospc(c, c.wa, c.sqs));
+  {
+    const double *in[2] = {c.wa, c.wc};
+    double *out[2] = {c.sqd, c.sqs};
+    HYD_CHECK(fft_ptospc_n(c, 2, in, out));
+  }
   {
     KTimer t(c, "source_finish", 40 * N);
     hipLaunchKernelGGL(k_source_finish, grid1(N), dim3(256), 0, c.s, c.sqd, c.qq, c.pp, c.epq,
@@ -413,6 +420,183 @@ int evolution_init(Ctx &c) {
   HYD_CHECK(con2grid(c, c.xq, c.yq, c.nextq, c.nptq, c.wa, nullptr));
   HYD_CHECK(fft_ptospc(c, c.wa, c.qc));
   hipLaunchKernelGGL(k_init_qd, grid1(N), dim3(256), 0, c.s, c.qd, c.qs, c.qc, c.fhi, N);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+// ---- mixed precision --------------------------------------------------------
+// FP32 physical fields and transforms, FP64 spectral state, node positions and
+// RK accumulation.  No host round trip: the integrating factors use the
+// device exp, the sums are parallel reductions, so the step can be captured
+// in a graph.
+
+// qqx = -uu*qqx - vv*qqy in FP32
+PW_KERNEL(k_advect_prod32, (float *__restrict__ qqx, const float *__restrict__ qqy,
+                            const float *__restrict__ uu, const float *__restrict__ vv,
+                            float beta, int withbeta, int64_t n),
+          {
+            if (withbeta)
+              qqx[k] = -uu[k] * qqx[k] - vv[k] * (qqy[k] + beta);
+            else
+              qqx[k] = -uu[k] * qqx[k] - vv[k] * qqy[k];
+          })
+
+// sum of zz^2 per block (FP64 accumulation) and the maxima of zz^2, uu^2+vv^2
+__global__ void __launch_bounds__(256) k_adapt_reduce32(const float *__restrict__ zz,
+                                                        const float *__restrict__ uu,
+                                                        const float *__restrict__ vv, int64_t n,
+                                                        double *__restrict__ part,
+                                                        unsigned long long *__restrict__ mx) {
+  double s = 0.0, z2m = 0.0, u2m = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const double z = (double)zz[k];
+    const double z2 = z * z;
+    s += z2;
+    const double u = (double)uu[k], v = (double)vv[k];
+    z2m = fmax(z2m, z2);
+    u2m = fmax(u2m, u * u + v * v);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_down(s, off);
+    z2m = fmax(z2m, __shfl_down(z2m, off));
+    u2m = fmax(u2m, __shfl_down(u2m, off));
+  }
+  __shared__ double ss[4], sz[4], su[4];
+  if ((threadIdx.x & 63) == 0) {
+    ss[threadIdx.x >> 6] = s;
+    sz[threadIdx.x >> 6] = z2m;
+    su[threadIdx.x >> 6] = u2m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      s += ss[w];
+      z2m = fmax(z2m, sz[w]);
+      u2m = fmax(u2m, su[w]);
+    }
+    part[blockIdx.x] = s;
+    atomicMax(&mx[0], (unsigned long long)__double_as_longlong(z2m));
+    atomicMax(&mx[1], (unsigned long long)__double_as_longlong(u2m));
+  }
+}
+
+__global__ void k_sum_parts(const double *part, int np, double *out, unsigned long long *mx,
+                            int clear) {
+  if (threadIdx.x != 0) return;
+  if (clear) {
+    mx[0] = 0;
+    mx[1] = 0;
+    return;
+  }
+  double s = 0.0;
+  for (int i = 0; i < np; i++) s += part[i];
+  *out = s;
+}
+
+// exp(dt/2*zzrms*qdiss) for the distinct qdiss values, on the device
+__global__ void k_exp_table(const double *__restrict__ qu, int nqu, const Scalars *sc,
+                            double *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nqu) return;
+  out[j] = exp(sc->dt2 * sc->zzrms * qu[j]);
+}
+
+static int adapt_mixed(Ctx &c) {
+  const int64_t N = c.N;
+  const double *src = c.qq;
+  if (!c.c.btropic) {
+    hipLaunchKernelGGL(k_qq_kd, grid1(N), dim3(256), 0, c.s, c.wa, c.qq, c.pp, c.c.kdsq, N);
+    src = c.wa;
+  }
+  HYD_CHECK(fft_spctop32(c, src, c.fc32, PRE_NONE, 1.0));
+  unsigned long long *mx = (unsigned long long *)(c.dsum + 4);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, c.s, c.part, 0, c.dsum + 1, mx, 1);
+  hipLaunchKernelGGL(k_adapt_reduce32, dim3(1024), dim3(256), 0, c.s, c.fc32, c.uu32, c.vv32, N,
+                     c.part, mx);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, c.s, c.part, 1024, c.dsum + 1, mx, 0);
+  AdaptArgs a;
+  a.dsumi = c.c.dsumi;
+  a.small = c.c.small;
+  a.glx = c.c.glx;
+  a.srwfm = c.c.srwfm;
+  a.dt_fixed = c.p.dt_fixed;
+  a.tgsave = c.p.tgsave;
+  a.tcsave = c.p.tcsave;
+  hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, mx, a);
+  hipLaunchKernelGGL(k_exp_table, dim3((c.nqu + 255) / 256), dim3(256), 0, c.s, c.qdiss_u_dev,
+                     c.nqu, c.dsc, c.qdiss_exp);
+  hipLaunchKernelGGL(k_expand_epq, grid1(N), dim3(256), 0, c.s, c.epq, c.emq, c.qdiss_map,
+                     c.qdiss_exp, N);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+static int source_mixed(Ctx &c, int lev) {
+  const int64_t N = c.N;
+  const int wb = c.c.beffect ? 1 : 0;
+  {
+    const double *in[4] = {c.qd, c.qd, c.qs, c.qs};
+    float *out[4] = {c.fa32, c.fb32, c.fd32, c.fe32};
+    const int pre[4] = {PRE_XDERIV, PRE_YDERIV, PRE_XDERIV, PRE_YDERIV};
+    const double sg[4] = {1.0, 1.0, 1.0, 1.0};
+    HYD_CHECK(fft_spctop32_n(c, 4, in, out, pre, sg, c.t32));
+  }
+  hipLaunchKernelGGL(k_advect_prod32, grid1(N), dim3(256), 0, c.s, c.fa32, c.fb32, c.uu32,
+                     c.vv32, (float)c.p.beta, 0, N);
+  hipLaunchKernelGGL(k_advect_prod32, grid1(N), dim3(256), 0, c.s, c.fd32, c.fe32, c.uu32,
+                     c.vv32, (float)c.p.beta, wb, N);
+  {
+    const float *in[2] = {c.fa32, c.fd32};
+    double *out[2] = {c.sqd, c.sqs};
+    HYD_CHECK(fft_ptospc32_n(c, 2, in, out, c.t32));
+  }
+  hipLaunchKernelGGL(k_source_finish, grid1(N), dim3(256), 0, c.s, c.sqd, c.qq, c.pp, c.epq,
+                     c.p.rekman, c.c.kdsq, lev, N);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+static int inversion_mixed(Ctx &c) {
+  const int64_t N = c.N;
+  HYD_CHECK(con2grid_mixed(c, c.fa32));
+  HYD_CHECK(fft_ptospc32(c, c.fa32, c.qc));
+  hipLaunchKernelGGL(k_combine_qq, grid1(N), dim3(256), 0, c.s, c.qq, c.flo, c.qs, c.qc, c.qd, N);
+  hipLaunchKernelGGL(k_green, grid1(N), dim3(256), 0, c.s, c.pp, c.green, c.qq, N);
+  HYD_HIP(hipGetLastError());
+  {
+    const double *in[2] = {c.pp, c.pp};
+    float *out[2] = {c.vv32, c.uu32};
+    const int pre[2] = {PRE_XDERIV, PRE_YDERIV};
+    const double sg[2] = {1.0, -1.0};
+    HYD_CHECK(fft_spctop32_n(c, 2, in, out, pre, sg, c.t32));
+  }
+  return HYD_OK;
+}
+
+// advance (evolution.f90:166-326) with the mixed-precision grid work
+int advance_mixed(Ctx &c) {
+  const int64_t N = c.N;
+  HYD_CHECK(inversion_mixed(c));
+  hipLaunchKernelGGL(k_reset, grid1(N), dim3(256), 0, c.s, c.qs, c.qd, c.qc, c.flo, c.fhi, N);
+  HYD_CHECK(adapt_mixed(c));
+  HYD_CHECK(source_mixed(c, 0));
+  HYD_CHECK(rk_nodes(c, 1));
+  HYD_CHECK(rk_fields(c, 1));
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
+  HYD_CHECK(inversion_mixed(c));
+  HYD_CHECK(source_mixed(c, 1));
+  HYD_CHECK(rk_nodes(c, 2));
+  HYD_CHECK(rk_fields(c, 2));
+  HYD_CHECK(inversion_mixed(c));
+  HYD_CHECK(source_mixed(c, 1));
+  HYD_CHECK(rk_nodes(c, 3));
+  HYD_CHECK(rk_fields(c, 3));
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
+  HYD_CHECK(inversion_mixed(c));
+  HYD_CHECK(source_mixed(c, 2));
+  HYD_CHECK(rk_nodes(c, 4));
+  HYD_CHECK(rk_fields(c, 4));
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }

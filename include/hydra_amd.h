@@ -53,7 +53,10 @@ typedef struct hyd_params {
   double tgsave, tcsave; /* parameters.f90:23 (only dtmax depends on them) */
   double dt_fixed;       /* > 0: fixed time step instead of adapt's choice */
   int64_t node_capacity; /* device node capacity (0: max(64*ng*ng, 2^20))   */
-  int32_t precision;     /* 0: plain FP64, operation-order faithful        */
+  int32_t precision;     /* 0: plain FP64, operation-order faithful;
+                            1: mixed -- FP32 grid fields and transforms,
+                            FP64 spectral state, nodes and RK accumulation
+                            (no reference counterpart; not bit-exact)     */
   int32_t device;        /* HIP device ordinal                             */
 } hyd_params;
 
