@@ -36,75 +36,115 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# HIP-event record name -> kernel symbol as rocprofv3 reports it
-KERNEL_SYMBOL = {"k:fft_fwd": "fft_kernel<N, true>", "k:fft_rev": "fft_kernel<N, false>"}
+# HIP-event record name -> kernel symbol as rocprofv3 reports it ({N} = ng)
+KERNEL_SYMBOL = {"k:fft_fwd": "fft_kernel<{N}, true, double, double, double>",
+                 "k:fft_rev": "fft_kernel<{N}, false, double, double, double>",
+                 "k:fft32_fwd": "fft_kernel<{N}, true, float, float, float|double>",
+                 "k:fft32_rev": "fft_kernel<{N}, false, double|float, float, float>"}
 
 
-def traffic_from_profile(kernel, ng):
-    """HBM bytes per launch of `kernel` at this ng, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel
-    of known size; see profiles/), or None when there is none."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+def traffic_from_profile(kernel, alg_bytes):
+    """HBM bytes per launch of `kernel`: its measured HBM bytes per
+    algorithmic byte (committed rocprofv3 --pmc summary: FETCH_SIZE and
+    WRITE_SIZE scaled by a copy kernel of known size) times this run's
+    algorithmic bytes per launch; None without a summary for the kernel."""
+    path = os.path.join(ROOT, "profiles", "fft_kernel_pmc_traffic_ng1024.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    rec = d.get(f"{kernel}@ng{ng}")
-    return None if rec is None else rec["traffic_bytes"]
+    rec = d.get(kernel)
+    return None if rec is None else int(round(rec["hbm_per_algorithmic_byte"] * alg_bytes))
 
 
-COPY_BYTES = 256 << 20  # each of read and write of the calibration copy
+COPY_BYTES = 96 << 20  # each of read and write of the calibration copy (<= 128 ng^2 B)
 
 
-def copy_calibration():
+def copy_calibration(h):
     """A plain device copy of known size (read COPY_BYTES, write COPY_BYTES):
     the reference point that scales the FETCH_SIZE/WRITE_SIZE counters."""
-    import torch
-    a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    torch.mul(a, 1.0, out=b)
-    torch.cuda.synchronize()
+    h.copy_calibration(COPY_BYTES)
 
 
 def pmc_summary(fetch_dir, write_dir, ng, out_json):
-    """Per-launch HBM traffic of the FFT kernels from two rocprofv3 --pmc runs
+    """HBM traffic of the FFT kernels from two rocprofv3 --pmc runs
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the same
-    counter on the calibration copy."""
+    counter on the calibration copy, as bytes per algorithmic byte.  A
+    dispatch's algorithmic bytes are (sizeof in + sizeof out) x n x vectors;
+    the vectors follow from its grid and its LDS size (2 V n + 2 n values of
+    the compute type, V vectors per workgroup of 256)."""
     import csv
     import glob
+    import re
+    size = {"double": 8, "float": 4}
 
-    def per_kernel(d, counter):
-        vals = {}
+    def rows(d, counter):
+        out = []
         for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             with open(f) as fh:
                 for row in csv.DictReader(fh):
-                    if row.get("Counter_Name") != counter:
-                        continue
-                    k = row["Kernel_Name"]
-                    key = ("copy" if "elementwise" in k or "mul" in k.lower() else
-                           "fft_kernel<N, true>" if "fft_kernel<" in k and "true>" in k else
-                           "fft_kernel<N, false>" if "fft_kernel<" in k and "false>" in k else None)
-                    if key:
-                        vals.setdefault(key, []).append(float(row["Counter_Value"]))
-        return {k: float(np.mean(v)) for k, v in vals.items()}
+                    if row.get("Counter_Name") == counter:
+                        out.append(row)
+        return out
 
-    fe = per_kernel(fetch_dir, "FETCH_SIZE")
-    wr = per_kernel(write_dir, "WRITE_SIZE")
+    def summarise(d, counter):
+        copy, per = [], {}
+        for row in rows(d, counter):
+            k = row["Kernel_Name"]
+            v = float(row["Counter_Value"])
+            if "k_copy_calib" in k:
+                copy.append(v)
+                continue
+            m = re.search(r"fft_kernel<(\d+), (true|false), (\w+), (\w+), (\w+)>", k)
+            if not m:
+                continue
+            n = int(m.group(1))
+            ti, t, to = (size[m.group(i)] for i in (3, 4, 5))
+            V = max(1, round(int(row["LDS_Block_Size"]) / (t * 2 * n) - 1))
+            vectors = int(row["Grid_Size"]) // 256 * V
+            alg = (ti + to) * n * vectors
+            rec = per.setdefault(m.group(0), [0.0, 0.0, 0])
+            rec[0] += v
+            rec[1] += alg
+            rec[2] += 1
+        return float(np.mean(copy)), per
+
+    cf, fe = summarise(fetch_dir, "FETCH_SIZE")
+    cw, wr = summarise(write_dir, "WRITE_SIZE")
     kib = COPY_BYTES / 1024.0
-    ff = kib / fe["copy"]
-    fw = kib / wr["copy"]
-    out = {"calibration": {"copy_bytes": COPY_BYTES, "fetch_kib": fe["copy"], "write_kib": wr["copy"],
+    ff, fw = kib / cf, kib / cw
+    out = {"calibration": {"copy_bytes_each_way": COPY_BYTES, "fetch_kib": cf, "write_kib": cw,
                            "fetch_factor": ff, "write_factor": fw}}
-    N = ng * ng
-    for k in ("fft_kernel<N, true>", "fft_kernel<N, false>"):
-        if k in fe and k in wr:
-            t = (fe[k] * ff + wr[k] * fw) * 1024.0
-            out[f"{k}@ng{ng}"] = {"fetch_kib": fe[k], "write_kib": wr[k], "traffic_bytes": round(t),
-                                  "algorithmic_bytes": 16 * N}
+    for k in sorted(fe):
+        if k not in wr:
+            continue
+        hbm = (fe[k][0] * ff + wr[k][0] * fw) * 1024.0
+        alg = fe[k][1]
+        out[k] = {"dispatches": fe[k][2], "fetch_kib_total": fe[k][0], "write_kib_total": wr[k][0],
+                  "algorithmic_bytes_total": alg, "hbm_per_algorithmic_byte": hbm / alg}
     with open(out_json, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out, indent=1))
+
+
+def stats_summary(db, out, title, command):
+    """Per-kernel table (calls, total and average duration in microseconds,
+    share of the kernel time) from a rocprofv3 --kernel-trace database."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, count(*), sum(end - start) from kernels "
+                       "group by name order by 3 desc").fetchall()
+    tot = sum(r[2] for r in rows)
+    lines = [f"# rocprofv3 kernel statistics: {title}", "#",
+             "# command (from the repository root on the MI355X box):"]
+    lines += ["#   " + c for c in command]
+    lines += ["# durations in microseconds", "#",
+              f"{'kernel':<72} {'calls':>6} {'total_us':>12} {'avg_us':>10} {'pct':>6}"]
+    for name, n, t in rows:
+        nm = name if len(name) <= 72 else name[:69] + "..."
+        lines.append(f"{nm:<72} {n:>6} {t / 1e3:>12.1f} {t / 1e3 / n:>10.2f} {100.0 * t / tot:>6.2f}")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
 
 
 def main():
@@ -120,11 +160,19 @@ def main():
                     help="0 plain FP64 (bit-exact), 1 mixed FP32 grid / FP64 state")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run the calibration copy (for rocprofv3 --pmc runs)")
+    ap.add_argument("--stats-summary", nargs=4, metavar=("DB", "OUT", "TITLE", "COMMAND"),
+                    help="write a per-kernel table from a rocprofv3 --kernel-trace database "
+                         "(COMMAND lines separated by ' ;; ')")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
-                    help="summarise two rocprofv3 --pmc runs into profiles/pmc_traffic.json")
+                    help="summarise two rocprofv3 --pmc runs into profiles/fft_kernel_pmc_traffic_ng<ng>.json")
     args = ap.parse_args()
+    if args.stats_summary:
+        db, out, title, cmd = args.stats_summary
+        stats_summary(db, out, title, cmd.split(" ;; "))
+        return
     if args.pmc_summary:
-        pmc_summary(*args.pmc_summary, args.ng, os.path.join(ROOT, "profiles", "pmc_traffic.json"))
+        pmc_summary(*args.pmc_summary, args.ng,
+                    os.path.join(ROOT, "profiles", f"fft_kernel_pmc_traffic_ng{args.ng}.json"))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs one flow on one GPU; multi-GPU runs are not part of it "
@@ -160,9 +208,8 @@ def main():
     for _ in range(args.warmup):
         step()
     h.sync()
-    # timed region: K steps; HIP events around each launch of the single
-    # kernels ("k:" records) on the library stream
-    h.set_timing(1)
+    # timed region: K steps, no per-kernel events (the mixed-precision step
+    # replays its captured graph here)
     h.sync()
     t0 = time.perf_counter()
     updates = 0
@@ -170,13 +217,20 @@ def main():
         updates += step()
     h.sync()
     wall = time.perf_counter() - t0
-    recs = h.kernel_times()
-    h.set_timing(0)
     P_end, nq_end = h.counts()
     ms_step = 1e3 * wall / args.steps
     value = updates / wall
     log(f"timed: {args.steps} steps {wall:.3f}s, nodes {P} -> {P_end}, contours {nq} -> {nq_end}")
 
+    # kernel durations: HIP events around each launch of the single kernels
+    # ("k:" records) on the library stream, over the next steps (eager)
+    kr = min(args.steps, 10)
+    h.set_timing(1)
+    for _ in range(kr):
+        step()
+    h.sync()
+    recs = h.kernel_times()
+    h.set_timing(0)
     tot = {}
     cnt = {}
     for name, ms, by in recs:
@@ -206,14 +260,15 @@ def main():
     avg_ms = tot[dom] / cnt[dom]
     alg = float(np.mean(kbytes[dom]))
     achieved = alg / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": KERNEL_SYMBOL.get(dom, dom), "achieved": round(achieved, 1),
+    ksym = KERNEL_SYMBOL.get(dom, dom).replace("{N}", str(ng))
+    roofline = {"bound": "hbm", "kernel": ksym, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profile(KERNEL_SYMBOL.get(dom, dom), ng),
-                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": cnt[dom] / args.steps,
+                "traffic": traffic_from_profile(ksym, alg),
+                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": cnt[dom] / kr,
                 "algorithmic_bytes_per_launch": int(alg)}
 
     if args.copy_calib:
-        copy_calibration()
+        copy_calibration(h)
     cpu = None
     if not args.no_cpu:
         try:
@@ -234,7 +289,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == 0 else "f32/f64",
         "data": "synthetic",
-        "config": {"workload": f"plain FP64 CAPS RK4 step, 2D Euler, ng={ng}, 80 PV levels, "
+        "config": {"workload": ("plain FP64" if args.precision == 0 else "mixed FP32-grid/FP64-state")
+                   + f" CAPS RK4 step, 2D Euler, ng={ng}, 80 PV levels, "
                                f"seeded k^3 exp(-2(k/6)^2) field (rms 4pi)",
                    "ng": ng, "nodes": P, "nodes_end": P_end, "contours": nq,
                    "precision": "plain-fp64" if args.precision == 0 else "mixed-fp32-grid",

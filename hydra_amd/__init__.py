@@ -91,6 +91,7 @@ def lib():
                                  C.POINTER(C.c_int32)]
         L.hyd_run.argtypes = [C.c_void_p, C.c_int32]
         L.hyd_get_tuning.argtypes = [C.c_void_p, _I]
+        L.hyd_copy_calibration.argtypes = [C.c_void_p, C.c_int64]
         L.hyd_set_timing.argtypes = [C.c_void_p, C.c_int32]
         L.hyd_kernel_times.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
                                        np.ctypeslib.ndpointer(dtype=np.float64),
@@ -285,6 +286,10 @@ class Hydra:
         v = np.zeros(4, dtype=np.int32)
         _check(self.L.hyd_get_tuning(self.h, v))
         return v.tolist()
+
+    def copy_calibration(self, nbytes):
+        """One streaming copy kernel of nbytes (traffic-counter reference)."""
+        _check(self.L.hyd_copy_calibration(self.h, int(nbytes)))
 
     def sync(self):
         _check(self.L.hyd_sync(self.h))

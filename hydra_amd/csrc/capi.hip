@@ -16,19 +16,21 @@ void set_error(const std::string &m) { g_err = m; }
 
 int inversion(Ctx &c);
 
-KTimer::KTimer(Ctx &ctx, const char *name, int64_t bytes) : c(ctx) {
+KTimer::KTimer(Ctx &ctx, const char *name, int64_t bytes, bool disp) : c(ctx), dispatch(disp) {
   if (!c.timing || (c.timing == 1 && strncmp(name, "k:", 2) != 0)) return;
   Ctx::KRec r;
   r.name = name;
   r.bytes = bytes;
   if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
-  hipEventRecord(r.a, c.s);
+  if (!dispatch) hipEventRecord(r.a, c.s);
   c.krec.push_back(r);
   idx = (int)c.krec.size() - 1;
 }
 KTimer::~KTimer() {
-  if (idx >= 0) hipEventRecord(c.krec[idx].b, c.s);
+  if (idx >= 0 && !dispatch) hipEventRecord(c.krec[idx].b, c.s);
 }
+hipEvent_t KTimer::start() const { return idx >= 0 ? c.krec[idx].a : nullptr; }
+hipEvent_t KTimer::stop() const { return idx >= 0 ? c.krec[idx].b : nullptr; }
 
 __global__ void k_xderiv(const double *__restrict__ var, double *__restrict__ der,
                          const double *__restrict__ dk, int nx, int ny) {
@@ -109,6 +111,7 @@ static void free_ctx(Ctx *c) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
   }
+  if (c->step_graph) hipGraphExecDestroy(c->step_graph);
   if (c->s) hipStreamDestroy(c->s);
   delete c;
 }
@@ -275,7 +278,7 @@ static int create(const hyd_params *p, hyd_ctx **out) {
     TRY(dalloc(&c->xtrig32, 2 * n));
     TRY(dalloc(&c->ytrig32, 2 * n));
     TRY(dalloc(&c->qdiss_u_dev, c->nqu));
-    TRY(dalloc(&c->part, 1024));
+    TRY(dalloc(&c->part, 1024 + 8));
     std::vector<float> tx(2 * n), ty(2 * n);
     for (int i = 0; i < 2 * n; i++) {
       tx[i] = (float)c->ht.xtrig[i];
@@ -458,6 +461,7 @@ int hyd_sum_sequential(hyd_ctx *h, const double *a, int64_t n, double *sum) {
 
 int hyd_set_field(hyd_ctx *h, const double *qq, double t0) {
   Ctx &c = *C(h);
+  c.graph_nptq = -1;  // state changed outside the captured step
   if (c.c.beffect) {
     set_error("beta != 0 is not supported by the device step");
     return HYD_EARG;
@@ -486,6 +490,7 @@ int hyd_set_contours(hyd_ctx *h, int32_t nptq, int32_t nq, const double *xq, con
                      const int32_t *nextq, const int32_t *i1q, const int32_t *i2q,
                      const int32_t *npq, const int32_t *indq, double qjump) {
   Ctx &c = *C(h);
+  c.graph_nptq = -1;  // state changed outside the captured step
   if (nptq < 0 || nptq > c.cap || nq < 0) {
     set_error("set_contours: counts outside capacity");
     return HYD_ECAPACITY;
@@ -607,6 +612,7 @@ static int advect_start(Ctx &c) {
 
 int hyd_evolution_init(hyd_ctx *h) {
   Ctx &c = *C(h);
+  c.graph_nptq = -1;  // state changed outside the captured step
   if (!c.have_field) {
     set_error("evolution_init before set_field");
     return HYD_ESTATE;
@@ -617,12 +623,14 @@ int hyd_evolution_init(hyd_ctx *h) {
 
 int hyd_prepare(hyd_ctx *h) {
   Ctx &c = *C(h);
+  c.graph_nptq = -1;  // state changed outside the captured step
   HYD_CHECK(prepare(c));
   return c2g_status(c);
 }
 
 int hyd_recontour(hyd_ctx *h) {
   Ctx &c = *C(h);
+  c.graph_nptq = -1;  // state changed outside the captured step
   if (!c.have_field) {
     set_error("recontour before set_field");
     return HYD_ESTATE;
@@ -666,6 +674,34 @@ int hyd_renode(hyd_ctx *h, const double *xd, const double *yd, int32_t npd, doub
   return HYD_OK;
 }
 
+// The mixed-precision step has no host round trip: after one eager call it
+// is captured as a graph and replayed; the capture is redone whenever the
+// node count changed (surgery, recontouring) and skipped while timing.
+static int advance_mixed_graph(Ctx &c) {
+  if (c.timing || c.mixed_calls++ == 0) return advance_mixed(c);
+  if (!c.step_graph || c.graph_nptq != c.nptq) {
+    if (c.step_graph) {
+      HYD_HIP(hipGraphExecDestroy(c.step_graph));
+      c.step_graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HYD_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+    const int rc = advance_mixed(c);
+    const hipError_t e = hipStreamEndCapture(c.s, &g);
+    if (rc != HYD_OK) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    HYD_HIP(e);
+    const hipError_t ei = hipGraphInstantiate(&c.step_graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HYD_HIP(ei);
+    c.graph_nptq = c.nptq;
+  }
+  HYD_HIP(hipGraphLaunch(c.step_graph, c.s));
+  return HYD_OK;
+}
+
 int hyd_inversion(hyd_ctx *h) {
   Ctx &c = *C(h);
   HYD_CHECK(inversion(c));
@@ -679,13 +715,17 @@ int hyd_advance(hyd_ctx *h) {
     set_error("advance before set_field");
     return HYD_ESTATE;
   }
-  HYD_CHECK(c.p.precision == 1 ? advance_mixed(c) : advance(c));
+  if (c.p.precision == 1)
+    HYD_CHECK(advance_mixed_graph(c));
+  else
+    HYD_CHECK(advance(c));
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
   return c2g_status(c);
 }
 
 int hyd_surgery(hyd_ctx *h) {
   Ctx &c = *C(h);
+  c.graph_nptq = -1;  // state changed outside the captured step
   HYD_CHECK(surgery(c));
   HYD_HIP(hipStreamSynchronize(c.s));
   return HYD_OK;
@@ -740,6 +780,28 @@ int hyd_run(hyd_ctx *h, int32_t nsteps) {
       done++;
     }
   }
+  HYD_HIP(hipStreamSynchronize(c.s));
+  return HYD_OK;
+}
+
+__global__ void k_copy_calib(const double *__restrict__ a, double *__restrict__ b, int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    b[k] = a[k];
+}
+
+int hyd_copy_calibration(hyd_ctx *h, int64_t nbytes) {
+  Ctx &c = *C(h);
+  const int64_t n = nbytes / 8;
+  if (n <= 0 || n > 16 * c.N) {
+    set_error("copy_calibration: size outside (0, 128 ng^2] bytes");
+    return HYD_EARG;
+  }
+  // reads the 16 ng^2-double fine-grid buffer, writes a work buffer
+  double *dst;
+  HYD_CHECK(dbuf(c, "calib_dst", n, &dst));
+  hipLaunchKernelGGL(k_copy_calib, dim3(2048), dim3(256), 0, c.s, c.fq, dst, n);
+  HYD_HIP(hipGetLastError());
   HYD_HIP(hipStreamSynchronize(c.s));
   return HYD_OK;
 }

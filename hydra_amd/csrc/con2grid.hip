@@ -577,17 +577,23 @@ __global__ void c2g_sum_partial(const float *a, int64_t n, double *part) {
   if (threadIdx.x == 0) part[blockIdx.x] = w[0] + w[1] + w[2] + w[3];
 }
 
-__global__ void c2g_subtract_mean(const float *qa, float *qc, int64_t n, const double *part,
-                                  int np, double ndiv) {
-  __shared__ double mean;
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < np; i++) s += part[i];
-    mean = s / ndiv;
-  }
+// part[1024] <- mean of the 1024 partial sums' total over ndiv (one
+// workgroup, fixed-order tree: deterministic)
+__global__ void __launch_bounds__(1024) c2g_mean(double *part, double ndiv) {
+  __shared__ double s[1024];
+  const int t = threadIdx.x;
+  s[t] = part[t];
   __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (t < o) s[t] += s[t + o];
+    __syncthreads();
+  }
+  if (t == 0) part[1024] = s[0] / ndiv;
+}
+
+__global__ void c2g_subtract_mean(const float *qa, float *qc, int64_t n, const double *mean) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) qc[k] = (float)((double)qa[k] - mean);
+  if (k < n) qc[k] = (float)((double)qa[k] - *mean);
 }
 
 int con2grid_mixed(Ctx &c, float *qc) {
@@ -643,8 +649,9 @@ int con2grid_mixed(Ctx &c, float *qc) {
   hipLaunchKernelGGL(c2g_avg9<float>, dim3((nh2 + 255) / 256, nh2), dim3(256), 0, c.s, c.mq32,
                      c.fc32, nh2);
   hipLaunchKernelGGL(c2g_sum_partial, dim3(1024), dim3(256), 0, c.s, c.fc32, N, c.part);
+  hipLaunchKernelGGL(c2g_mean, dim3(1), dim3(1024), 0, c.s, c.part, (double)N);
   hipLaunchKernelGGL(c2g_subtract_mean, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.s,
-                     c.fc32, qc, N, c.part, 1024, (double)N);
+                     c.fc32, qc, N, c.part + 1024);
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }

@@ -20,6 +20,8 @@
 // rows (this is the transpose of sta2dfft.f90:122-126), then an in-place
 // strided y-transform.  spctop = strided y-inverse into a work array, then
 // an x-inverse reading rows and writing strided (sta2dfft.f90:152-162).
+#include <hip/hip_ext.h>
+
 #include "hyd_internal.h"
 
 namespace hyd {
@@ -389,7 +391,8 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
 }
 
 template <int N, class TI, class T, class TO>
-static int launch_n(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftArgs &g) {
+static int launch_n(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftArgs &g,
+                    hipEvent_t ea, hipEvent_t eb) {
   // more than 64 KiB of LDS per workgroup must be requested (gfx950: 160 KiB)
   static bool raised = false;
   if (!raised && shm > 65536) {
@@ -399,10 +402,20 @@ static int launch_n(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftArg
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     raised = true;
   }
-  if (fwd)
+  // with timing events the launch records them at the dispatch's own start
+  // and end; without, an ordinary launch (the one stream capture records)
+  if (ea && eb) {
+    if (fwd)
+      hipExtLaunchKernelGGL((fft_kernel<N, true, TI, T, TO>), grid, dim3(256), (uint32_t)shm, s,
+                            ea, eb, 0u, g);
+    else
+      hipExtLaunchKernelGGL((fft_kernel<N, false, TI, T, TO>), grid, dim3(256), (uint32_t)shm, s,
+                            ea, eb, 0u, g);
+  } else if (fwd) {
     hipLaunchKernelGGL((fft_kernel<N, true, TI, T, TO>), grid, dim3(256), shm, s, g);
-  else
+  } else {
     hipLaunchKernelGGL((fft_kernel<N, false, TI, T, TO>), grid, dim3(256), shm, s, g);
+  }
   return HYD_OK;
 }
 
@@ -446,18 +459,18 @@ static int fft_launch_t(Ctx &c, bool fwd, int F, const TI *const *in, TO *const 
   const dim3 grid(m / V, F);
   // one launch reads and writes F x m vectors of n values
   KTimer tk(c, sizeof(T) == 8 ? (fwd ? "k:fft_fwd" : "k:fft_rev") : (fwd ? "k:fft32_fwd" : "k:fft32_rev"),
-            (int64_t)(sizeof(TI) + sizeof(TO)) * n * m * F);
+            (int64_t)(sizeof(TI) + sizeof(TO)) * n * m * F, true);
   switch (n) {
-    case 8: HYD_CHECK((launch_n<8, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
-    case 16: HYD_CHECK((launch_n<16, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
-    case 32: HYD_CHECK((launch_n<32, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
-    case 64: HYD_CHECK((launch_n<64, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
-    case 128: HYD_CHECK((launch_n<128, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
-    case 256: HYD_CHECK((launch_n<256, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
-    case 512: HYD_CHECK((launch_n<512, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
-    case 1024: HYD_CHECK((launch_n<1024, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
-    case 2048: HYD_CHECK((launch_n<2048, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
-    case 4096: HYD_CHECK((launch_n<4096, TI, T, TO>(fwd, grid, shm, c.s, g))); break;
+    case 8: HYD_CHECK((launch_n<8, TI, T, TO>(fwd, grid, shm, c.s, g, tk.start(), tk.stop()))); break;
+    case 16: HYD_CHECK((launch_n<16, TI, T, TO>(fwd, grid, shm, c.s, g, tk.start(), tk.stop()))); break;
+    case 32: HYD_CHECK((launch_n<32, TI, T, TO>(fwd, grid, shm, c.s, g, tk.start(), tk.stop()))); break;
+    case 64: HYD_CHECK((launch_n<64, TI, T, TO>(fwd, grid, shm, c.s, g, tk.start(), tk.stop()))); break;
+    case 128: HYD_CHECK((launch_n<128, TI, T, TO>(fwd, grid, shm, c.s, g, tk.start(), tk.stop()))); break;
+    case 256: HYD_CHECK((launch_n<256, TI, T, TO>(fwd, grid, shm, c.s, g, tk.start(), tk.stop()))); break;
+    case 512: HYD_CHECK((launch_n<512, TI, T, TO>(fwd, grid, shm, c.s, g, tk.start(), tk.stop()))); break;
+    case 1024: HYD_CHECK((launch_n<1024, TI, T, TO>(fwd, grid, shm, c.s, g, tk.start(), tk.stop()))); break;
+    case 2048: HYD_CHECK((launch_n<2048, TI, T, TO>(fwd, grid, shm, c.s, g, tk.start(), tk.stop()))); break;
+    case 4096: HYD_CHECK((launch_n<4096, TI, T, TO>(fwd, grid, shm, c.s, g, tk.start(), tk.stop()))); break;
     default:
       set_error("fft: transform length must be a power of two in [8, 4096]");
       return HYD_EFACTOR;

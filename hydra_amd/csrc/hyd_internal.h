@@ -151,6 +151,10 @@ struct Ctx {
   int64_t seq_cap = 0;      // chunks it holds
   // work buffers of recontour / renode, allocated on first use
   std::map<std::string, DBuf> cg_bufs;
+  // captured mixed-precision step (replayed while the node count is unchanged)
+  hipGraphExec_t step_graph = nullptr;
+  int graph_nptq = -1;
+  int mixed_calls = 0;
   // timing
   int timing = 0;  // 0 off, 1 single-kernel ("k:") records only, 2 all records
   struct KRec { std::string name; hipEvent_t a, b; int64_t bytes; };
@@ -225,8 +229,14 @@ int renode_batch(Ctx &c, const double *xd, const double *yd, const int64_t *off_
 struct KTimer {
   Ctx &c;
   int idx = -1;
-  KTimer(Ctx &ctx, const char *name, int64_t bytes);
+  bool dispatch = false;
+  // dispatch = true: the caller launches one kernel with start()/stop() as
+  // the launch's own events (hipExtLaunchKernel), which time the dispatch
+  // itself rather than markers placed around it on the stream
+  KTimer(Ctx &ctx, const char *name, int64_t bytes, bool dispatch = false);
   ~KTimer();
+  hipEvent_t start() const;
+  hipEvent_t stop() const;
 };
 
 }  // namespace hyd

@@ -481,17 +481,21 @@ __global__ void __launch_bounds__(256) k_adapt_reduce32(const float *__restrict_
   }
 }
 
-__global__ void k_sum_parts(const double *part, int np, double *out, unsigned long long *mx,
-                            int clear) {
-  if (threadIdx.x != 0) return;
-  if (clear) {
-    mx[0] = 0;
-    mx[1] = 0;
-    return;
+__global__ void k_clear_max(unsigned long long *mx) {
+  if (threadIdx.x < 2) mx[threadIdx.x] = 0;
+}
+
+// total of 1024 partial sums (one workgroup, fixed-order tree)
+__global__ void __launch_bounds__(1024) k_sum_parts(const double *part, double *out) {
+  __shared__ double s[1024];
+  const int t = threadIdx.x;
+  s[t] = part[t];
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (t < o) s[t] += s[t + o];
+    __syncthreads();
   }
-  double s = 0.0;
-  for (int i = 0; i < np; i++) s += part[i];
-  *out = s;
+  if (t == 0) *out = s[0];
 }
 
 // exp(dt/2*zzrms*qdiss) for the distinct qdiss values, on the device
@@ -511,10 +515,10 @@ static int adapt_mixed(Ctx &c) {
   }
   HYD_CHECK(fft_spctop32(c, src, c.fc32, PRE_NONE, 1.0));
   unsigned long long *mx = (unsigned long long *)(c.dsum + 4);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, c.s, c.part, 0, c.dsum + 1, mx, 1);
+  hipLaunchKernelGGL(k_clear_max, dim3(1), dim3(64), 0, c.s, mx);
   hipLaunchKernelGGL(k_adapt_reduce32, dim3(1024), dim3(256), 0, c.s, c.fc32, c.uu32, c.vv32, N,
                      c.part, mx);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, c.s, c.part, 1024, c.dsum + 1, mx, 0);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, c.s, c.part, c.dsum + 1);
   AdaptArgs a;
   a.dsumi = c.c.dsumi;
   a.small = c.c.small;

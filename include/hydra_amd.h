@@ -171,6 +171,10 @@ int hyd_sync(hyd_ctx *ctx);
    strided->rows, forward strided->strided, reverse strided->strided and
    reverse rows->strided (v[0..3]). */
 int hyd_get_tuning(hyd_ctx *ctx, int32_t *v);
+/* No reference counterpart (measurement aid): one plain streaming copy
+   kernel of nbytes (<= 128 ng^2) read and nbytes written, the reference
+   point that scales hardware traffic counters. */
+int hyd_copy_calibration(hyd_ctx *ctx, int64_t nbytes);
 /* No reference counterpart: per-launch timing with HIP events on the
    library's stream.  Level 0 off, 1 the single kernels whose records start
    with "k:" (one record per launch), 2 every timed group as well.

@@ -53,3 +53,31 @@ def test_mixed_run_with_surgery():
     assert n > 1000 and nc > 10
     assert np.isfinite(h.field("qs")).all()
     h.close()
+
+
+def test_mixed_graph_replay_equals_eager():
+    """The captured step (replayed from the second call on) against the eager
+    step (timing on forces it): bit-identical after every step, with host
+    reads and synchronisation between the replays."""
+    from hydra_amd import Hydra
+    from hydra_amd.ic import random_pv
+    ng = 64
+    qq = random_pv(ng, seed=5)
+    hs = []
+    for eager in (True, False):
+        h = Hydra(ng, precision=1)
+        h.set_field(qq)
+        h.recontour()
+        h.evolution_init()
+        if eager:
+            h.set_timing(2)
+        hs.append(h)
+    for step in range(6):
+        for h in hs:
+            h.advance()
+        a, b = (h.contours() for h in hs)
+        assert np.array_equal(a["x"].view(np.int64), b["x"].view(np.int64)), step
+        assert np.array_equal(hs[0].field("qs").view(np.int64), hs[1].field("qs").view(np.int64))
+        assert hs[0].scalar("t") == hs[1].scalar("t")
+    for h in hs:
+        h.close()
