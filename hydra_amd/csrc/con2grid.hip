@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(1024) c2g_qbot(const int32_t *qjx, double *qbo
         while (mask) {
           const int b = __ffsll((unsigned long long)mask) - 1;
           mask &= mask - 1;
-          v = __shfl(cv, b) + v;
+          v = readlane_f64(cv, b) + v;
           if (l >= b) mine = v;
         }
         if (ok) qbot[part + t + 1] = mine;
@@ -348,7 +348,7 @@ __global__ void c2g_colscan(const int32_t *cnt, const double *qbot, double *fq, 
       while (mask) {
         const int b = __ffsll((unsigned long long)mask) - 1;
         mask &= mask - 1;
-        v = __shfl(cv, b) + v;
+        v = readlane_f64(cv, b) + v;
         if (l >= b) mine = v;
       }
       if (r < nyf) out[r] = mine;
@@ -375,6 +375,78 @@ __global__ void c2g_avg9(const T *__restrict__ f, T *__restrict__ out, int nh) {
       (T)0.125 * (qev1_a + qev1_b + qod0 + qod2) + (T)0.25 * qod1;
 }
 
+// y sweep fused with the first nine-point average (contours.f90:272-277
+// and 298-326, the 4ng -> 2ng step): a workgroup of 2K+1 wavefronts sweeps
+// the fine columns 2 i0 - 1 .. 2 i0 + 2K - 1 (one each, in the order of the
+// y sweep, 64 rows at a time) and averages them into output columns
+// i0 .. i0+K-1 through LDS, so the fine field never reaches HBM.  Output
+// row 0 needs the last fine row (periodic) and is written at the end.
+// Needs n2 % 64 == 0 and (n2 / 2) % K == 0.
+static constexpr int kAvgK = 4;
+static constexpr int kAvgW = 2 * kAvgK + 1;
+
+__global__ void __launch_bounds__(64 * kAvgW)
+    c2g_colscan_avg9(const int32_t *__restrict__ cnt, const double *__restrict__ qbot,
+                     double *__restrict__ out, int n2, double qjump, Buckets bk) {
+  __shared__ double tile[2][kAvgW][65];  // [buffer][column][1 + row], row 0: carry
+  __shared__ double edge[kAvgW][3];      // rows 0, 1 and n2 - 1 of each column
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int nh = n2 / 2;
+  const int i0 = blockIdx.x * kAvgK;
+  int col = 2 * i0 - 1 + w;
+  if (col < 0) col += n2;
+  const int32_t *cc = cnt + (int64_t)col * n2;
+  double v = qbot[col];
+  constexpr int U = 8;  // tiles of 64 rows loaded ahead
+  const int tid = threadIdx.x, oc = tid >> 5, jr = tid & 31;
+  for (int base = 0; base < n2; base += 64 * U) {
+    int32_t wv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int r = base + 64 * u + l;
+      wv[u] = (r < n2 && r >= 1) ? cc[r] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = (base >> 6) + u;
+      if (t * 64 >= n2) break;
+      const int b = t & 1;
+      const int r = base + 64 * u + l;
+      if (l == 0) tile[b][w][0] = v;  // row 64 t - 1 (unused for t = 0)
+      const double cv = wv[u] ? cell_value(wv[u], (int64_t)col * n2 + r, qjump, bk) : 0.0;
+      uint64_t mask = __ballot(wv[u] != 0);
+      double mine = v;
+      while (mask) {
+        const int bb = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1;
+        v = readlane_f64(cv, bb) + v;
+        if (l >= bb) mine = v;
+      }
+      tile[b][w][1 + l] = mine;
+      if (t == 0 && l < 2) edge[w][l] = mine;
+      __syncthreads();
+      if (tid < 32 * kAvgK && (t > 0 || jr > 0)) {
+        const double(*T)[65] = tile[b];
+        const int c0 = 2 * oc, c1 = c0 + 1, c2 = c0 + 2, r0 = 2 * jr, r1 = r0 + 1, r2 = r0 + 2;
+        const double A = T[c2][r2], B = T[c2][r0], C = T[c0][r2], D = T[c0][r0];
+        const double E = T[c1][r2], F = T[c1][r0], G = T[c2][r1], H = T[c0][r1];
+        const double I = T[c1][r1];
+        out[(int64_t)(i0 + oc) * nh + 32 * t + jr] =
+            0.0625 * (A + B + C + D) + 0.125 * (E + F + G + H) + 0.25 * I;
+      }
+    }
+  }
+  if (l == 0) edge[w][2] = v;  // row n2 - 1
+  __syncthreads();
+  if (tid < kAvgK) {
+    const int o = tid, c0 = 2 * o, c1 = c0 + 1, c2 = c0 + 2;
+    const double A = edge[c2][1], B = edge[c2][2], C = edge[c0][1], D = edge[c0][2];
+    const double E = edge[c1][1], F = edge[c1][2], G = edge[c2][0], H = edge[c0][0];
+    const double I = edge[c1][0];
+    out[(int64_t)(i0 + o) * nh] = 0.0625 * (A + B + C + D) + 0.125 * (E + F + G + H) + 0.25 * I;
+  }
+}
+
 __global__ void c2g_subtract(const double *qa, double *qc, int64_t n, const double *qsum,
                              double ndiv) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -391,9 +463,19 @@ __global__ void fill_zero(double *a, int64_t n) {
 // Crossing accumulation and y sweep of the fine (or ultra-fine) grid:
 // contours.f90:206-277 (ultra=0) / congen.f90:353-423 (ultra=1).  Leaves the
 // swept field in fq (column-major, nyf rows).
+bool colscan_avg9_fits(int nxf, int nyf) {
+  return nxf == nyf && nyf % 64 == 0 && (nxf / 2) % kAvgK == 0;
+}
+
+// With avg non-null (and colscan_avg9_fits) the sweep writes the first
+// nine-point average of the fine field to avg instead of the field to fq.
 int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n, int ultra,
                int nxf, int nyf, double glxfi, double glyfi, const double *xg, int32_t *cnt,
-               int32_t *qjx, double *qbot, double *fq) {
+               int32_t *qjx, double *qbot, double *fq, double *avg) {
+  if (avg && !colscan_avg9_fits(nxf, nyf)) {
+    set_error("fine_sweep: grid shape unsuitable for the fused average");
+    return HYD_EARG;
+  }
   const Consts &k = c.c;
   const int64_t nfine = (int64_t)nxf * nyf;
   if (nfine + nxf > INT32_MAX) {
@@ -448,8 +530,12 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
   }
   hipLaunchKernelGGL(c2g_bucket_sum, dim3(256), dim3(256), 0, c.s, a.bk, c.flags, a.qjump);
   hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(1024), 0, c.s, qjx, qbot, nxf, nfine, a.qjump, a.bk);
-  hipLaunchKernelGGL(c2g_colscan, dim3((nxf + 3) / 4), dim3(256), 0, c.s, cnt, qbot, fq, nxf,
-                     nyf, a.qjump, a.bk);
+  if (avg)
+    hipLaunchKernelGGL(c2g_colscan_avg9, dim3(nxf / 2 / kAvgK), dim3(64 * kAvgW), 0, c.s, cnt,
+                       qbot, avg, nxf, a.qjump, a.bk);
+  else
+    hipLaunchKernelGGL(c2g_colscan, dim3((nxf + 3) / 4), dim3(256), 0, c.s, cnt, qbot, fq, nxf,
+                       nyf, a.qjump, a.bk);
   hipLaunchKernelGGL(c2g_bucket_clear, dim3(256), dim3(256), 0, c.s, a.bk, c.flags);
   HYD_HIP(hipGetLastError());
   return HYD_OK;
@@ -485,11 +571,14 @@ int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, i
   }
   const int64_t nfine = (int64_t)k.nxf * k.nyf;
   KTimer tm(c, "con2grid", (int64_t)n * 24 + 8 * N);
-  HYD_CHECK(fine_sweep(c, xq, yq, nextq, n, 0, k.nxf, k.nyf, k.glxfi, k.glyfi, c.xgf, c.cnt,
-                       c.qjx_cnt, c.qbot, c.fq));
-  // 4ng -> 2ng -> ng
+  // 4ng -> 2ng (fused with the sweep unless the fine field is wanted) -> ng
   const int nh1 = k.nxf / 2, nh2 = k.nx;
-  hipLaunchKernelGGL(c2g_avg9<double>, dim3((nh1 + 255) / 256, nh1), dim3(256), 0, c.s, c.fq, c.mq, nh1);
+  const bool fused = !counts_out && colscan_avg9_fits(k.nxf, k.nyf);
+  HYD_CHECK(fine_sweep(c, xq, yq, nextq, n, 0, k.nxf, k.nyf, k.glxfi, k.glyfi, c.xgf, c.cnt,
+                       c.qjx_cnt, c.qbot, c.fq, fused ? c.mq : nullptr));
+  if (!fused)
+    hipLaunchKernelGGL(c2g_avg9<double>, dim3((nh1 + 255) / 256, nh1), dim3(256), 0, c.s, c.fq,
+                       c.mq, nh1);
   hipLaunchKernelGGL(c2g_avg9<double>, dim3((nh2 + 255) / 256, nh2), dim3(256), 0, c.s, c.mq, c.wd, nh2);
   HYD_HIP(hipGetLastError());
   // sequential mean, contours.f90:330-336

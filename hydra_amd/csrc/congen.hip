@@ -33,7 +33,7 @@ namespace hyd {
 
 int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n, int ultra,
                int nxf, int nyf, double glxfi, double glyfi, const double *xg, int32_t *cnt,
-               int32_t *qjx, double *qbot, double *fq);
+               int32_t *qjx, double *qbot, double *fq, double *avg);
 int renode_batch(Ctx &c, const double *xd, const double *yd, const int64_t *off_in,
                  const int32_t *npd, int ncont, int64_t nin, double *xr, double *yr,
                  int64_t *off_out, int32_t *npr, int64_t *ntot_out, int64_t cap_out);
@@ -632,7 +632,7 @@ int recontour(Ctx &c, const double *qq) {
     HYD_CHECK(dbuf(c, "uqbot", nxu, &uqbot));
     HYD_CHECK(dbuf(c, "usweep", NU, &usweep));
     HYD_CHECK(fine_sweep(c, c.xq, c.yq, c.nextq, c.nptq, 1, nxu, nyu, k.glxui, k.glyui, xgu, ucnt,
-                         uqjx, uqbot, usweep));
+                         uqjx, uqbot, usweep, nullptr));
     // congen.f90:426-432: sequential mean over the ultra-fine grid
     void *saved_w = c.seq_work;
     const int64_t saved_cap = c.seq_cap;

@@ -24,6 +24,13 @@ namespace hyd {
 
 void set_error(const std::string &msg);
 
+// Lane l's double, broadcast to the wavefront (l wavefront-uniform).
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+  return __hiloint2double(hi, lo);
+}
+
 #define HYD_HIP(call)                                                              \
   do {                                                                             \
     hipError_t e_ = (call);                                                        \

@@ -17,16 +17,18 @@
 //   * Level 1: chunks of kChunk values are simulated in parallel from a
 //     guessed start (double-double prefix of the chunk totals).  The guessed
 //     run is cut into segments wherever its binade changes; a segment starts
-//     with that "head" addition, which the chain performs exactly.
-//   * Level 2: one wavefront walks the segments 64 at a time.  Between heads
-//     the lanes take their start parities from a GF(2) scan, their
-//     candidate starts from a scan of the chosen translations, and accept
-//     while the candidate lies on the guessed run's grid with a 4-ulp
-//     margin to the binade edges.  While all earlier lanes are accepted
-//     every scan partial is a difference of two values of one binade, hence
-//     exact, so the first rejected lane's candidate is the true sequential
-//     value; that segment is summed in order (a wavefront-level version of
-//     the same scheme).
+//     with that "head" addition, which the chain performs exactly.  Windows
+//     of kWin chunks whose runs are single segments fold into composites
+//     (translations and admissible start intervals per start parity).
+//   * Level 2: one wavefront walks the windows 64 at a time.  Lanes take
+//     their start parities from a GF(2) scan, their candidate starts from a
+//     scan of the chosen translations, and accept while the candidate lies
+//     on the guessed run's grid inside the window's admissible interval.
+//     While all earlier lanes are accepted every scan partial is a
+//     difference of two values of one binade, hence exact, so the first
+//     rejected lane's candidate is the true sequential value.  That window,
+//     and every window that did not fold, is walked record by record the
+//     same way (heads added in order, rejected segments summed in order).
 // Every accepted translation is exact by construction and every rejected
 // piece is summed in order, so the result is the sequential sum bit for bit.
 #include "hyd_internal.h"
@@ -35,11 +37,6 @@ namespace hyd {
 
 static constexpr int kChunk = 128;   // values per chunk
 static constexpr int kTile = 32;     // columns staged through LDS per pass
-
-struct ChunkInfo {
-  double total;   // sequential sum of the chunk from 0 (for the guess)
-  double g;       // guessed start
-};
 
 // A segment of a chunk's guessed run: an optional head addition, then
 // values whose guessed partial sums all share the binade of g.
@@ -138,11 +135,7 @@ __device__ __forceinline__ int wave_incl_map(int v) {
   return v;
 }
 
-// ---- level 1 --------------------------------------------------------------
-// One wavefront owns 64 consecutive chunks (a thread per chunk); values are
-// staged through LDS in kTile-wide column tiles, the next tile's loads in
-// flight while the current one is summed.
-// MODE 0: chunk totals from +0.  MODE 1: segments of the guessed run.
+// ---- segments of a guessed run ---------------------------------------------
 __device__ __forceinline__ int segkey(double x) {
   return (int)((uint64_t)__double_as_longlong(x) >> 52);  // sign and exponent
 }
@@ -187,11 +180,114 @@ __device__ __forceinline__ void seg_close(const SegState &z, int e, int64_t c, S
   cnt++;
 }
 
+__device__ __forceinline__ int parity_of(double num, double uinv) {
+  // num is an exact multiple of u = 1/uinv (a power of two), |num/u| < 2^53
+  return (int)((long long)(num * uinv) & 1);
+}
+
+// ---- composites -----------------------------------------------------------
+// A window of kWin consecutive chunks whose guessed runs each form a single
+// segment (no binade change, no replay) folds into one composite.  For a
+// start S_1 = g_1 + delta u on the grid of that binade the members are pure
+// translations: with p = parity(delta), member i starts at S_1 + off_i(p)
+// (off_i the sum of the earlier members' chosen translations, parities
+// following the GF(2) maps) and the window adds D(p) = off_(K+1)(p).  The
+// member chain would accept member i iff its start lies on the grid, inside
+// the binade with the 4-ulp margins around its run; those conditions are
+// intervals in delta, intersected per parity here with one extra ulp for
+// the rounding of the margin tests.  A start inside its interval is
+// therefore accepted by every member and gives the member chain's value;
+// any other window is summed by the member chain.
+static constexpr int kWin = 8;
+
+struct CompRec {
+  double g;                   // first member's guess
+  double d0, d1;              // window translations for start parity 0 / 1
+  double lo0, hi0, lo1, hi1;  // admissible delta (units of u, magnitude) per parity
+  int32_t win;                // window index
+  int32_t meta;               // nm | ok << 4 | (map k << 1 | map c) << 6
+};
+
+// Folds members R[0..nm) (one record per chunk); gnext is the guess of the
+// record after the window (has_next: one exists).
+__device__ CompRec compose_window(const SegRec *R, int nm, bool has_next, double gnext,
+                                  int64_t win) {
+  const SegRec &A = R[0];
+  const int key = segkey(A.g), eb = binade(A.g);
+  int ok = !((A.meta >> 18) & 3) && eb >= 64 && eb <= 2045;
+  const double ui = ok ? pow2_field(2098 - eb) : 0.0;
+  const double sig = A.g < 0.0 ? -1.0 : 1.0;
+  const double G1 = ok ? (fabs(A.g) - pow2_field(eb)) * ui : 0.0;
+  double D[2] = {0.0, 0.0}, lo[2] = {-0x1p60, -0x1p60}, hi[2] = {0x1p60, 0x1p60};
+  int pn[2] = {0, 0};
+  for (int i = 0; i < nm && ok; i++) {
+    const SegRec &Ri = R[i];
+    if (i > 0 && (((Ri.meta >> 18) & 3) != 0 || segkey(Ri.g) != key)) {
+      ok = 0;
+      break;
+    }
+    if (!run_inside(Ri.g, eb, Ri.lo, Ri.hi)) {
+      ok = 0;
+      break;
+    }
+    const double rlo = sig > 0.0 ? Ri.lo : -Ri.hi;
+    const double rhi = sig > 0.0 ? Ri.hi : -Ri.lo;
+    const double Li = ceil(5.0 - rlo * ui) + 1.0;
+    const double Ui = floor(0x1p52 - 5.0 - rhi * ui) - 1.0;
+    const bool last = i + 1 == nm;
+    const double gn = last ? gnext : R[i + 1].g;
+    const int am = (!last || has_next) ? parity_of((Ri.g + Ri.d0) - gn, ui) : 0;
+    const int bm = parity_of(Ri.d1 - Ri.d0, ui);
+#pragma unroll
+    for (int p0 = 0; p0 < 2; p0++) {
+      // member i's start magnitude offset is G1 + delta + sig * off_i / u
+      const double O = sig * D[p0] * ui;
+      lo[p0] = fmax(lo[p0], Li - G1 - O);
+      hi[p0] = fmin(hi[p0], Ui - G1 - O);
+      const int p = i == 0 ? p0 : pn[p0];
+      D[p0] = D[p0] + (p ? Ri.d1 : Ri.d0);
+      pn[p0] = bm ? am : (p ^ am);
+      // partial translations stay well inside 2^53 u, so every sum is exact
+      if (fabs(D[p0]) * ui > 0x1p51) ok = 0;
+    }
+  }
+  CompRec C;
+  C.g = A.g;
+  C.d0 = D[0];
+  C.d1 = D[1];
+  C.lo0 = lo[0];
+  C.hi0 = hi[0];
+  C.lo1 = lo[1];
+  C.hi1 = hi[1];
+  C.win = (int32_t)win;
+  const int mc = pn[0], mk = pn[0] ^ pn[1];
+  C.meta = nm | (ok << 4) | (((mk << 1) | mc) << 6);
+  return C;
+}
+
+// ---- level 1 --------------------------------------------------------------
+// One wavefront owns 64 consecutive chunks (a thread per chunk); values are
+// staged through LDS in kTile-wide column tiles, the next tile's loads in
+// flight while the current one is summed.
+// MODE 0: chunk totals from +0 (tot) and the block's double-double total
+//         (blk).
+// MODE 1: guesses (double-double prefix of the chunk totals: earlier blocks,
+//         then the block's own scan), segments of the guessed runs, and the
+//         composites of the block's kWin-chunk windows.
+__device__ __forceinline__ void dd_shfl_xor(double &h, double &l, int m) {
+  const double oh = __shfl_xor(h, m), ol = __shfl_xor(l, m);
+  dd_add(h, l, oh, ol, h, l);
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, int64_t n,
-                                                 ChunkInfo *ci, int64_t nch, SegRec *segs,
-                                                 int32_t *segcnt, int cap) {
+                                                 int64_t nch, double *tot, double2 *blk,
+                                                 SegRec *segs, int32_t *segcnt, CompRec *cps,
+                                                 int cap) {
   __shared__ double tile[64][kTile + 1];
+  __shared__ SegRec first[64];
+  __shared__ int32_t fcnt[64];
+  __shared__ double gnx;
   const int t = threadIdx.x;
   const int64_t c0 = (int64_t)blockIdx.x * 64;
   const int64_t c = c0 + t;
@@ -199,7 +295,32 @@ __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, i
   SegState z;
   int cnt = 0;
   SegRec *out = segs + c * cap;
-  if (MODE == 1 && c < nch) seg_open(z, 0, 0, 0.0, ci[c].g);
+  if (MODE == 1) {
+    // offset: earlier blocks' totals (any order: the guess needs no exactness)
+    double oh = 0.0, ol = 0.0;
+    for (int64_t k = t; k < (int64_t)blockIdx.x; k += 64) {
+      const double2 v = blk[k];
+      dd_add(oh, ol, v.x, v.y, oh, ol);
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) dd_shfl_xor(oh, ol, m);
+    // inclusive scan of this block's chunk totals
+    double ih = c < nch ? tot[c] : 0.0, il = 0.0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const double ph = __shfl_up(ih, o), pl = __shfl_up(il, o);
+      if (t >= o) dd_add(ph, pl, ih, il, ih, il);
+    }
+    double eh = __shfl_up(ih, 1), el = __shfl_up(il, 1);
+    if (t == 0) eh = el = 0.0;
+    double gh, gl;
+    dd_add(oh, ol, eh, el, gh, gl);
+    if (c < nch) seg_open(z, 0, 0, 0.0, gh + gl);
+    if (t == 63) {
+      dd_add(oh, ol, ih, il, gh, gl);
+      gnx = gh + gl;  // guess of the next block's first chunk
+    }
+  }
   constexpr int per = 64 * kTile / 64;  // values each thread stages per tile
   double nxt[per];
   auto fetch = [&](int col0) {
@@ -247,98 +368,52 @@ __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, i
       }
     }
   }
+  if (MODE == 0) {
+    if (c < nch) tot[c] = s;
+    double h = s, l = 0.0;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) dd_shfl_xor(h, l, m);
+    if (t == 0) blk[blockIdx.x] = make_double2(h, l);
+    return;
+  }
   if (c < nch) {
-    if (MODE == 0) {
-      ci[c].total = s;
-    } else {
-      const int e = (int)min((int64_t)kChunk, n - c * kChunk);
-      seg_close(z, e, c, out, cnt, cap);
-      if (cnt > cap) {
-        // too many binade changes: the whole chunk is summed in order
-        SegRec r;
-        r.g = 0.0;
-        r.d0 = r.d1 = r.lo = r.hi = r.xh = 0.0;
-        r.chunk = (int32_t)c;
-        r.meta = 0 | (e << 9) | (1 << 19);
-        out[0] = r;
-        cnt = 1;
+    const int e = (int)min((int64_t)kChunk, n - c * kChunk);
+    seg_close(z, e, c, out, cnt, cap);
+    if (cnt > cap) {
+      // too many binade changes: the whole chunk is summed in order
+      SegRec r;
+      r.g = 0.0;
+      r.d0 = r.d1 = r.lo = r.hi = r.xh = 0.0;
+      r.chunk = (int32_t)c;
+      r.meta = 0 | (e << 9) | (1 << 19);
+      out[0] = r;
+      cnt = 1;
+    }
+    segcnt[c] = cnt;
+    first[t] = out[0];
+    fcnt[t] = cnt;
+  }
+  __syncthreads();
+  // composites: thread w folds window w of the block
+  if (t < 64 / kWin) {
+    const int64_t w0 = c0 + (int64_t)t * kWin;
+    if (w0 < nch) {
+      const int nm = (int)min((int64_t)kWin, nch - w0);
+      bool single = true;
+      for (int i = 0; i < nm; i++) single = single && fcnt[t * kWin + i] == 1;
+      const bool has_next = w0 + nm < nch;
+      const double gnext = (t + 1) * kWin < 64 ? first[(t + 1) * kWin].g : gnx;
+      CompRec C;
+      if (single) {
+        C = compose_window(&first[t * kWin], nm, has_next, gnext, w0 / kWin);
+      } else {
+        C.g = C.d0 = C.d1 = C.lo0 = C.hi0 = C.lo1 = C.hi1 = 0.0;
+        C.win = (int32_t)(w0 / kWin);
+        C.meta = nm;
       }
-      segcnt[c] = cnt;
+      cps[w0 / kWin] = C;
     }
   }
-}
-
-// Guesses: double-double exclusive prefix of the chunk totals (one
-// workgroup: per-thread runs, then a scan of the run totals).
-__global__ void __launch_bounds__(1024) seq_guess(ChunkInfo *ci, int64_t nch) {
-  __shared__ double ph[1024], pl[1024];
-  const int t = threadIdx.x;
-  const int64_t per = (nch + 1023) / 1024;
-  const int64_t b = min(nch, per * t), e = min(nch, b + per);
-  double hi = 0.0, lo = 0.0;
-  for (int64_t c = b; c < e; c++) dd_add(hi, lo, ci[c].total, 0.0, hi, lo);
-  ph[t] = hi;
-  pl[t] = lo;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    double oh = 0.0, ol = 0.0;
-    if (t >= off) {
-      oh = ph[t - off];
-      ol = pl[t - off];
-    }
-    __syncthreads();
-    if (t >= off) {
-      double sh, sl;
-      dd_add(oh, ol, ph[t], pl[t], sh, sl);
-      ph[t] = sh;
-      pl[t] = sl;
-    }
-    __syncthreads();
-  }
-  // exclusive start of this thread's run
-  double gh = 0.0, gl = 0.0;
-  if (t > 0) {
-    gh = ph[t - 1];
-    gl = pl[t - 1];
-  }
-  for (int64_t c = b; c < e; c++) {
-    ci[c].g = gh + gl;
-    dd_add(gh, gl, ci[c].total, 0.0, gh, gl);
-  }
-}
-
-// Exclusive scan of the segment counts (one workgroup).
-__global__ void __launch_bounds__(1024) seq_offsets(const int32_t *__restrict__ segcnt,
-                                                    int64_t nch, int64_t *off, int64_t *total) {
-  __shared__ int64_t part[1024];
-  const int t = threadIdx.x;
-  const int64_t per = (nch + 1023) / 1024;
-  const int64_t b = min(nch, per * t), e = min(nch, b + per);
-  int64_t s = 0;
-  for (int64_t c = b; c < e; c++) s += segcnt[c];
-  part[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int64_t v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int64_t acc = part[t] - s;
-  for (int64_t c = b; c < e; c++) {
-    off[c] = acc;
-    acc += segcnt[c];
-  }
-  if (t == 1023) *total = part[1023];
-}
-
-// Compaction: each thread moves one chunk's records.
-__global__ void seq_compact(const SegRec *__restrict__ segs, const int32_t *__restrict__ segcnt,
-                            const int64_t *__restrict__ off, int64_t nch, int cap, SegRec *out) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nch) return;
-  const int k = segcnt[c];
-  for (int j = 0; j < k; j++) out[off[c] + j] = segs[c * cap + j];
 }
 
 // Exact sequential sum of a[b, e) (e - b <= 2 * 64) starting from S: the
@@ -355,16 +430,7 @@ __device__ double lane_seq(const double *__restrict__ a, int64_t b, int64_t e, d
   return S;
 }
 
-__device__ __forceinline__ int parity_of(double num, double uinv) {
-  // num is an exact multiple of u = 1/uinv (a power of two), |num/u| < 2^53
-  return (int)((long long)(num * uinv) & 1);
-}
-
-// Level 2: the chain over the compacted segments.  Wave 0 runs the chain;
-// waves 1-3 stream the next block of records into LDS meanwhile (double
-// buffered, one barrier per block).
-static constexpr int kBlockRecs = 512;  // records per LDS buffer (8 groups)
-
+// The member chain over up to 64 compacted records (one per lane).
 __device__ void chain_group(const double *__restrict__ a, const SegRec &R, int nv, double &S) {
   const int l = lane_id();
   const bool valid = l < nv;
@@ -433,31 +499,131 @@ __device__ void chain_group(const double *__restrict__ a, const SegRec &R, int n
   }
 }
 
-__global__ void __launch_bounds__(256) seq_chain(const double *__restrict__ a, int64_t n,
+// Records of window win (chunks kWin win .. + nm), one per lane in order,
+// summed by the member chain from S.
+__device__ void member_level(const double *__restrict__ a, const SegRec *__restrict__ sg,
+                             const int32_t *__restrict__ segcnt, int cap, int64_t win, int nm,
+                             double &S) {
+  const int l = lane_id();
+  const int64_t c0 = win * kWin;
+  const int cl = l < nm ? segcnt[c0 + l] : 0;
+  // exclusive prefix of the chunk counts (nm <= kWin lanes)
+  int off = 0, tot = 0;
+  for (int i = 0; i < nm; i++) {
+    const int ci = __builtin_amdgcn_readlane(cl, i);
+    if (i < l) off += ci;
+    tot += ci;
+  }
+  // lane r takes record r: chunk i with off_i <= r < off_i + cnt_i
+  int ch = 0, k = l;
+  for (int i = 0; i < nm; i++) {
+    const int oi = __builtin_amdgcn_readlane(off, i);
+    if (l >= oi) {
+      ch = i;
+      k = l - oi;
+    }
+  }
+  SegRec R;
+  if (l < tot) {
+    R = sg[(c0 + ch) * cap + k];
+  } else {
+    R.meta = 0, R.g = 0.0, R.d0 = R.d1 = R.lo = R.hi = R.xh = 0.0, R.chunk = 0;
+  }
+  chain_group(a, R, tot, S);
+}
+
+// The chain over 64 windows: as chain_group, with each composite's parity
+// map, translations and admissible intervals; windows that did not fold,
+// and the first rejected one, go to the member chain.
+__device__ void chain_comp(const double *__restrict__ a, const SegRec *__restrict__ sg,
+                           const int32_t *__restrict__ segcnt, int cap, const CompRec &C,
+                           int nv, double &S) {
+  const int l = lane_id();
+  const bool valid = l < nv;
+  const int ok = valid ? (C.meta >> 4) & 1 : 0;
+  const int key = segkey(C.g);
+  int q = 0;
+  while (q < nv) {
+    const int meta_q = __builtin_amdgcn_readlane(C.meta, q);
+    const int win_q = __builtin_amdgcn_readlane(C.win, q);
+    if (!((meta_q >> 4) & 1)) {
+      member_level(a, sg, segcnt, cap, win_q, meta_q & 15, S);
+      q++;
+      continue;
+    }
+    const int key_q = __builtin_amdgcn_readlane(key, q);
+    const uint64_t stop = __ballot(valid && l > q && (!ok || key != key_q));
+    const int h = stop ? __ffsll((unsigned long long)stop) - 1 : nv;
+    const bool in = l >= q && l < h;
+    const double g_q = readlane_d(C.g, q);
+    const int eq = binade(g_q);
+    const double ui = (eq >= 2 && eq <= 2045) ? pow2_field(2098 - eq) : 0.0;
+    const int mp = (in && l + 1 < h) ? (C.meta >> 6) & 3 : 2;
+    const int incl_m = wave_incl_map(mp);
+    int ex_m = dpp_i<0x138>(incl_m);
+    if (l == 0) ex_m = 2;
+    const int p_q = segkey(S) == key_q ? parity_of(S - g_q, ui) : 0;
+    const int p = (ex_m >> 1) ? (p_q ^ (ex_m & 1)) : (ex_m & 1);
+    const double d = in ? (p ? C.d1 : C.d0) : 0.0;
+    const double excl = wave_prev_d(wave_incl_add(d));
+    const double Sl = S + excl;
+    bool acc = false;
+    if (in && segkey(Sl) == key) {
+      const double off = Sl - C.g;  // exact: one binade
+      const double delta = (C.g < 0.0 ? -off : off) * ui;
+      acc = parity_of(off, ui) == p && delta >= (p ? C.lo1 : C.lo0) &&
+            delta <= (p ? C.hi1 : C.hi0);
+    }
+    const uint64_t rej = __ballot(in && !acc);
+    if (rej == 0) {
+      S = readlane_d(Sl + d, h - 1);
+      q = h;
+      continue;
+    }
+    const int f = __ffsll((unsigned long long)rej) - 1;
+    S = readlane_d(Sl, f);
+    const int meta_f = __builtin_amdgcn_readlane(C.meta, f);
+    const int win_f = __builtin_amdgcn_readlane(C.win, f);
+    member_level(a, sg, segcnt, cap, win_f, meta_f & 15, S);
+    q = f + 1;
+  }
+}
+
+// Level 2: the chain over the windows.  Wave 0 runs the chain; waves 1-3
+// stream the next block of composites into LDS meanwhile (double buffered,
+// one barrier per block).
+static constexpr int kBlockComps = 256;
+
+__global__ void __launch_bounds__(256) seq_chain(const double *__restrict__ a,
                                                  const SegRec *__restrict__ sg,
-                                                 const int64_t *total, double *out) {
-  __shared__ SegRec buf[2][kBlockRecs];
+                                                 const int32_t *__restrict__ segcnt, int cap,
+                                                 const CompRec *__restrict__ cp, int64_t M,
+                                                 double *out) {
+  __shared__ CompRec buf[2][kBlockComps];
   const int t = threadIdx.x, w = t >> 6;
-  const int64_t M = *total;
-  const int64_t nblk = (M + kBlockRecs - 1) / kBlockRecs;
+  const int64_t nblk = (M + kBlockComps - 1) / kBlockComps;
   double S = 0.0;  // the reference's accumulator starts at +0
-  // block 0 by every wave
-  for (int j = t; j < kBlockRecs; j += 256)
-    if (j < M) buf[0][j] = sg[j];
+  for (int j = t; j < kBlockComps; j += 256)
+    if (j < M) buf[0][j] = cp[j];
   __syncthreads();
   for (int64_t k = 0; k < nblk; k++) {
     if (w != 0) {
-      const int64_t b1 = (k + 1) * kBlockRecs;
-      for (int j = t - 64; j < kBlockRecs; j += 192)
-        if (b1 + j < M) buf[(k + 1) & 1][j] = sg[b1 + j];
+      const int64_t b1 = (k + 1) * kBlockComps;
+      for (int j = t - 64; j < kBlockComps; j += 192)
+        if (b1 + j < M) buf[(k + 1) & 1][j] = cp[b1 + j];
     } else {
-      const int64_t b0 = k * kBlockRecs;
-      for (int g0 = 0; g0 < kBlockRecs && b0 + g0 < M; g0 += 64) {
+      const int64_t b0 = k * kBlockComps;
+      for (int g0 = 0; g0 < kBlockComps && b0 + g0 < M; g0 += 64) {
         const int nv = (int)min((int64_t)64, M - b0 - g0);
-        SegRec R;
-        if (t < nv) R = buf[k & 1][g0 + t];
-        else R.meta = 0, R.g = 0.0, R.d0 = R.d1 = R.lo = R.hi = R.xh = 0.0, R.chunk = 0;
-        chain_group(a, R, nv, S);
+        CompRec C;
+        if (t < nv) {
+          C = buf[k & 1][g0 + t];
+        } else {
+          C.g = C.d0 = C.d1 = C.lo0 = C.hi0 = C.lo1 = C.hi1 = 0.0;
+          C.win = 0;
+          C.meta = 0;
+        }
+        chain_comp(a, sg, segcnt, cap, C, nv, S);
       }
     }
     __syncthreads();
@@ -469,25 +635,28 @@ __global__ void seq_zero(double *out) {
   if (threadIdx.x == 0) *out = 0.0;
 }
 
-static int seg_cap(int64_t nch) { return nch <= 65536 ? 8 : 2; }
+// records per chunk (a window of kWin chunks must fit one wavefront); very
+// long arrays keep 2 to bound the work space
+static int seg_cap(int64_t nch) { return nch <= 65536 ? 64 / kWin : 2; }
 
 struct SeqLayout {
-  int64_t nch, ci, segs, cnt, off, comp, tot, bytes;
+  int64_t nch, nb, nw, tot, blk, segs, cnt, cps, bytes;
   int cap;
 };
 
 static SeqLayout seq_layout(int64_t n) {
   SeqLayout L;
   L.nch = (n + kChunk - 1) / kChunk;
+  L.nb = (L.nch + 63) / 64;
+  L.nw = (L.nch + kWin - 1) / kWin;
   L.cap = seg_cap(L.nch);
   auto al = [](int64_t x) { return (x + 255) / 256 * 256; };
-  L.ci = 0;
-  L.segs = al(L.ci + (int64_t)sizeof(ChunkInfo) * L.nch);
+  L.tot = 0;
+  L.blk = al(L.tot + 8 * L.nch);
+  L.segs = al(L.blk + 16 * L.nb);
   L.cnt = al(L.segs + (int64_t)sizeof(SegRec) * L.nch * L.cap);
-  L.off = al(L.cnt + 4 * L.nch);
-  L.comp = al(L.off + 8 * L.nch);
-  L.tot = al(L.comp + (int64_t)sizeof(SegRec) * L.nch * L.cap);
-  L.bytes = al(L.tot + 8);
+  L.cps = al(L.cnt + 4 * L.nch);
+  L.bytes = al(L.cps + (int64_t)sizeof(CompRec) * L.nw);
   return L;
 }
 
@@ -498,28 +667,22 @@ int seqsum(Ctx &c, const double *a, int64_t n, double *dout) {
     return HYD_ECAPACITY;
   }
   char *w = (char *)c.seq_work;
-  ChunkInfo *ci = (ChunkInfo *)(w + L.ci);
+  double *tot = (double *)(w + L.tot);
+  double2 *blk = (double2 *)(w + L.blk);
   SegRec *segs = (SegRec *)(w + L.segs);
   int32_t *cnt = (int32_t *)(w + L.cnt);
-  int64_t *off = (int64_t *)(w + L.off);
-  SegRec *comp = (SegRec *)(w + L.comp);
-  int64_t *tot = (int64_t *)(w + L.tot);
+  CompRec *cps = (CompRec *)(w + L.cps);
   KTimer t(c, "seqsum", 8 * n);
   if (n == 0) {
     hipLaunchKernelGGL(seq_zero, dim3(1), dim3(64), 0, c.s, dout);
     HYD_HIP(hipGetLastError());
     return HYD_OK;
   }
-  const int nb = (int)((L.nch + 63) / 64);
-  hipLaunchKernelGGL(seq_chunks<0>, dim3(nb), dim3(64), 0, c.s, a, n, ci, L.nch, segs, cnt,
-                     L.cap);
-  hipLaunchKernelGGL(seq_guess, dim3(1), dim3(1024), 0, c.s, ci, L.nch);
-  hipLaunchKernelGGL(seq_chunks<1>, dim3(nb), dim3(64), 0, c.s, a, n, ci, L.nch, segs, cnt,
-                     L.cap);
-  hipLaunchKernelGGL(seq_offsets, dim3(1), dim3(1024), 0, c.s, cnt, L.nch, off, tot);
-  hipLaunchKernelGGL(seq_compact, dim3((unsigned)((L.nch + 255) / 256)), dim3(256), 0, c.s, segs,
-                     cnt, off, L.nch, L.cap, comp);
-  hipLaunchKernelGGL(seq_chain, dim3(1), dim3(256), 0, c.s, a, n, comp, tot, dout);
+  hipLaunchKernelGGL(seq_chunks<0>, dim3((unsigned)L.nb), dim3(64), 0, c.s, a, n, L.nch, tot, blk,
+                     segs, cnt, cps, L.cap);
+  hipLaunchKernelGGL(seq_chunks<1>, dim3((unsigned)L.nb), dim3(64), 0, c.s, a, n, L.nch, tot, blk,
+                     segs, cnt, cps, L.cap);
+  hipLaunchKernelGGL(seq_chain, dim3(1), dim3(256), 0, c.s, a, segs, cnt, L.cap, cps, L.nw, dout);
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }
