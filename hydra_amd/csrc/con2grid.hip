@@ -388,52 +388,59 @@ static constexpr int kAvgW = 2 * kAvgK + 1;
 __global__ void __launch_bounds__(64 * kAvgW)
     c2g_colscan_avg9(const int32_t *__restrict__ cnt, const double *__restrict__ qbot,
                      double *__restrict__ out, int n2, double qjump, Buckets bk) {
-  __shared__ double tile[2][kAvgW][65];  // [buffer][column][1 + row], row 0: carry
-  __shared__ double edge[kAvgW][3];      // rows 0, 1 and n2 - 1 of each column
+  constexpr int TPS = 4;  // tiles of 64 rows per stage (one barrier each)
+  __shared__ double tile[2][kAvgW][64 * TPS + 1];  // [buffer][column][1 + row], 0: carry
+  __shared__ double edge[kAvgW][3];                // rows 0, 1 and n2 - 1 of each column
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int nh = n2 / 2;
+  const int nh = n2 / 2, ntiles = n2 / 64;
   const int i0 = blockIdx.x * kAvgK;
   int col = 2 * i0 - 1 + w;
   if (col < 0) col += n2;
   const int32_t *cc = cnt + (int64_t)col * n2;
   double v = qbot[col];
-  constexpr int U = 8;  // tiles of 64 rows loaded ahead
-  const int tid = threadIdx.x, oc = tid >> 5, jr = tid & 31;
-  for (int base = 0; base < n2; base += 64 * U) {
-    int32_t wv[U];
+  const int tid = threadIdx.x, oc = tid >> 7, jr = tid & 127;
+  // counts of the next stage load while the current one is swept
+  int32_t wv[TPS], wn[TPS];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int r = base + 64 * u + l;
-      wv[u] = (r < n2 && r >= 1) ? cc[r] : 0;
+  for (int u = 0; u < TPS; u++) {
+    const int r = 64 * u + l;
+    wn[u] = (u < ntiles && r >= 1) ? cc[r] : 0;
+  }
+  for (int st = 0; st * TPS < ntiles; st++) {
+    const int t0 = st * TPS, b = st & 1;
+#pragma unroll
+    for (int u = 0; u < TPS; u++) {
+      wv[u] = wn[u];
+      wn[u] = t0 + TPS + u < ntiles ? cc[64 * (t0 + TPS + u) + l] : 0;
     }
+    if (l == 0) tile[b][w][0] = v;  // row 64 t0 - 1 (unused for st = 0)
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int t = (base >> 6) + u;
-      if (t * 64 >= n2) break;
-      const int b = t & 1;
-      const int r = base + 64 * u + l;
-      if (l == 0) tile[b][w][0] = v;  // row 64 t - 1 (unused for t = 0)
-      const double cv = wv[u] ? cell_value(wv[u], (int64_t)col * n2 + r, qjump, bk) : 0.0;
-      uint64_t mask = __ballot(wv[u] != 0);
-      double mine = v;
-      while (mask) {
-        const int bb = __ffsll((unsigned long long)mask) - 1;
-        mask &= mask - 1;
-        v = readlane_f64(cv, bb) + v;
-        if (l >= bb) mine = v;
+    for (int u = 0; u < TPS; u++) {
+      if (t0 + u < ntiles) {
+        const int r = 64 * (t0 + u) + l;
+        const double cv = wv[u] ? cell_value(wv[u], (int64_t)col * n2 + r, qjump, bk) : 0.0;
+        uint64_t mask = __ballot(wv[u] != 0);
+        double mine = v;
+        while (mask) {
+          const int bb = __ffsll((unsigned long long)mask) - 1;
+          mask &= mask - 1;
+          v = readlane_f64(cv, bb) + v;
+          if (l >= bb) mine = v;
+        }
+        tile[b][w][1 + 64 * u + l] = mine;
+        if (st == 0 && u == 0 && l < 2) edge[w][l] = mine;
       }
-      tile[b][w][1 + l] = mine;
-      if (t == 0 && l < 2) edge[w][l] = mine;
-      __syncthreads();
-      if (tid < 32 * kAvgK && (t > 0 || jr > 0)) {
-        const double(*T)[65] = tile[b];
-        const int c0 = 2 * oc, c1 = c0 + 1, c2 = c0 + 2, r0 = 2 * jr, r1 = r0 + 1, r2 = r0 + 2;
-        const double A = T[c2][r2], B = T[c2][r0], C = T[c0][r2], D = T[c0][r0];
-        const double E = T[c1][r2], F = T[c1][r0], G = T[c2][r1], H = T[c0][r1];
-        const double I = T[c1][r1];
-        out[(int64_t)(i0 + oc) * nh + 32 * t + jr] =
-            0.0625 * (A + B + C + D) + 0.125 * (E + F + G + H) + 0.25 * I;
-      }
+    }
+    __syncthreads();
+    const int nrows = 32 * min(TPS, ntiles - t0);
+    if (tid < 128 * kAvgK && jr < nrows && (st > 0 || jr > 0)) {
+      const double(*T)[64 * TPS + 1] = tile[b];
+      const int c0 = 2 * oc, c1 = c0 + 1, c2 = c0 + 2, r0 = 2 * jr, r1 = r0 + 1, r2 = r0 + 2;
+      const double A = T[c2][r2], B = T[c2][r0], C = T[c0][r2], D = T[c0][r0];
+      const double E = T[c1][r2], F = T[c1][r0], G = T[c2][r1], H = T[c0][r1];
+      const double I = T[c1][r1];
+      out[(int64_t)(i0 + oc) * nh + 32 * t0 + jr] =
+          0.0625 * (A + B + C + D) + 0.125 * (E + F + G + H) + 0.25 * I;
     }
   }
   if (l == 0) edge[w][2] = v;  // row n2 - 1

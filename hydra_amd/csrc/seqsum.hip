@@ -206,6 +206,8 @@ struct CompRec {
   double lo0, hi0, lo1, hi1;  // admissible delta (units of u, magnitude) per parity
   int32_t win;                // window index
   int32_t meta;               // nm | ok << 4 | (map k << 1 | map c) << 6
+  uint32_t cnts;              // record count of member chunk i in bits 4i..4i+3
+  int32_t pad;
 };
 
 // Folds members R[0..nm) (one record per chunk); gnext is the guess of the
@@ -403,6 +405,8 @@ __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, i
       for (int i = 0; i < nm; i++) single = single && fcnt[t * kWin + i] == 1;
       const bool has_next = w0 + nm < nch;
       const double gnext = (t + 1) * kWin < 64 ? first[(t + 1) * kWin].g : gnx;
+      uint32_t cn = 0;
+      for (int i = 0; i < nm; i++) cn |= (uint32_t)fcnt[t * kWin + i] << (4 * i);
       CompRec C;
       if (single) {
         C = compose_window(&first[t * kWin], nm, has_next, gnext, w0 / kWin);
@@ -411,6 +415,8 @@ __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, i
         C.win = (int32_t)(w0 / kWin);
         C.meta = nm;
       }
+      C.cnts = cn;
+      C.pad = 0;
       cps[w0 / kWin] = C;
     }
   }
@@ -499,29 +505,20 @@ __device__ void chain_group(const double *__restrict__ a, const SegRec &R, int n
   }
 }
 
-// Records of window win (chunks kWin win .. + nm), one per lane in order,
-// summed by the member chain from S.
-__device__ void member_level(const double *__restrict__ a, const SegRec *__restrict__ sg,
-                             const int32_t *__restrict__ segcnt, int cap, int64_t win, int nm,
-                             double &S) {
+// Records of window win (chunks kWin win .. + nm, counts packed in cnts),
+// one per lane in order, summed by the member chain from S.
+__device__ void member_level(const double *__restrict__ a, const SegRec *__restrict__ sg, int cap,
+                             int64_t win, int nm, uint32_t cnts, double &S) {
   const int l = lane_id();
   const int64_t c0 = win * kWin;
-  const int cl = l < nm ? segcnt[c0 + l] : 0;
-  // exclusive prefix of the chunk counts (nm <= kWin lanes)
-  int off = 0, tot = 0;
-  for (int i = 0; i < nm; i++) {
-    const int ci = __builtin_amdgcn_readlane(cl, i);
-    if (i < l) off += ci;
-    tot += ci;
-  }
   // lane r takes record r: chunk i with off_i <= r < off_i + cnt_i
-  int ch = 0, k = l;
+  int ch = 0, k = l, tot = 0;
   for (int i = 0; i < nm; i++) {
-    const int oi = __builtin_amdgcn_readlane(off, i);
-    if (l >= oi) {
+    if (l >= tot) {
       ch = i;
-      k = l - oi;
+      k = l - tot;
     }
+    tot += (cnts >> (4 * i)) & 15;
   }
   SegRec R;
   if (l < tot) {
@@ -535,9 +532,8 @@ __device__ void member_level(const double *__restrict__ a, const SegRec *__restr
 // The chain over 64 windows: as chain_group, with each composite's parity
 // map, translations and admissible intervals; windows that did not fold,
 // and the first rejected one, go to the member chain.
-__device__ void chain_comp(const double *__restrict__ a, const SegRec *__restrict__ sg,
-                           const int32_t *__restrict__ segcnt, int cap, const CompRec &C,
-                           int nv, double &S) {
+__device__ void chain_comp(const double *__restrict__ a, const SegRec *__restrict__ sg, int cap,
+                           const CompRec &C, int nv, double &S) {
   const int l = lane_id();
   const bool valid = l < nv;
   const int ok = valid ? (C.meta >> 4) & 1 : 0;
@@ -547,7 +543,7 @@ __device__ void chain_comp(const double *__restrict__ a, const SegRec *__restric
     const int meta_q = __builtin_amdgcn_readlane(C.meta, q);
     const int win_q = __builtin_amdgcn_readlane(C.win, q);
     if (!((meta_q >> 4) & 1)) {
-      member_level(a, sg, segcnt, cap, win_q, meta_q & 15, S);
+      member_level(a, sg, cap, win_q, meta_q & 15, __builtin_amdgcn_readlane(C.cnts, q), S);
       q++;
       continue;
     }
@@ -584,7 +580,7 @@ __device__ void chain_comp(const double *__restrict__ a, const SegRec *__restric
     S = readlane_d(Sl, f);
     const int meta_f = __builtin_amdgcn_readlane(C.meta, f);
     const int win_f = __builtin_amdgcn_readlane(C.win, f);
-    member_level(a, sg, segcnt, cap, win_f, meta_f & 15, S);
+    member_level(a, sg, cap, win_f, meta_f & 15, __builtin_amdgcn_readlane(C.cnts, f), S);
     q = f + 1;
   }
 }
@@ -595,8 +591,7 @@ __device__ void chain_comp(const double *__restrict__ a, const SegRec *__restric
 static constexpr int kBlockComps = 256;
 
 __global__ void __launch_bounds__(256) seq_chain(const double *__restrict__ a,
-                                                 const SegRec *__restrict__ sg,
-                                                 const int32_t *__restrict__ segcnt, int cap,
+                                                 const SegRec *__restrict__ sg, int cap,
                                                  const CompRec *__restrict__ cp, int64_t M,
                                                  double *out) {
   __shared__ CompRec buf[2][kBlockComps];
@@ -622,8 +617,9 @@ __global__ void __launch_bounds__(256) seq_chain(const double *__restrict__ a,
           C.g = C.d0 = C.d1 = C.lo0 = C.hi0 = C.lo1 = C.hi1 = 0.0;
           C.win = 0;
           C.meta = 0;
+          C.cnts = 0;
         }
-        chain_comp(a, sg, segcnt, cap, C, nv, S);
+        chain_comp(a, sg, cap, C, nv, S);
       }
     }
     __syncthreads();
@@ -682,7 +678,7 @@ int seqsum(Ctx &c, const double *a, int64_t n, double *dout) {
                      segs, cnt, cps, L.cap);
   hipLaunchKernelGGL(seq_chunks<1>, dim3((unsigned)L.nb), dim3(64), 0, c.s, a, n, L.nch, tot, blk,
                      segs, cnt, cps, L.cap);
-  hipLaunchKernelGGL(seq_chain, dim3(1), dim3(256), 0, c.s, a, segs, cnt, L.cap, cps, L.nw, dout);
+  hipLaunchKernelGGL(seq_chain, dim3(1), dim3(256), 0, c.s, a, segs, L.cap, cps, L.nw, dout);
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }
