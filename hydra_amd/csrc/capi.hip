@@ -94,7 +94,7 @@ static void free_ctx(Ctx *c) {
                   c->rp_tid, c->rp_list, c->xq, c->yq, c->xqi, c->yqi, c->xqf, c->yqf, c->uq,
                   c->vq, c->nextq, c->nxt_tmp, c->dsc, c->dsum, c->seq_work, c->uu32,
                   c->vv32, c->fa32, c->fb32, c->fc32, c->tmp32, c->fq32, c->mq32, c->xtrig32,
-                  c->ytrig32, c->qb32, c->qdiss_u_dev, c->part};
+                  c->ytrig32, c->qb32, c->qdiss_u_dev, c->part, c->c2g_q, c->c2g_w};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int k = 0; k < 4; k++) {
@@ -106,12 +106,15 @@ static void free_ctx(Ctx *c) {
   if (c->hsc) hipHostFree(c->hsc);
   if (c->exp_host) hipHostFree(c->exp_host);
   if (c->ev_sc) hipEventDestroy(c->ev_sc);
+  for (hipEvent_t e : {c->ev_fork, c->ev_join})
+    if (e) hipEventDestroy(e);
   free_congen(*c);
   for (auto &r : c->krec) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
   }
   if (c->step_graph) hipGraphExecDestroy(c->step_graph);
+  if (c->s2) hipStreamDestroy(c->s2);
   if (c->s) hipStreamDestroy(c->s);
   delete c;
 }
@@ -180,7 +183,7 @@ static int create(const hyd_params *p, hyd_ctx **out) {
   const int64_t N = c->N;
   double **fields[] = {&c->qs,  &c->qd,  &c->qc,  &c->qq,  &c->pp,  &c->uu, &c->vv,
                        &c->qr,  &c->emq, &c->epq, &c->qsi, &c->qsf, &c->sqs, &c->qdi,
-                       &c->qdf, &c->sqd, &c->wa,  &c->wb,  &c->wc,  &c->wd};
+                       &c->qdf, &c->sqd, &c->wa,  &c->wb,  &c->wc,  &c->wd, &c->c2g_q, &c->c2g_w};
   for (double **f : fields) TRY(dalloc(f, N));
   for (int k = 0; k < 4; k++) TRY(dalloc(&c->t4[k], N));
   TRY(dalloc(&c->green, N));
@@ -245,8 +248,11 @@ static int create(const hyd_params *p, hyd_ctx **out) {
   memset(c->hsc, 0, sizeof(Scalars));
   if (hipHostMalloc((void **)&c->exp_host, sizeof(double) * std::max<size_t>(c->ht.qdiss_u.size(), 1),
                     hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_sc, hipEventDisableTiming) != hipSuccess) {
-    set_error("hipHostMalloc / hipEventCreate failed");
+      hipEventCreateWithFlags(&c->ev_sc, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipHostMalloc / hipEventCreate / hipStreamCreate failed");
     free_ctx(c);
     return HYD_ENOMEM;
   }

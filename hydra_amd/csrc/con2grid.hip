@@ -586,11 +586,12 @@ int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, i
   if (!fused)
     hipLaunchKernelGGL(c2g_avg9<double>, dim3((nh1 + 255) / 256, nh1), dim3(256), 0, c.s, c.fq,
                        c.mq, nh1);
-  hipLaunchKernelGGL(c2g_avg9<double>, dim3((nh2 + 255) / 256, nh2), dim3(256), 0, c.s, c.mq, c.wd, nh2);
+  hipLaunchKernelGGL(c2g_avg9<double>, dim3((nh2 + 255) / 256, nh2), dim3(256), 0, c.s, c.mq,
+                     c.c2g_w, nh2);
   HYD_HIP(hipGetLastError());
   // sequential mean, contours.f90:330-336
-  HYD_CHECK(seqsum(c, c.wd, N, c.dsum));
-  hipLaunchKernelGGL(c2g_subtract, dim3((N + 255) / 256), dim3(256), 0, c.s, c.wd, qc, N, c.dsum,
+  HYD_CHECK(seqsum(c, c.c2g_w, N, c.dsum));
+  hipLaunchKernelGGL(c2g_subtract, dim3((N + 255) / 256), dim3(256), 0, c.s, c.c2g_w, qc, N, c.dsum,
                      (double)N);
   HYD_HIP(hipGetLastError());
   if (counts_out) {

@@ -114,6 +114,9 @@ struct Ctx {
   double *uu = nullptr, *vv = nullptr, *qr = nullptr, *emq = nullptr, *epq = nullptr;
   double *qsi = nullptr, *qsf = nullptr, *sqs = nullptr, *qdi = nullptr, *qdf = nullptr;
   double *sqd = nullptr, *wa = nullptr, *wb = nullptr, *wc = nullptr, *wd = nullptr;
+  // con2grid's own grids (the plain step builds the next stage's contour
+  // grid while the source of the current one runs on s2)
+  double *c2g_q = nullptr, *c2g_w = nullptr;
   // con2grid work
   int32_t *cnt = nullptr;      // (nyf+1) x nxf packed counts
   double *fq = nullptr;        // nyf x nxf swept fine field
@@ -154,6 +157,9 @@ struct Ctx {
   double *dsum = nullptr;   // device scratch for sequential sums (8 slots)
   double *exp_host = nullptr;    // pinned: host-evaluated exp table of the step
   hipEvent_t ev_sc = nullptr;    // adapt's scalars have reached the host mirror
+  // second stream of the plain step (default priority) and its fork/join events
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void *seq_work = nullptr; // chunk table of seqsum
   int64_t seq_cap = 0;      // chunks it holds
   // work buffers of recontour / renode, allocated on first use

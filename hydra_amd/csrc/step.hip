@@ -203,10 +203,16 @@ __global__ void k_rk_fields(double *__restrict__ qs, double *__restrict__ qd,
 
 static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-int inversion(Ctx &c) {
+// The contour part of inversion: qc = ptospc(con2grid) (evolution.f90:507-508).
+static int contour_grid(Ctx &c) {
+  HYD_CHECK(con2grid(c, c.xq, c.yq, c.nextq, c.nptq, c.c2g_q, nullptr));
+  HYD_CHECK(fft_ptospc(c, c.c2g_q, c.qc));
+  return HYD_OK;
+}
+
+// The field part of inversion: qq, pp and the velocity (evolution.f90:510-520).
+static int inversion_fields(Ctx &c) {
   const int64_t N = c.N;
-  HYD_CHECK(con2grid(c, c.xq, c.yq, c.nextq, c.nptq, c.wa, nullptr));
-  HYD_CHECK(fft_ptospc(c, c.wa, c.qc));
   {
     KTimer t(c, "combine_qq", 40 * N);
     hipLaunchKernelGGL(k_combine_qq, grid1(N), dim3(256), 0, c.s, c.qq, c.flo, c.qs, c.qc, c.qd, N);
@@ -222,6 +228,11 @@ int inversion(Ctx &c) {
     HYD_CHECK(fft_spctop_n(c, 2, in, out, pre, sg, c.t4));
   }
   return HYD_OK;
+}
+
+int inversion(Ctx &c) {
+  HYD_CHECK(contour_grid(c));
+  return inversion_fields(c);
 }
 
 // main_invert (spectral.f90:304-342) on device arrays; tmp is a work array.
@@ -362,33 +373,63 @@ __global__ void k_tick(Scalars *sc) {
   if (threadIdx.x == 0) sc->t = sc->t + sc->dt2;
 }
 
+// One RK4 step (evolution.f90:166-326).  Stage k's source and field update
+// (which need the velocity of stage k) run on the second stream while the
+// main stream moves the nodes and builds the contour grid of stage k+1 (which
+// needs only the moved nodes); the field part of stage k+1's inversion waits
+// for the update.  Every kernel sees the same inputs as in the reference's
+// serial order.
+// Runs fn on the second stream (c.s points at it meanwhile, so that every
+// launch and timer inside goes there).
+template <class Fn>
+static int on_s2(Ctx &c, Fn fn) {
+  hipStream_t main = c.s;
+  c.s = c.s2;
+  const int rc = fn();
+  c.s = main;
+  return rc;
+}
+
+// Fork: the second stream continues after everything enqueued so far on
+// the main stream.
+static int fork_s2(Ctx &c) {
+  HYD_HIP(hipEventRecord(c.ev_fork, c.s));
+  HYD_HIP(hipStreamWaitEvent(c.s2, c.ev_fork, 0));
+  return HYD_OK;
+}
+
 int advance(Ctx &c) {
   const int64_t N = c.N;
+  auto fields = [&](int stage) {
+    return on_s2(c, [&]() {
+      HYD_CHECK(rk_fields(c, stage));
+      HYD_HIP(hipEventRecord(c.ev_join, c.s));
+      return HYD_OK;
+    });
+  };
   // stage 1
   HYD_CHECK(inversion(c));
   hipLaunchKernelGGL(k_reset, grid1(N), dim3(256), 0, c.s, c.qs, c.qd, c.qc, c.flo, c.fhi, N);
   HYD_CHECK(adapt(c));
-  HYD_CHECK(source(c, 0));
+  HYD_CHECK(fork_s2(c));
+  HYD_CHECK(on_s2(c, [&]() { return source(c, 0); }));
   HYD_CHECK(rk_nodes(c, 1));
   HYD_CHECK(adapt_finish(c));
-  HYD_CHECK(rk_fields(c, 1));
-  // stage 2
-  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
-  HYD_CHECK(inversion(c));
-  HYD_CHECK(source(c, 1));
-  HYD_CHECK(rk_nodes(c, 2));
-  HYD_CHECK(rk_fields(c, 2));
-  // stage 3
-  HYD_CHECK(inversion(c));
-  HYD_CHECK(source(c, 1));
-  HYD_CHECK(rk_nodes(c, 3));
-  HYD_CHECK(rk_fields(c, 3));
-  // stage 4
-  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
-  HYD_CHECK(inversion(c));
-  HYD_CHECK(source(c, 2));
-  HYD_CHECK(rk_nodes(c, 4));
-  HYD_CHECK(rk_fields(c, 4));
+  // the field update needs the integrating factors adapt_finish uploads
+  HYD_CHECK(fork_s2(c));
+  HYD_CHECK(fields(1));
+  // stages 2-4
+  for (int stage = 2; stage <= 4; stage++) {
+    if (stage != 3) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
+    HYD_CHECK(contour_grid(c));
+    HYD_HIP(hipStreamWaitEvent(c.s, c.ev_join, 0));
+    HYD_CHECK(inversion_fields(c));
+    HYD_CHECK(fork_s2(c));
+    HYD_CHECK(on_s2(c, [&]() { return source(c, stage == 4 ? 2 : 1); }));
+    HYD_CHECK(rk_nodes(c, stage));
+    HYD_CHECK(fields(stage));
+  }
+  HYD_HIP(hipStreamWaitEvent(c.s, c.ev_join, 0));
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }
