@@ -94,7 +94,7 @@ static void free_ctx(Ctx *c) {
                   c->rp_tid, c->rp_list, c->xq, c->yq, c->xqi, c->yqi, c->xqf, c->yqf, c->uq,
                   c->vq, c->nextq, c->nxt_tmp, c->dsc, c->dsum, c->seq_work, c->uu32,
                   c->vv32, c->fa32, c->fb32, c->fc32, c->tmp32, c->fq32, c->mq32, c->xtrig32,
-                  c->ytrig32, c->qb32, c->qdiss_u_dev, c->part, c->c2g_q, c->c2g_w};
+                  c->ytrig32, c->qb32, c->qdiss_u_dev, c->part, c->c2g_q, c->c2g_w, c->c2g_q32};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int k = 0; k < 4; k++) {
@@ -276,7 +276,8 @@ static int create(const hyd_params *p, hyd_ctx **out) {
   }
   if (p->precision == 1) {
     float **f32[] = {&c->uu32, &c->vv32, &c->fa32,   &c->fb32,   &c->fc32,  &c->tmp32,
-                     &c->fd32, &c->fe32, &c->t32[0], &c->t32[1], &c->t32[2], &c->t32[3]};
+                     &c->fd32, &c->fe32, &c->t32[0], &c->t32[1], &c->t32[2], &c->t32[3],
+                     &c->c2g_q32};
     for (float **f : f32) TRY(dalloc(f, N));
     TRY(dalloc(&c->fq32, nfine));
     TRY(dalloc(&c->mq32, nfine / 4));

@@ -134,6 +134,7 @@ struct Ctx {
   int64_t replay_cap = 0;
   // mixed precision (precision == 1): FP32 physical fields and twiddles
   float *uu32 = nullptr, *vv32 = nullptr, *fa32 = nullptr, *fb32 = nullptr, *fc32 = nullptr;
+  float *c2g_q32 = nullptr;  // con2grid_mixed's output grid
   float *tmp32 = nullptr, *fq32 = nullptr, *mq32 = nullptr;
   float *fd32 = nullptr, *fe32 = nullptr, *t32[4] = {nullptr, nullptr, nullptr, nullptr};
   double *t4[4] = {nullptr, nullptr, nullptr, nullptr};  // FFT temporaries of a batch

@@ -602,10 +602,14 @@ static int source_mixed(Ctx &c, int lev) {
   return HYD_OK;
 }
 
-static int inversion_mixed(Ctx &c) {
+static int contour_grid_mixed(Ctx &c) {
+  HYD_CHECK(con2grid_mixed(c, c.c2g_q32));
+  HYD_CHECK(fft_ptospc32(c, c.c2g_q32, c.qc));
+  return HYD_OK;
+}
+
+static int inversion_fields_mixed(Ctx &c) {
   const int64_t N = c.N;
-  HYD_CHECK(con2grid_mixed(c, c.fa32));
-  HYD_CHECK(fft_ptospc32(c, c.fa32, c.qc));
   hipLaunchKernelGGL(k_combine_qq, grid1(N), dim3(256), 0, c.s, c.qq, c.flo, c.qs, c.qc, c.qd, N);
   hipLaunchKernelGGL(k_green, grid1(N), dim3(256), 0, c.s, c.pp, c.green, c.qq, N);
   HYD_HIP(hipGetLastError());
@@ -619,29 +623,35 @@ static int inversion_mixed(Ctx &c) {
   return HYD_OK;
 }
 
-// advance (evolution.f90:166-326) with the mixed-precision grid work
+// The same stage overlap as the plain step (advance); the fork and join are
+// events, so a stream capture of the main stream records both streams.
 int advance_mixed(Ctx &c) {
   const int64_t N = c.N;
-  HYD_CHECK(inversion_mixed(c));
+  auto fields = [&](int stage) {
+    return on_s2(c, [&]() {
+      HYD_CHECK(rk_fields(c, stage));
+      HYD_HIP(hipEventRecord(c.ev_join, c.s));
+      return HYD_OK;
+    });
+  };
+  HYD_CHECK(contour_grid_mixed(c));
+  HYD_CHECK(inversion_fields_mixed(c));
   hipLaunchKernelGGL(k_reset, grid1(N), dim3(256), 0, c.s, c.qs, c.qd, c.qc, c.flo, c.fhi, N);
   HYD_CHECK(adapt_mixed(c));
-  HYD_CHECK(source_mixed(c, 0));
-  HYD_CHECK(rk_nodes(c, 1));
-  HYD_CHECK(rk_fields(c, 1));
-  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
-  HYD_CHECK(inversion_mixed(c));
-  HYD_CHECK(source_mixed(c, 1));
-  HYD_CHECK(rk_nodes(c, 2));
-  HYD_CHECK(rk_fields(c, 2));
-  HYD_CHECK(inversion_mixed(c));
-  HYD_CHECK(source_mixed(c, 1));
-  HYD_CHECK(rk_nodes(c, 3));
-  HYD_CHECK(rk_fields(c, 3));
-  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
-  HYD_CHECK(inversion_mixed(c));
-  HYD_CHECK(source_mixed(c, 2));
-  HYD_CHECK(rk_nodes(c, 4));
-  HYD_CHECK(rk_fields(c, 4));
+  for (int stage = 1; stage <= 4; stage++) {
+    if (stage > 1) {
+      if (stage != 3) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
+      HYD_CHECK(contour_grid_mixed(c));
+      HYD_HIP(hipStreamWaitEvent(c.s, c.ev_join, 0));
+      HYD_CHECK(inversion_fields_mixed(c));
+    }
+    HYD_CHECK(fork_s2(c));
+    const int lev = stage == 1 ? 0 : (stage == 4 ? 2 : 1);
+    HYD_CHECK(on_s2(c, [&]() { return source_mixed(c, lev); }));
+    HYD_CHECK(rk_nodes(c, stage));
+    HYD_CHECK(fields(stage));
+  }
+  HYD_HIP(hipStreamWaitEvent(c.s, c.ev_join, 0));
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }
