@@ -71,8 +71,8 @@ def pmc_summary(fetch_dir, write_dir, ng, out_json):
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the same
     counter on the calibration copy, as bytes per algorithmic byte.  A
     dispatch's algorithmic bytes are (sizeof in + sizeof out) x n x vectors;
-    the vectors follow from its grid and its LDS size (2 V n + 2 n values of
-    the compute type, V vectors per workgroup of 256)."""
+    the vectors follow from its grid and its LDS size (2 V n values of the
+    compute type, V vectors per workgroup of 256)."""
     import csv
     import glob
     import re
@@ -100,7 +100,7 @@ def pmc_summary(fetch_dir, write_dir, ng, out_json):
                 continue
             n = int(m.group(1))
             ti, t, to = (size[m.group(i)] for i in (3, 4, 5))
-            V = max(1, round(int(row["LDS_Block_Size"]) / (t * 2 * n) - 1))
+            V = max(1, int(row["LDS_Block_Size"]) // (t * 2 * n))
             vectors = int(row["Grid_Size"]) // 256 * V
             alg = (ti + to) * n * vectors
             rec = per.setdefault(m.group(0), [0.0, 0.0, 0])

@@ -30,6 +30,12 @@ namespace hyd {
 // (blockIdx.y selects the field); each field has its own input, output,
 // fused derivative and scaling.
 static constexpr int kMaxF = 4;
+// Twiddles read from the (L1/L2-resident) global table rather than staged in
+// LDS: the smaller LDS footprint lets more blocks share a CU.
+static constexpr bool kTrigInLds = false;
+static size_t fft_lds_bytes(int n, int V, size_t tsize) {
+  return tsize * (2 * (size_t)V * n + (kTrigInLds ? 2 * (size_t)n : 0));
+}
 struct FftArgs {
   const void *in[kMaxF];    // TI
   void *out[kMaxF];         // TO
@@ -282,10 +288,11 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
   const int bid = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
   const int v0 = bid * V;
   T *bufA = lds, *bufB = lds + V * N;
-  T *tcs = lds + 2 * V * N, *tsn = tcs + N;
+  const T *tcs = kTrigInLds ? lds + 2 * V * N : gtrig, *tsn = tcs + N;
   const int tot = V * N;
   constexpr int nhalf = N / 2;
-  for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) tcs[i] = gtrig[i];
+  if (kTrigInLds)
+    for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) lds[2 * V * N + i] = gtrig[i];
   // ---- load
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     int v, e;
@@ -455,7 +462,7 @@ static int fft_launch_t(Ctx &c, bool fwd, int F, const TI *const *in, TO *const 
   g.out_rows = out_rows;
   g.n2 = c.plan.n2;
   g.n4 = c.plan.n4;
-  const size_t shm = sizeof(T) * (2 * (size_t)V * n + 2 * (size_t)n);
+  const size_t shm = fft_lds_bytes(n, V, sizeof(T));
   const dim3 grid(m / V, F);
   // one launch reads and writes F x m vectors of n values
   KTimer tk(c, sizeof(T) == 8 ? (fwd ? "k:fft_fwd" : "k:fft_rev") : (fwd ? "k:fft32_fwd" : "k:fft32_rev"),
@@ -509,7 +516,7 @@ int fft_autotune(Ctx &c) {
     float best = 1e30f;
     int bestV = 0;
     for (int V = 1; V <= 32 && V <= n; V *= 2) {
-      if ((2 * (size_t)V * n + 2 * (size_t)n) * sizeof(double) > 160 * 1024) break;
+      if (fft_lds_bytes(n, V, sizeof(double)) > 160 * 1024) break;
       c.fft_v[cls] = V;
       const double *trig = fw[cls] ? c.xtrig : c.ytrig;
       rc = fft_launch(c, fw[cls], c.wa, c.wb, n, n, inr[cls], outr[cls], 0, nullptr, 1.0, trig);
