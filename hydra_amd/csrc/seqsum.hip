@@ -35,7 +35,7 @@
 
 namespace hyd {
 
-static constexpr int kChunk = 128;   // values per chunk
+static constexpr int kChunk = 64;    // values per chunk
 static constexpr int kTile = 32;     // columns staged through LDS per pass
 
 // A segment of a chunk's guessed run: an optional head addition, then
@@ -96,6 +96,11 @@ __device__ __forceinline__ double readlane_d(double x, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
   return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t x, int l) {
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)x, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
 }
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -198,16 +203,15 @@ __device__ __forceinline__ int parity_of(double num, double uinv) {
 // the rounding of the margin tests.  A start inside its interval is
 // therefore accepted by every member and gives the member chain's value;
 // any other window is summed by the member chain.
-static constexpr int kWin = 8;
+static constexpr int kWin = 16;
 
 struct CompRec {
   double g;                   // first member's guess
   double d0, d1;              // window translations for start parity 0 / 1
   double lo0, hi0, lo1, hi1;  // admissible delta (units of u, magnitude) per parity
   int32_t win;                // window index
-  int32_t meta;               // nm | ok << 4 | (map k << 1 | map c) << 6
-  uint32_t cnts;              // record count of member chunk i in bits 4i..4i+3
-  int32_t pad;
+  int32_t meta;               // nm | ok << 5 | (map k << 1 | map c) << 6
+  uint64_t cnts;              // record count of member chunk i in bits 4i..4i+3
 };
 
 // Folds members R[0..nm) (one record per chunk); gnext is the guess of the
@@ -263,7 +267,7 @@ __device__ CompRec compose_window(const SegRec *R, int nm, bool has_next, double
   C.hi1 = hi[1];
   C.win = (int32_t)win;
   const int mc = pn[0], mk = pn[0] ^ pn[1];
-  C.meta = nm | (ok << 4) | (((mk << 1) | mc) << 6);
+  C.meta = nm | (ok << 5) | (((mk << 1) | mc) << 6);
   return C;
 }
 
@@ -405,8 +409,8 @@ __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, i
       for (int i = 0; i < nm; i++) single = single && fcnt[t * kWin + i] == 1;
       const bool has_next = w0 + nm < nch;
       const double gnext = (t + 1) * kWin < 64 ? first[(t + 1) * kWin].g : gnx;
-      uint32_t cn = 0;
-      for (int i = 0; i < nm; i++) cn |= (uint32_t)fcnt[t * kWin + i] << (4 * i);
+      uint64_t cn = 0;
+      for (int i = 0; i < nm; i++) cn |= (uint64_t)fcnt[t * kWin + i] << (4 * i);
       CompRec C;
       if (single) {
         C = compose_window(&first[t * kWin], nm, has_next, gnext, w0 / kWin);
@@ -416,7 +420,6 @@ __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, i
         C.meta = nm;
       }
       C.cnts = cn;
-      C.pad = 0;
       cps[w0 / kWin] = C;
     }
   }
@@ -444,6 +447,17 @@ __device__ void chain_group(const double *__restrict__ a, const SegRec &R, int n
   const int rep = valid ? (R.meta >> 19) & 1 : 0;
   const int key = segkey(R.g);
   const double gn = dpp_d<0x130>(R.g);  // next lane's guess (wave_shl:1)
+  // Everything that depends on the lane's own record only, computed once:
+  // 1/u = 2^(52 - E) of its binade (exact; used only while in range), its
+  // parity map p -> b ? a : p ^ a towards the next lane (used only when the
+  // next lane continues the run, hence shares the binade), and whether the
+  // guessed run itself keeps the margins.
+  const int eb = binade(R.g);
+  const bool eb_ok = eb >= 2 && eb <= 2045;
+  const double ui = eb_ok ? pow2_field(2098 - eb) : 0.0;
+  const int map_own = ((parity_of(R.d1 - R.d0, ui) ? 0 : 1) << 1) |
+                      parity_of((R.g + R.d0) - gn, ui);
+  const bool g_inside = eb_ok && run_inside(R.g, eb, R.lo, R.hi);
   int q = 0;
   while (q < nv) {
     const int meta_q = __builtin_amdgcn_readlane(R.meta, q);
@@ -462,21 +476,14 @@ __device__ void chain_group(const double *__restrict__ a, const SegRec &R, int n
     const int h = stop ? __ffsll((unsigned long long)stop) - 1 : nv;
     const bool in = l >= q && l < h;
     const double g_q = readlane_d(R.g, q);
-    const int eq = binade(g_q);
-    // 1/u = 2^(52 - E) for the binade of g_q (exact; only used while in range)
-    const double ui = (eq >= 2 && eq <= 2045) ? pow2_field(2098 - eq) : 0.0;
+    const double ui_q = readlane_d(ui, q);
     // start parities: p_(l+1) = b_l ? a_l : p_l ^ a_l, a GF(2) affine scan
-    int mp = 2;  // identity map
-    if (in && l + 1 < h) {
-      const int al = parity_of((R.g + R.d0) - gn, ui);
-      const int bl = parity_of(R.d1 - R.d0, ui);
-      mp = ((bl ? 0 : 1) << 1) | al;
-    }
+    const int mp = (in && l + 1 < h) ? map_own : 2;  // 2: identity
     const int incl_m = wave_incl_map(mp);
     int ex_m = dpp_i<0x138>(incl_m);
     if (l == 0) ex_m = 2;
     // maps of lanes < q are identities, so each prefix starts at lane q
-    const int p_q = segkey(S) == key_q ? parity_of(S - g_q, ui) : 0;
+    const int p_q = segkey(S) == key_q ? parity_of(S - g_q, ui_q) : 0;
     const int p = (ex_m >> 1) ? (p_q ^ (ex_m & 1)) : (ex_m & 1);
     const double d = in ? (p ? R.d1 : R.d0) : 0.0;
     // lanes < q add zeros, so lane l's exclusive partial covers [q, l)
@@ -484,9 +491,8 @@ __device__ void chain_group(const double *__restrict__ a, const SegRec &R, int n
     const double Sl = S + excl;
     bool acc = false;
     if (in) {
-      const int eb = binade(R.g);
-      acc = segkey(Sl) == key && eb >= 2 && eb <= 2045 && run_inside(Sl, eb, R.lo, R.hi) &&
-            run_inside(R.g, eb, R.lo, R.hi) && parity_of(Sl - R.g, ui) == p;
+      acc = g_inside && segkey(Sl) == key && run_inside(Sl, eb, R.lo, R.hi) &&
+            parity_of(Sl - R.g, ui) == p;
     }
     const uint64_t rej = __ballot(in && !acc);
     if (rej == 0) {
@@ -508,7 +514,7 @@ __device__ void chain_group(const double *__restrict__ a, const SegRec &R, int n
 // Records of window win (chunks kWin win .. + nm, counts packed in cnts),
 // one per lane in order, summed by the member chain from S.
 __device__ void member_level(const double *__restrict__ a, const SegRec *__restrict__ sg, int cap,
-                             int64_t win, int nm, uint32_t cnts, double &S) {
+                             int64_t win, int nm, uint64_t cnts, double &S) {
   const int l = lane_id();
   const int64_t c0 = win * kWin;
   // lane r takes record r: chunk i with off_i <= r < off_i + cnt_i
@@ -536,14 +542,14 @@ __device__ void chain_comp(const double *__restrict__ a, const SegRec *__restric
                            const CompRec &C, int nv, double &S) {
   const int l = lane_id();
   const bool valid = l < nv;
-  const int ok = valid ? (C.meta >> 4) & 1 : 0;
+  const int ok = valid ? (C.meta >> 5) & 1 : 0;
   const int key = segkey(C.g);
   int q = 0;
   while (q < nv) {
     const int meta_q = __builtin_amdgcn_readlane(C.meta, q);
     const int win_q = __builtin_amdgcn_readlane(C.win, q);
-    if (!((meta_q >> 4) & 1)) {
-      member_level(a, sg, cap, win_q, meta_q & 15, __builtin_amdgcn_readlane(C.cnts, q), S);
+    if (!((meta_q >> 5) & 1)) {
+      member_level(a, sg, cap, win_q, meta_q & 31, readlane_u64(C.cnts, q), S);
       q++;
       continue;
     }
@@ -580,7 +586,7 @@ __device__ void chain_comp(const double *__restrict__ a, const SegRec *__restric
     S = readlane_d(Sl, f);
     const int meta_f = __builtin_amdgcn_readlane(C.meta, f);
     const int win_f = __builtin_amdgcn_readlane(C.win, f);
-    member_level(a, sg, cap, win_f, meta_f & 15, __builtin_amdgcn_readlane(C.cnts, f), S);
+    member_level(a, sg, cap, win_f, meta_f & 31, readlane_u64(C.cnts, f), S);
     q = f + 1;
   }
 }
