@@ -49,6 +49,12 @@ struct FftArgs {
 };
 
 // ---------------------------------------------------------------------------
+// Radix stages.  The butterflies of one stage fall into three kinds (k = 0,
+// the pairs k / lv-k, and the middle k = lv/2 for even lv); each kind has
+// its own loop so that a wavefront never steps through another kind's code.
+// Items of a kind are numbered with the offset (irem / ic) fastest, then k,
+// then the vector.
+
 // forward stages (decimation in time)
 
 template <class T>
@@ -57,33 +63,34 @@ __device__ __forceinline__ void fwd_r2(const T *__restrict__ a, T *__restrict__ 
                                        const T *__restrict__ cs,
                                        const T *__restrict__ sn) {
   const int nh = (lv - 1) / 2;
-  const int nk = 1 + nh;
-  const int per = rem * nk;
-  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
-    const int v = it / per, r = it - v * per;
-    const int kk = r / rem, irem = r - kk * rem;
-    const T *av = a + v * n;
-    T *bv = b + v * n;
 #define A(j, k) av[irem + rem * ((j) + 2 * (k))]
 #define B(k, j) bv[irem + rem * ((k) + lv * (j))]
-    if (kk == 0) {
-      const T a00 = A(0, 0), a10 = A(1, 0);
-      B(0, 0) = a00 + a10;
-      B(0, 1) = a00 - a10;
-    } else {
-      const int k = kk, kc = lv - k;
-      const T c1k = cs[k], s1k = sn[k];
-      const T a1k = A(1, k), a1kc = A(1, kc), a0k = A(0, k), a0kc = A(0, kc);
-      const T x1 = c1k * a1k - s1k * a1kc;
-      const T y1 = c1k * a1kc + s1k * a1k;
-      B(k, 0) = a0k + x1;
-      B(kc, 0) = a0k - x1;
-      B(k, 1) = y1 - a0kc;
-      B(kc, 1) = a0kc + y1;
-    }
+  for (int it = threadIdx.x; it < V * rem; it += blockDim.x) {
+    const int v = it / rem, irem = it - v * rem;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
+    const T a00 = A(0, 0), a10 = A(1, 0);
+    B(0, 0) = a00 + a10;
+    B(0, 1) = a00 - a10;
+  }
+  const int per = rem * nh;
+  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+    const int v = it / per, r = it - v * per;
+    const int kq = r / rem, irem = r - kq * rem;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
+    const int k = 1 + kq, kc = lv - k;
+    const T c1k = cs[k], s1k = sn[k];
+    const T a1k = A(1, k), a1kc = A(1, kc), a0k = A(0, k), a0kc = A(0, kc);
+    const T x1 = c1k * a1k - s1k * a1kc;
+    const T y1 = c1k * a1kc + s1k * a1k;
+    B(k, 0) = a0k + x1;
+    B(kc, 0) = a0k - x1;
+    B(k, 1) = y1 - a0kc;
+    B(kc, 1) = a0kc + y1;
+  }
 #undef A
 #undef B
-  }
 }
 
 template <class T>
@@ -94,54 +101,61 @@ __device__ __forceinline__ void fwd_r4(const T *__restrict__ a, T *__restrict__ 
   const T rtf12 = (T)0.7071067811865475244008443621048490392848359376884740365883398689953662;
   const int nh = (lv - 1) / 2;
   const int even = (lv & 1) == 0;
-  const int nk = 1 + nh + even;
-  const int per = rem * nk;
-  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
-    const int v = it / per, r = it - v * per;
-    const int kk = r / rem, irem = r - kk * rem;
-    const T *av = a + v * n;
-    T *bv = b + v * n;
 #define A(j, k) av[irem + rem * ((j) + 4 * (k))]
 #define B(k, j) bv[irem + rem * ((k) + lv * (j))]
-    if (kk == 0) {
-      const T a0 = A(0, 0), a1 = A(1, 0), a2 = A(2, 0), a3 = A(3, 0);
-      const T t1r = a0 + a2;
-      const T t2r = a1 + a3;
-      B(0, 0) = t1r + t2r;
-      B(0, 1) = a0 - a2;
-      B(0, 2) = t1r - t2r;
-      B(0, 3) = a3 - a1;
-    } else if (kk <= nh) {
-      const int k = kk, kc = lv - k;
-      const T c1k = cs[k], s1k = sn[k];
-      const T c2k = cs[k + lv], s2k = sn[k + lv];
-      const T c3k = cs[k + 2 * lv], s3k = sn[k + 2 * lv];
-      const T a0k = A(0, k), a0kc = A(0, kc), a1k = A(1, k), a1kc = A(1, kc);
-      const T a2k = A(2, k), a2kc = A(2, kc), a3k = A(3, k), a3kc = A(3, kc);
-      const T x1p = c1k * a1k - s1k * a1kc;
-      const T y1p = c1k * a1kc + s1k * a1k;
-      const T x2p = c2k * a2k - s2k * a2kc;
-      const T y2p = c2k * a2kc + s2k * a2k;
-      const T x3p = c3k * a3k - s3k * a3kc;
-      const T y3p = c3k * a3kc + s3k * a3k;
-      const T t1r = a0k + x2p;
-      const T t1i = a0kc + y2p;
-      const T t2r = x1p + x3p;
-      const T t2i = y1p + y3p;
-      const T t3r = a0k - x2p;
-      const T t3i = a0kc - y2p;
-      const T t4r = x3p - x1p;
-      const T t4i = y1p - y3p;
-      B(k, 0) = t1r + t2r;
-      B(kc, 0) = t3r - t4i;
-      B(k, 1) = t3r + t4i;
-      B(kc, 1) = t1r - t2r;
-      B(k, 2) = t2i - t1i;
-      B(kc, 2) = t3i + t4r;
-      B(k, 3) = t4r - t3i;
-      B(kc, 3) = t1i + t2i;
-    } else {
-      const int l2 = lv / 2;
+  for (int it = threadIdx.x; it < V * rem; it += blockDim.x) {
+    const int v = it / rem, irem = it - v * rem;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
+    const T a0 = A(0, 0), a1 = A(1, 0), a2 = A(2, 0), a3 = A(3, 0);
+    const T t1r = a0 + a2;
+    const T t2r = a1 + a3;
+    B(0, 0) = t1r + t2r;
+    B(0, 1) = a0 - a2;
+    B(0, 2) = t1r - t2r;
+    B(0, 3) = a3 - a1;
+  }
+  const int per = rem * nh;
+  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+    const int v = it / per, r = it - v * per;
+    const int kq = r / rem, irem = r - kq * rem;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
+    const int k = 1 + kq, kc = lv - k;
+    const T c1k = cs[k], s1k = sn[k];
+    const T c2k = cs[k + lv], s2k = sn[k + lv];
+    const T c3k = cs[k + 2 * lv], s3k = sn[k + 2 * lv];
+    const T a0k = A(0, k), a0kc = A(0, kc), a1k = A(1, k), a1kc = A(1, kc);
+    const T a2k = A(2, k), a2kc = A(2, kc), a3k = A(3, k), a3kc = A(3, kc);
+    const T x1p = c1k * a1k - s1k * a1kc;
+    const T y1p = c1k * a1kc + s1k * a1k;
+    const T x2p = c2k * a2k - s2k * a2kc;
+    const T y2p = c2k * a2kc + s2k * a2k;
+    const T x3p = c3k * a3k - s3k * a3kc;
+    const T y3p = c3k * a3kc + s3k * a3k;
+    const T t1r = a0k + x2p;
+    const T t1i = a0kc + y2p;
+    const T t2r = x1p + x3p;
+    const T t2i = y1p + y3p;
+    const T t3r = a0k - x2p;
+    const T t3i = a0kc - y2p;
+    const T t4r = x3p - x1p;
+    const T t4i = y1p - y3p;
+    B(k, 0) = t1r + t2r;
+    B(kc, 0) = t3r - t4i;
+    B(k, 1) = t3r + t4i;
+    B(kc, 1) = t1r - t2r;
+    B(k, 2) = t2i - t1i;
+    B(kc, 2) = t3i + t4r;
+    B(k, 3) = t4r - t3i;
+    B(kc, 3) = t1i + t2i;
+  }
+  if (even) {
+    const int l2 = lv / 2;
+    for (int it = threadIdx.x; it < V * rem; it += blockDim.x) {
+      const int v = it / rem, irem = it - v * rem;
+      const T *av = a + v * n;
+      T *bv = b + v * n;
       const T a0 = A(0, l2), a1 = A(1, l2), a2 = A(2, l2), a3 = A(3, l2);
       const T q1 = rtf12 * (a1 - a3);
       const T q2 = rtf12 * (a1 + a3);
@@ -150,9 +164,9 @@ __device__ __forceinline__ void fwd_r4(const T *__restrict__ a, T *__restrict__ 
       B(l2, 2) = a2 - q2;
       B(l2, 3) = -a2 - q2;
     }
+  }
 #undef A
 #undef B
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -166,56 +180,63 @@ __device__ __forceinline__ void rev_r4(const T *__restrict__ a, T *__restrict__ 
   const T rtf12 = (T)0.7071067811865475244008443621048490392848359376884740365883398689953662;
   const int nh = (lv - 1) / 2;
   const int even = (lv & 1) == 0;
-  const int nk = 1 + nh + even;
-  const int per = cum * nk;
-  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
-    const int v = it / per, r = it - v * per;
-    const int kk = r / cum, ic = r - kk * cum;
-    const T *av = a + v * n;
-    T *bv = b + v * n;
 #define A(k, j) av[ic + cum * ((k) + lv * (j))]
 #define B(j, k) bv[ic + cum * ((j) + 4 * (k))]
-    if (kk == 0) {
-      const T a0 = A(0, 0), a1 = A(0, 1), a2 = A(0, 2), a3 = A(0, 3);
-      const T t1r = a0 + a2;
-      const T t2r = a1;
-      const T t3r = a0 - a2;
-      const T t4r = a3;
-      B(0, 0) = t1r + t2r;
-      B(1, 0) = t3r + t4r;
-      B(2, 0) = t1r - t2r;
-      B(3, 0) = t3r - t4r;
-    } else if (kk <= nh) {
-      const int k = kk, kc = lv - k;
-      const T c1k = cs[k], s1k = sn[k];
-      const T c2k = cs[k + lv], s2k = sn[k + lv];
-      const T c3k = cs[k + 2 * lv], s3k = sn[k + 2 * lv];
-      const T ak0 = A(k, 0), ak1 = A(k, 1), ak2 = A(k, 2), ak3 = A(k, 3);
-      const T akc0 = A(kc, 0), akc1 = A(kc, 1), akc2 = A(kc, 2), akc3 = A(kc, 3);
-      const T t1r = ak0 + akc1;
-      const T t1i = akc3 - ak2;
-      const T t2r = ak1 + akc0;
-      const T t2i = akc2 - ak3;
-      const T t3r = ak0 - akc1;
-      const T t3i = akc3 + ak2;
-      const T t4r = ak1 - akc0;
-      const T t4i = akc2 + ak3;
-      const T x1p = t3r + t4i;
-      const T y1p = t3i - t4r;
-      const T x2p = t1r - t2r;
-      const T y2p = t1i - t2i;
-      const T x3p = t3r - t4i;
-      const T y3p = t3i + t4r;
-      B(0, k) = t1r + t2r;
-      B(0, kc) = t1i + t2i;
-      B(1, k) = c1k * x1p - s1k * y1p;
-      B(1, kc) = c1k * y1p + s1k * x1p;
-      B(2, k) = c2k * x2p - s2k * y2p;
-      B(2, kc) = c2k * y2p + s2k * x2p;
-      B(3, k) = c3k * x3p - s3k * y3p;
-      B(3, kc) = c3k * y3p + s3k * x3p;
-    } else {
-      const int l2 = lv / 2;
+  for (int it = threadIdx.x; it < V * cum; it += blockDim.x) {
+    const int v = it / cum, ic = it - v * cum;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
+    const T a0 = A(0, 0), a1 = A(0, 1), a2 = A(0, 2), a3 = A(0, 3);
+    const T t1r = a0 + a2;
+    const T t2r = a1;
+    const T t3r = a0 - a2;
+    const T t4r = a3;
+    B(0, 0) = t1r + t2r;
+    B(1, 0) = t3r + t4r;
+    B(2, 0) = t1r - t2r;
+    B(3, 0) = t3r - t4r;
+  }
+  const int per = cum * nh;
+  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+    const int v = it / per, r = it - v * per;
+    const int kq = r / cum, ic = r - kq * cum;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
+    const int k = 1 + kq, kc = lv - k;
+    const T c1k = cs[k], s1k = sn[k];
+    const T c2k = cs[k + lv], s2k = sn[k + lv];
+    const T c3k = cs[k + 2 * lv], s3k = sn[k + 2 * lv];
+    const T ak0 = A(k, 0), ak1 = A(k, 1), ak2 = A(k, 2), ak3 = A(k, 3);
+    const T akc0 = A(kc, 0), akc1 = A(kc, 1), akc2 = A(kc, 2), akc3 = A(kc, 3);
+    const T t1r = ak0 + akc1;
+    const T t1i = akc3 - ak2;
+    const T t2r = ak1 + akc0;
+    const T t2i = akc2 - ak3;
+    const T t3r = ak0 - akc1;
+    const T t3i = akc3 + ak2;
+    const T t4r = ak1 - akc0;
+    const T t4i = akc2 + ak3;
+    const T x1p = t3r + t4i;
+    const T y1p = t3i - t4r;
+    const T x2p = t1r - t2r;
+    const T y2p = t1i - t2i;
+    const T x3p = t3r - t4i;
+    const T y3p = t3i + t4r;
+    B(0, k) = t1r + t2r;
+    B(0, kc) = t1i + t2i;
+    B(1, k) = c1k * x1p - s1k * y1p;
+    B(1, kc) = c1k * y1p + s1k * x1p;
+    B(2, k) = c2k * x2p - s2k * y2p;
+    B(2, kc) = c2k * y2p + s2k * x2p;
+    B(3, k) = c3k * x3p - s3k * y3p;
+    B(3, kc) = c3k * y3p + s3k * x3p;
+  }
+  if (even) {
+    const int l2 = lv / 2;
+    for (int it = threadIdx.x; it < V * cum; it += blockDim.x) {
+      const int v = it / cum, ic = it - v * cum;
+      const T *av = a + v * n;
+      T *bv = b + v * n;
       const T a0 = A(l2, 0), a1 = A(l2, 1), a2 = A(l2, 2), a3 = A(l2, 3);
       B(0, l2) = a0 + a1;
       B(2, l2) = a3 - a2;
@@ -224,9 +245,9 @@ __device__ __forceinline__ void rev_r4(const T *__restrict__ a, T *__restrict__ 
       B(1, l2) = rtf12 * (t3r + t4r);
       B(3, l2) = rtf12 * (t4r - t3r);
     }
+  }
 #undef A
 #undef B
-  }
 }
 
 template <class T>
@@ -235,33 +256,34 @@ __device__ __forceinline__ void rev_r2(const T *__restrict__ a, T *__restrict__ 
                                        const T *__restrict__ cs,
                                        const T *__restrict__ sn) {
   const int nh = (lv - 1) / 2;
-  const int nk = 1 + nh;
-  const int per = cum * nk;
-  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
-    const int v = it / per, r = it - v * per;
-    const int kk = r / cum, ic = r - kk * cum;
-    const T *av = a + v * n;
-    T *bv = b + v * n;
 #define A(k, j) av[ic + cum * ((k) + lv * (j))]
 #define B(j, k) bv[ic + cum * ((j) + 2 * (k))]
-    if (kk == 0) {
-      const T a0 = A(0, 0), a1 = A(0, 1);
-      B(0, 0) = a0 + a1;
-      B(1, 0) = a0 - a1;
-    } else {
-      const int k = kk, kc = lv - k;
-      const T c1k = cs[k], s1k = sn[k];
-      const T ak0 = A(k, 0), akc0 = A(kc, 0), ak1 = A(k, 1), akc1 = A(kc, 1);
-      const T x1p = ak0 - akc0;
-      const T y1p = akc1 + ak1;
-      B(0, k) = ak0 + akc0;
-      B(0, kc) = akc1 - ak1;
-      B(1, k) = c1k * x1p - s1k * y1p;
-      B(1, kc) = c1k * y1p + s1k * x1p;
-    }
+  for (int it = threadIdx.x; it < V * cum; it += blockDim.x) {
+    const int v = it / cum, ic = it - v * cum;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
+    const T a0 = A(0, 0), a1 = A(0, 1);
+    B(0, 0) = a0 + a1;
+    B(1, 0) = a0 - a1;
+  }
+  const int per = cum * nh;
+  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+    const int v = it / per, r = it - v * per;
+    const int kq = r / cum, ic = r - kq * cum;
+    const T *av = a + v * n;
+    T *bv = b + v * n;
+    const int k = 1 + kq, kc = lv - k;
+    const T c1k = cs[k], s1k = sn[k];
+    const T ak0 = A(k, 0), akc0 = A(kc, 0), ak1 = A(k, 1), akc1 = A(kc, 1);
+    const T x1p = ak0 - akc0;
+    const T y1p = akc1 + ak1;
+    B(0, k) = ak0 + akc0;
+    B(0, kc) = akc1 - ak1;
+    B(1, k) = c1k * x1p - s1k * y1p;
+    B(1, kc) = c1k * y1p + s1k * x1p;
+  }
 #undef A
 #undef B
-  }
 }
 
 // ---------------------------------------------------------------------------
