@@ -563,10 +563,17 @@ int c2g_status(Ctx &c) {
   return HYD_ECAPACITY;
 }
 
+// defer_mean (n > 0, no counts_out): the averaged grid is left in c.c2g_w
+// and its sequential sum in c.dsum; the caller removes the mean (the
+// forward FFT does it while loading) and qc is not written.
 int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n,
-             double *qc, int32_t *counts_out) {
+             double *qc, int32_t *counts_out, bool defer_mean) {
   const Consts &k = c.c;
   const int64_t N = c.N;
+  if (defer_mean && (n == 0 || counts_out)) {
+    set_error("con2grid: deferred mean needs nodes and no counts");
+    return HYD_EARG;
+  }
   if (n == 0) {
     hipLaunchKernelGGL(fill_zero, dim3((N + 255) / 256), dim3(256), 0, c.s, qc, N);
     HYD_HIP(hipGetLastError());
@@ -591,6 +598,7 @@ int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, i
   HYD_HIP(hipGetLastError());
   // sequential mean, contours.f90:330-336
   HYD_CHECK(seqsum(c, c.c2g_w, N, c.dsum));
+  if (defer_mean) return HYD_OK;
   hipLaunchKernelGGL(c2g_subtract, dim3((N + 255) / 256), dim3(256), 0, c.s, c.c2g_w, qc, N, c.dsum,
                      (double)N);
   HYD_HIP(hipGetLastError());

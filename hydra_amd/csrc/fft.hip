@@ -43,6 +43,8 @@ struct FftArgs {
   int pre[kMaxF];           // 0 none, 1 derivative across vectors, 2 along elements
   double post[kMaxF];       // normalisation (and sign) folded into the store
   const void *trig;         // T, 2n: cos then -sin (stafft.f90:103-122)
+  const double *sub;        // forward: subtract *sub / subdiv from every input value
+  double subdiv;
   int n, m, V, lgV;
   int in_rows, out_rows;
   int n2, n4;
@@ -328,7 +330,9 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
     const int vg = v0 + v;
     T x;
     if (FWD) {
-      x = (T)(g.in_rows ? gin[(int64_t)vg * N + e] : gin[vg + (int64_t)m * e]);
+      const TI raw = g.in_rows ? gin[(int64_t)vg * N + e] : gin[vg + (int64_t)m * e];
+      // con2grid's mean removal (contours.f90:338-345) fused into the load
+      x = g.sub ? (T)(raw - *g.sub / g.subdiv) : (T)raw;
     } else {
       // optional fused derivative (sta2dfft.f90:169-245)
       if (gpre == 2) {
@@ -451,7 +455,8 @@ static int launch_n(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftArg
 template <class TI, class T, class TO>
 static int fft_launch_t(Ctx &c, bool fwd, int F, const TI *const *in, TO *const *out, int n,
                         int m, int in_rows, int out_rows, const int *pre,
-                        const double *const *dk, const double *post, const T *trig) {
+                        const double *const *dk, const double *post, const T *trig,
+                        const double *sub = nullptr, double subdiv = 1.0) {
   if (F < 1 || F > kMaxF) {
     set_error("fft: batch of 1..4 fields");
     return HYD_EARG;
@@ -466,6 +471,8 @@ static int fft_launch_t(Ctx &c, bool fwd, int F, const TI *const *in, TO *const 
     g.post[f] = post[q];
   }
   g.trig = trig;
+  g.sub = sub;
+  g.subdiv = subdiv;
   g.n = n;
   g.m = m;
   // batch per workgroup: the autotuned value of this layout class when it
@@ -568,6 +575,19 @@ int fft_autotune(Ctx &c) {
 
 // ptospc, sta2dfft.f90:103-132: x-transform (ny strided vectors) written as
 // rows = svar(kx,iy); then the y-transform of svar in place.
+// ptospc of rvar - *sum / ndiv (the subtraction done while loading).
+int fft_ptospc_mean(Ctx &c, const double *rvar, const double *sum, double ndiv, double *svar) {
+  const int n = c.c.nx;
+  KTimer t(c, "fft_ptospc", 4 * 8 * c.N);
+  const double *in[1] = {rvar};
+  double *out[1] = {svar};
+  const double p[1] = {c.plan.fnorm};
+  HYD_CHECK((fft_launch_t<double, double, double>(c, true, 1, in, out, n, n, 0, 1, nullptr,
+                                                   nullptr, p, c.xtrig, sum, ndiv)));
+  HYD_CHECK(fft_launch(c, true, svar, svar, n, n, 0, 0, 0, nullptr, c.plan.fnorm, c.ytrig));
+  return HYD_OK;
+}
+
 int fft_ptospc(Ctx &c, const double *rvar, double *svar) {
   const int n = c.c.nx;
   KTimer t(c, "fft_ptospc", 4 * 8 * c.N);

@@ -179,6 +179,7 @@ struct Ctx {
 // FFT: ptospc / spctop with optional fused spectral pre-operations.
 enum SpecPre { PRE_NONE = 0, PRE_XDERIV = 1, PRE_YDERIV = 2 };
 int fft_ptospc(Ctx &c, const double *rvar, double *svar);
+int fft_ptospc_mean(Ctx &c, const double *rvar, const double *sum, double ndiv, double *svar);
 int fft_spctop(Ctx &c, const double *svar, double *rvar, int pre, double post_sign,
                double *tmp);
 int fft_forward_1d(Ctx &c, double *x, int m);   // forfft(m, ng, x)
@@ -208,7 +209,7 @@ int64_t seqsum_work_chunks(int64_t n);
 int velint(Ctx &c, const double *uu, const double *vv, const double *xq, const double *yq,
            int n, double *uq, double *vq);
 int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n,
-             double *qc, int32_t *counts_out);
+             double *qc, int32_t *counts_out, bool defer_mean = false);
 int c2g_status(Ctx &c);
 
 template <class T>

@@ -34,6 +34,17 @@ PW_KERNEL(k_combine_qq,
            const double *__restrict__ qc, const double *__restrict__ qd, int64_t n),
           { qq[k] = flo[k] * (qs[k] - qc[k]) + qc[k] + qd[k]; })
 
+// qq as k_combine_qq, then pp = green*qq (spectral.f90:321-325), one pass
+PW_KERNEL(k_combine_green,
+          (double *__restrict__ qq, double *__restrict__ pp, const double *__restrict__ flo,
+           const double *__restrict__ green, const double *__restrict__ qs,
+           const double *__restrict__ qc, const double *__restrict__ qd, int64_t n),
+          {
+            const double q = flo[k] * (qs[k] - qc[k]) + qc[k] + qd[k];
+            qq[k] = q;
+            pp[k] = green[k] * q;
+          })
+
 // pp = green*qq  (spectral.f90:321-325)
 PW_KERNEL(k_green, (double *__restrict__ pp, const double *__restrict__ green,
                     const double *__restrict__ qq, int64_t n),
@@ -205,18 +216,21 @@ static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); 
 
 // The contour part of inversion: qc = ptospc(con2grid) (evolution.f90:507-508).
 static int contour_grid(Ctx &c) {
-  HYD_CHECK(con2grid(c, c.xq, c.yq, c.nextq, c.nptq, c.c2g_q, nullptr));
-  HYD_CHECK(fft_ptospc(c, c.c2g_q, c.qc));
-  return HYD_OK;
+  if (c.nptq == 0) {
+    HYD_CHECK(con2grid(c, c.xq, c.yq, c.nextq, c.nptq, c.c2g_q, nullptr));
+    return fft_ptospc(c, c.c2g_q, c.qc);
+  }
+  HYD_CHECK(con2grid(c, c.xq, c.yq, c.nextq, c.nptq, nullptr, nullptr, true));
+  return fft_ptospc_mean(c, c.c2g_w, c.dsum, (double)c.N, c.qc);
 }
 
 // The field part of inversion: qq, pp and the velocity (evolution.f90:510-520).
 static int inversion_fields(Ctx &c) {
   const int64_t N = c.N;
   {
-    KTimer t(c, "combine_qq", 40 * N);
-    hipLaunchKernelGGL(k_combine_qq, grid1(N), dim3(256), 0, c.s, c.qq, c.flo, c.qs, c.qc, c.qd, N);
-    hipLaunchKernelGGL(k_green, grid1(N), dim3(256), 0, c.s, c.pp, c.green, c.qq, N);
+    KTimer t(c, "combine_qq", 56 * N);
+    hipLaunchKernelGGL(k_combine_green, grid1(N), dim3(256), 0, c.s, c.qq, c.pp, c.flo, c.green,
+                       c.qs, c.qc, c.qd, N);
   }
   HYD_HIP(hipGetLastError());
   // main_invert: vv = spctop(xderiv(pp)), uu = -spctop(yderiv(pp)), one batch
