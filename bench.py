@@ -69,10 +69,10 @@ def copy_calibration(h):
 def pmc_summary(fetch_dir, write_dir, ng, out_json):
     """HBM traffic of the FFT kernels from two rocprofv3 --pmc runs
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the same
-    counter on the calibration copy, as bytes per algorithmic byte.  A
-    dispatch's algorithmic bytes are (sizeof in + sizeof out) x n x vectors;
-    the vectors follow from its grid and its LDS size (2 V n values of the
-    compute type, V vectors per workgroup of 256)."""
+    counter on the calibration copy, as bytes per algorithmic byte
+    (aggregated over the batch variants VT).  A dispatch's algorithmic bytes
+    are (sizeof in + sizeof out) x n x vectors; the vectors are its blocks of
+    256 threads times the VT in the kernel's name."""
     import csv
     import glob
     import re
@@ -95,15 +95,16 @@ def pmc_summary(fetch_dir, write_dir, ng, out_json):
             if "k_copy_calib" in k:
                 copy.append(v)
                 continue
-            m = re.search(r"fft_kernel<(\d+), (true|false), (\w+), (\w+), (\w+)>", k)
+            m = re.search(r"fft_kernel<(\d+), (true|false), (\w+), (\w+), (\w+), (\d+)>", k)
             if not m:
                 continue
             n = int(m.group(1))
-            ti, t, to = (size[m.group(i)] for i in (3, 4, 5))
-            V = max(1, int(row["LDS_Block_Size"]) // (t * 2 * n))
-            vectors = int(row["Grid_Size"]) // 256 * V
+            ti, to = size[m.group(3)], size[m.group(5)]
+            # blocks of 256 threads, VT vectors each (all fields of the launch)
+            vectors = int(row["Grid_Size"]) // 256 * int(m.group(6))
             alg = (ti + to) * n * vectors
-            rec = per.setdefault(m.group(0), [0.0, 0.0, 0])
+            key = "fft_kernel<%s, %s, %s, %s, %s>" % m.group(1, 2, 3, 4, 5)
+            rec = per.setdefault(key, [0.0, 0.0, 0])
             rec[0] += v
             rec[1] += alg
             rec[2] += 1

@@ -30,6 +30,7 @@ namespace hyd {
 // (blockIdx.y selects the field); each field has its own input, output,
 // fused derivative and scaling.
 static constexpr int kMaxF = 4;
+static constexpr int kMaxV = 8;  // vectors per block: 1, 2, 4 or 8
 // Twiddles read from the (L1/L2-resident) global table rather than staged in
 // LDS: the smaller LDS footprint lets more blocks share a CU.
 static constexpr bool kTrigInLds = false;
@@ -292,10 +293,10 @@ __device__ __forceinline__ void rev_r2(const T *__restrict__ a, T *__restrict__ 
 
 // N (the transform length) is a compile-time constant so that the stage
 // loops unroll and every index division becomes a shift or a multiply; the
-// batch V (a power of two) is a run-time shift.  Blocks are mapped so that
-// the blocks one XCD runs hold neighbouring vectors (shared L2 lines on the
-// strided side).
-template <int N, bool FWD, class TI, class T, class TO>
+// batch VT (vectors per block, 1..kMaxV) is one as well.  Blocks are mapped
+// so that the blocks one XCD runs hold neighbouring vectors (shared L2 lines
+// on the strided side).
+template <int N, bool FWD, class TI, class T, class TO, int VT>
 __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *lds = (T *)lds_raw;
@@ -307,7 +308,8 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
   const T *gtrig = (const T *)g.trig;
   constexpr int L2N = __builtin_ctz(N);
   constexpr int n2 = L2N & 1, n4 = L2N >> 1;
-  const int m = g.m, V = g.V, lv = g.lgV;
+  const int m = g.m;
+  constexpr int V = VT, lv = __builtin_ctz(VT);
   const int nb = gridDim.x;
   const int bid = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
   const int v0 = bid * V;
@@ -423,15 +425,15 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
   }
 }
 
-template <int N, class TI, class T, class TO>
-static int launch_n(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftArgs &g,
-                    hipEvent_t ea, hipEvent_t eb) {
+template <int N, class TI, class T, class TO, int VT>
+static int launch_nv(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftArgs &g,
+                     hipEvent_t ea, hipEvent_t eb) {
   // more than 64 KiB of LDS per workgroup must be requested (gfx950: 160 KiB)
   static bool raised = false;
   if (!raised && shm > 65536) {
-    HYD_HIP(hipFuncSetAttribute((const void *)fft_kernel<N, true, TI, T, TO>,
+    HYD_HIP(hipFuncSetAttribute((const void *)fft_kernel<N, true, TI, T, TO, VT>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HYD_HIP(hipFuncSetAttribute((const void *)fft_kernel<N, false, TI, T, TO>,
+    HYD_HIP(hipFuncSetAttribute((const void *)fft_kernel<N, false, TI, T, TO, VT>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     raised = true;
   }
@@ -439,17 +441,29 @@ static int launch_n(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftArg
   // and end; without, an ordinary launch (the one stream capture records)
   if (ea && eb) {
     if (fwd)
-      hipExtLaunchKernelGGL((fft_kernel<N, true, TI, T, TO>), grid, dim3(256), (uint32_t)shm, s,
-                            ea, eb, 0u, g);
+      hipExtLaunchKernelGGL((fft_kernel<N, true, TI, T, TO, VT>), grid, dim3(256), (uint32_t)shm,
+                            s, ea, eb, 0u, g);
     else
-      hipExtLaunchKernelGGL((fft_kernel<N, false, TI, T, TO>), grid, dim3(256), (uint32_t)shm, s,
-                            ea, eb, 0u, g);
+      hipExtLaunchKernelGGL((fft_kernel<N, false, TI, T, TO, VT>), grid, dim3(256), (uint32_t)shm,
+                            s, ea, eb, 0u, g);
   } else if (fwd) {
-    hipLaunchKernelGGL((fft_kernel<N, true, TI, T, TO>), grid, dim3(256), shm, s, g);
+    hipLaunchKernelGGL((fft_kernel<N, true, TI, T, TO, VT>), grid, dim3(256), shm, s, g);
   } else {
-    hipLaunchKernelGGL((fft_kernel<N, false, TI, T, TO>), grid, dim3(256), shm, s, g);
+    hipLaunchKernelGGL((fft_kernel<N, false, TI, T, TO, VT>), grid, dim3(256), shm, s, g);
   }
   return HYD_OK;
+}
+
+template <int N, class TI, class T, class TO>
+static int launch_n(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftArgs &g,
+                    hipEvent_t ea, hipEvent_t eb) {
+  switch (g.V) {
+    case 1: return launch_nv<N, TI, T, TO, 1>(fwd, grid, shm, s, g, ea, eb);
+    case 2: return launch_nv<N, TI, T, TO, 2>(fwd, grid, shm, s, g, ea, eb);
+    case 4: return launch_nv<N, TI, T, TO, 4>(fwd, grid, shm, s, g, ea, eb);
+    case 8: return launch_nv<N, TI, T, TO, 8>(fwd, grid, shm, s, g, ea, eb);
+    default: set_error("fft: unsupported batch"); return HYD_EARG;
+  }
 }
 
 template <class TI, class T, class TO>
@@ -476,11 +490,11 @@ static int fft_launch_t(Ctx &c, bool fwd, int F, const TI *const *in, TO *const 
   g.n = n;
   g.m = m;
   // batch per workgroup: the autotuned value of this layout class when it
-  // divides m, else 4096/n
+  // divides m, else min(4096/n, kMaxV)
   const int cls = fwd ? (out_rows ? 0 : 1) : (in_rows ? 3 : 2);
   int V = c.fft_v[cls];
-  if (V < 1 || V > m || m % V) {
-    V = 4096 / n;
+  if (V < 1 || V > kMaxV || V > m || m % V) {
+    V = std::min(4096 / n, kMaxV);
     if (V < 1) V = 1;
     if (V > m) V = m;
     while (m % V) V /= 2;
@@ -544,7 +558,7 @@ int fft_autotune(Ctx &c) {
   for (int cls = 0; cls < 4 && rc == HYD_OK; cls++) {
     float best = 1e30f;
     int bestV = 0;
-    for (int V = 1; V <= 32 && V <= n; V *= 2) {
+    for (int V = 1; V <= kMaxV && V <= n; V *= 2) {
       if (fft_lds_bytes(n, V, sizeof(double)) > 160 * 1024) break;
       c.fft_v[cls] = V;
       const double *trig = fw[cls] ? c.xtrig : c.ytrig;
