@@ -157,6 +157,9 @@ def main():
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision timing and its error against the FP64 step")
+    ap.add_argument("--err-steps", type=int, default=100)
     ap.add_argument("--precision", type=int, default=0,
                     help="0 plain FP64 (bit-exact), 1 mixed FP32 grid / FP64 state")
     ap.add_argument("--copy-calib", action="store_true",
@@ -277,6 +280,9 @@ def main():
         except Exception as e:  # the baseline is a report, not the product
             log(f"cpu baseline failed: {e}")
     h.close()
+    mixed = None
+    if args.precision == 0 and not args.no_mixed:
+        mixed = mixed_report(qq, ng, args.steps, args.warmup, args.err_steps, value)
     out = {
         "metric": "contour node-updates/s at ng=1024: plain FP64 vs FP32/batched at matched error",
         "value": round(value, 1),
@@ -300,7 +306,67 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if mixed is not None:
+        out["mixed"] = mixed
     print(json.dumps(out), flush=True)
+
+
+def mixed_report(qq, ng, steps, warmup, err_steps, plain_value):
+    """The mixed-precision step (FP32 grids and transforms, FP64 state and
+    nodes) on the same workload: its throughput over the same loop, and its
+    error against the FP64 step after err_steps RK4 steps from the same
+    contours (no surgery in between, so the nodes correspond one to one):
+    RMS and maximum node displacement (periodic, domain units) and the
+    largest spectral PV difference relative to the largest value."""
+    from hydra_amd import Hydra
+    h = Hydra(ng, ncontq=80, precision=1)
+    h.set_field(qq)
+    h.recontour()
+    h.evolution_init()
+
+    def step():
+        while h.advect_step() == 1:
+            h.recontour()
+            h.evolution_init()
+        return h.counts()[0]
+
+    for _ in range(warmup):
+        step()
+    h.sync()
+    t0 = time.perf_counter()
+    updates = 0
+    for _ in range(steps):
+        updates += step()
+    h.sync()
+    wall = time.perf_counter() - t0
+    h.close()
+    runs = []
+    for prec in (0, 1):
+        g = Hydra(ng, ncontq=80, precision=prec)
+        g.set_field(qq)
+        g.recontour()
+        g.evolution_init()
+        for _ in range(err_steps):
+            g.advance()
+        g.sync()
+        runs.append((g.contours(), g.field("qs")))
+        g.close()
+    (c64, qs64), (c32, qs32) = runs
+    same = all(np.array_equal(c64[k], c32[k]) for k in ("nextq", "i1q", "i2q", "npq", "indq"))
+    L = 2 * np.pi
+    d = np.r_[c64["x"] - c32["x"], c64["y"] - c32["y"]]
+    d = d - L * np.round(d / L)
+    log(f"mixed: {steps} steps {wall:.3f}s; after {err_steps} steps rms node displacement "
+        f"{np.sqrt(np.mean(d * d)):.3e}")
+    return {"value": round(updates / wall, 1), "unit": "node-updates/s",
+            "ms_per_step": round(1e3 * wall / steps, 4),
+            "speedup_vs_plain": round(updates / wall / plain_value, 3),
+            "dtype": "f32 grids/f64 nodes and state",
+            "error_vs_fp64": {"steps": err_steps, "topology_equal": bool(same),
+                              "rms_node_displacement": float(np.sqrt(np.mean(d * d))),
+                              "max_node_displacement": float(np.abs(d).max()),
+                              "qs_max_rel_diff": float(np.abs(qs64 - qs32).max()
+                                                       / np.abs(qs64).max())}}
 
 
 def cpu_baseline(h, qq, ng, nsteps):
