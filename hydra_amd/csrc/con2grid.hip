@@ -670,6 +670,84 @@ __global__ void c2g_colscan_int(const int32_t *cnt, const int32_t *qb, float *fq
   }
 }
 
+// Inclusive wavefront scan of ints (row shifts, then row broadcasts).
+template <int CTRL>
+__device__ __forceinline__ int c2g_dpp(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int wave_scan_i32(int v) {
+  const int l = threadIdx.x & 63, rl = l & 15;
+  int t;
+  t = c2g_dpp<0x111>(v); if (rl >= 1) v += t;
+  t = c2g_dpp<0x112>(v); if (rl >= 2) v += t;
+  t = c2g_dpp<0x114>(v); if (rl >= 4) v += t;
+  t = c2g_dpp<0x118>(v); if (rl >= 8) v += t;
+  t = c2g_dpp<0x142>(v); if ((l & 31) >= 16) v += t;
+  t = c2g_dpp<0x143>(v); if (l >= 32) v += t;
+  return v;
+}
+
+// Integer y sweep fused with the first FP32 nine-point average (4ng ->
+// 2ng): the blocking of c2g_colscan_avg9, with each 64-row tile of a column
+// summed by a wavefront scan (integer sums are order free) and converted to
+// (level * qjump) in FP32 as c2g_colscan_int does.
+__global__ void __launch_bounds__(64 * kAvgW)
+    c2g_colscan_int_avg9(const int32_t *__restrict__ cnt, const int32_t *__restrict__ qb,
+                         float *__restrict__ out, int n2, double qjump) {
+  constexpr int TPS = 4;
+  __shared__ float tile[2][kAvgW][64 * TPS + 1];
+  __shared__ float edge[kAvgW][3];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int nh = n2 / 2, ntiles = n2 / 64;
+  const int i0 = blockIdx.x * kAvgK;
+  int col = 2 * i0 - 1 + w;
+  if (col < 0) col += n2;
+  const int32_t *cc = cnt + (int64_t)col * n2;
+  int32_t run = qb[col];
+  const int tid = threadIdx.x, oc = tid >> 7, jr = tid & 127;
+  for (int st = 0; st * TPS < ntiles; st++) {
+    const int t0 = st * TPS, b = st & 1;
+    int32_t wv[TPS];
+#pragma unroll
+    for (int u = 0; u < TPS; u++) {
+      const int r = 64 * (t0 + u) + l;
+      wv[u] = (t0 + u < ntiles && r >= 1) ? level_jump(cc[r]) : 0;
+    }
+    if (l == 0) tile[b][w][0] = (float)((double)run * qjump);  // row 64 t0 - 1 (unused for st = 0)
+#pragma unroll
+    for (int u = 0; u < TPS; u++) {
+      if (t0 + u < ntiles) {
+        const int32_t v = wave_scan_i32(wv[u]);
+        const float f = (float)((double)(run + v) * qjump);
+        tile[b][w][1 + 64 * u + l] = f;
+        if (st == 0 && u == 0 && l < 2) edge[w][l] = f;
+        run += __builtin_amdgcn_readlane(v, 63);
+      }
+    }
+    __syncthreads();
+    const int nrows = 32 * min(TPS, ntiles - t0);
+    if (tid < 128 * kAvgK && jr < nrows && (st > 0 || jr > 0)) {
+      const float(*T)[64 * TPS + 1] = tile[b];
+      const int c0 = 2 * oc, c1 = c0 + 1, c2 = c0 + 2, r0 = 2 * jr, r1 = r0 + 1, r2 = r0 + 2;
+      const float A = T[c2][r2], B = T[c2][r0], C = T[c0][r2], D = T[c0][r0];
+      const float E = T[c1][r2], F = T[c1][r0], G = T[c2][r1], H = T[c0][r1];
+      const float I = T[c1][r1];
+      out[(int64_t)(i0 + oc) * nh + 32 * t0 + jr] =
+          0.0625f * (A + B + C + D) + 0.125f * (E + F + G + H) + 0.25f * I;
+    }
+  }
+  if (l == 0) edge[w][2] = (float)((double)run * qjump);  // row n2 - 1
+  __syncthreads();
+  if (tid < kAvgK) {
+    const int o = tid, c0 = 2 * o, c1 = c0 + 1, c2 = c0 + 2;
+    const float A = edge[c2][1], B = edge[c2][2], C = edge[c0][1], D = edge[c0][2];
+    const float E = edge[c1][1], F = edge[c1][2], G = edge[c2][0], H = edge[c0][0];
+    const float I = edge[c1][0];
+    out[(int64_t)(i0 + o) * nh] =
+        0.0625f * (A + B + C + D) + 0.125f * (E + F + G + H) + 0.25f * I;
+  }
+}
+
 __global__ void c2g_sum_partial(const float *a, int64_t n, double *part) {
   double s = 0.0;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
@@ -746,11 +824,16 @@ int con2grid_mixed(Ctx &c, float *qc) {
   }
   const double qjump = c.hsc->qjump;  // fixed between recontourings
   hipLaunchKernelGGL(c2g_qbot_int, dim3(1), dim3(1024), 0, c.s, c.qjx_cnt, c.qb32, nxf);
-  hipLaunchKernelGGL(c2g_colscan_int, dim3((nxf + 3) / 4), dim3(256), 0, c.s, c.cnt, c.qb32,
-                     c.fq32, nxf, nyf, qjump);
   const int nh1 = nxf / 2, nh2 = k.nx;
-  hipLaunchKernelGGL(c2g_avg9<float>, dim3((nh1 + 255) / 256, nh1), dim3(256), 0, c.s, c.fq32,
-                     c.mq32, nh1);
+  if (colscan_avg9_fits(nxf, nyf)) {
+    hipLaunchKernelGGL(c2g_colscan_int_avg9, dim3(nxf / 2 / kAvgK), dim3(64 * kAvgW), 0, c.s,
+                       c.cnt, c.qb32, c.mq32, nxf, qjump);
+  } else {
+    hipLaunchKernelGGL(c2g_colscan_int, dim3((nxf + 3) / 4), dim3(256), 0, c.s, c.cnt, c.qb32,
+                       c.fq32, nxf, nyf, qjump);
+    hipLaunchKernelGGL(c2g_avg9<float>, dim3((nh1 + 255) / 256, nh1), dim3(256), 0, c.s, c.fq32,
+                       c.mq32, nh1);
+  }
   hipLaunchKernelGGL(c2g_avg9<float>, dim3((nh2 + 255) / 256, nh2), dim3(256), 0, c.s, c.mq32,
                      c.fc32, nh2);
   hipLaunchKernelGGL(c2g_sum_partial, dim3(1024), dim3(256), 0, c.s, c.fc32, N, c.part);
