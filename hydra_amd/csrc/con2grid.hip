@@ -625,6 +625,21 @@ __global__ void c2g_fill_i32(int32_t *a, int64_t n) {
   if (k < n) a[k] = 0;
 }
 
+// Zero n ints (n % 4 == 0, a 16-byte aligned), 16 bytes per store.
+__global__ void c2g_fill_i32x4(int4 *a, int64_t n4) {
+  const int4 z = make_int4(0, 0, 0, 0);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n4;
+       k += (int64_t)gridDim.x * blockDim.x)
+    a[k] = z;
+}
+
+// Zero the bottom-row counters and the per-sweep flags (flags[0..7]).
+__global__ void c2g_reset_small(int32_t *qjx, int nxf, int32_t *flags) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nxf; k += gridDim.x * blockDim.x)
+    qjx[k] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 8) flags[threadIdx.x] = 0;
+}
+
 // qb[ix] = sum_{j < ix} jump(qjx[j]) (one workgroup)
 __global__ void __launch_bounds__(1024) c2g_qbot_int(const int32_t *qjx, int32_t *qb, int nxf) {
   __shared__ int32_t part[1024];
@@ -779,18 +794,22 @@ __global__ void c2g_subtract_mean(const float *qa, float *qc, int64_t n, const d
   if (k < n) qc[k] = (float)((double)qa[k] - *mean);
 }
 
-int con2grid_mixed(Ctx &c, float *qc) {
+// defer_mean: the averaged grid is left in c.fc32 and its mean in
+// c.part[1024]; qc is not written (the forward FFT removes the mean).
+int con2grid_mixed(Ctx &c, float *qc, bool defer_mean) {
   const Consts &k = c.c;
   const int64_t N = c.N;
   const int n = c.nptq;
   const int nxf = k.nxf, nyf = k.nyf;
   const int64_t nfine = (int64_t)nxf * nyf;
   KTimer tm(c, "con2grid32", (int64_t)n * 24 + 4 * N);
-  hipLaunchKernelGGL(c2g_fill_i32, dim3((unsigned)((nfine + 255) / 256)), dim3(256), 0, c.s,
-                     c.cnt, nfine);
-  hipLaunchKernelGGL(c2g_fill_i32, dim3((nxf + 255) / 256), dim3(256), 0, c.s, c.qjx_cnt,
-                     (int64_t)nxf);
-  hipLaunchKernelGGL(c2g_fill_i32, dim3(1), dim3(64), 0, c.s, c.flags, (int64_t)8);
+  if (nfine % 4 == 0)
+    hipLaunchKernelGGL(c2g_fill_i32x4, dim3(4096), dim3(256), 0, c.s, (int4 *)c.cnt, nfine / 4);
+  else
+    hipLaunchKernelGGL(c2g_fill_i32, dim3((unsigned)((nfine + 255) / 256)), dim3(256), 0, c.s,
+                       c.cnt, nfine);
+  hipLaunchKernelGGL(c2g_reset_small, dim3((nxf + 255) / 256), dim3(256), 0, c.s, c.qjx_cnt, nxf,
+                     c.flags);
   if (n > 0) {
     C2GArgs a;
     a.xq = c.xq;
@@ -838,6 +857,10 @@ int con2grid_mixed(Ctx &c, float *qc) {
                      c.fc32, nh2);
   hipLaunchKernelGGL(c2g_sum_partial, dim3(1024), dim3(256), 0, c.s, c.fc32, N, c.part);
   hipLaunchKernelGGL(c2g_mean, dim3(1), dim3(1024), 0, c.s, c.part, (double)N);
+  if (defer_mean) {
+    HYD_HIP(hipGetLastError());
+    return HYD_OK;
+  }
   hipLaunchKernelGGL(c2g_subtract_mean, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.s,
                      c.fc32, qc, N, c.part + 1024);
   HYD_HIP(hipGetLastError());

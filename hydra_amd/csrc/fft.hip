@@ -538,9 +538,10 @@ static int fft_launch(Ctx &c, bool fwd, const double *in, double *out, int n, in
 
 template <class TI, class T, class TO>
 static int fft_launch1(Ctx &c, bool fwd, const TI *in, TO *out, int n, int m, int in_rows,
-                       int out_rows, int pre, const double *dk, double post, const T *trig) {
+                       int out_rows, int pre, const double *dk, double post, const T *trig,
+                       const double *sub = nullptr, double subdiv = 1.0) {
   return fft_launch_t<TI, T, TO>(c, fwd, 1, &in, &out, n, m, in_rows, out_rows, &pre, &dk, &post,
-                                 trig);
+                                 trig, sub, subdiv);
 }
 
 // Autotuner: for each layout class (forward strided->rows, forward
@@ -629,6 +630,17 @@ int fft_spctop(Ctx &c, const double *svar, double *rvar, int pre, double post_si
 // Mixed-precision transforms (FP32 arithmetic, FP64 spectral state):
 // ptospc from an FP32 physical field to FP64 spectral coefficients, spctop
 // from FP64 coefficients (optionally differentiated) to an FP32 field.
+// ptospc32 of rvar - *mean (the subtraction done in double while loading,
+// as con2grid_mixed's mean removal).
+int fft_ptospc32_mean(Ctx &c, const float *rvar, const double *mean, double *svar) {
+  const int n = c.c.nx;
+  HYD_CHECK((fft_launch1<float, float, float>(c, true, rvar, c.tmp32, n, n, 0, 1, 0, nullptr,
+                                                c.plan.fnorm, c.xtrig32, mean, 1.0)));
+  HYD_CHECK((fft_launch1<float, float, double>(c, true, c.tmp32, svar, n, n, 0, 0, 0, nullptr,
+                                                 c.plan.fnorm, c.ytrig32)));
+  return HYD_OK;
+}
+
 int fft_ptospc32(Ctx &c, const float *rvar, double *svar) {
   const int n = c.c.nx;
   HYD_CHECK((fft_launch1<float, float, float>(c, true, rvar, c.tmp32, n, n, 0, 1, 0, nullptr,

@@ -192,7 +192,8 @@ int fft_spctop32_n(Ctx &c, int F, const double *const *svar, float *const *rvar,
 int fft_ptospc32_n(Ctx &c, int F, const float *const *rvar, double *const *svar,
                    float *const *tmp);
 int fft_spctop32(Ctx &c, const double *svar, float *rvar, int pre, double post_sign);
-int con2grid_mixed(Ctx &c, float *qc);
+int con2grid_mixed(Ctx &c, float *qc, bool defer_mean = false);
+int fft_ptospc32_mean(Ctx &c, const float *rvar, const double *mean, double *svar);
 int advance_mixed(Ctx &c);
 int fft_autotune(Ctx &c);
 int fft_reverse_1d(Ctx &c, double *x, int m);   // revfft(m, ng, x)

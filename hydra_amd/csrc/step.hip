@@ -617,9 +617,8 @@ static int source_mixed(Ctx &c, int lev) {
 }
 
 static int contour_grid_mixed(Ctx &c) {
-  HYD_CHECK(con2grid_mixed(c, c.c2g_q32));
-  HYD_CHECK(fft_ptospc32(c, c.c2g_q32, c.qc));
-  return HYD_OK;
+  HYD_CHECK(con2grid_mixed(c, nullptr, true));
+  return fft_ptospc32_mean(c, c.fc32, c.part + 1024, c.qc);
 }
 
 static int inversion_fields_mixed(Ctx &c) {
