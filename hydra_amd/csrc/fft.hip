@@ -31,6 +31,9 @@ namespace hyd {
 // fused derivative and scaling.
 static constexpr int kMaxF = 4;
 static constexpr int kMaxV = 8;  // vectors per block: 1, 2, 4 or 8
+// threads per block of every FFT launch (a constant, so that the item loops
+// of a stage have compile-time trip counts)
+static constexpr int kFftThreads = 256;
 // Twiddles read from the (L1/L2-resident) global table rather than staged in
 // LDS: the smaller LDS footprint lets more blocks share a CU.
 static constexpr bool kTrigInLds = false;
@@ -68,7 +71,7 @@ __device__ __forceinline__ void fwd_r2(const T *__restrict__ a, T *__restrict__ 
   const int nh = (lv - 1) / 2;
 #define A(j, k) av[irem + rem * ((j) + 2 * (k))]
 #define B(k, j) bv[irem + rem * ((k) + lv * (j))]
-  for (int it = threadIdx.x; it < V * rem; it += blockDim.x) {
+  for (int it = threadIdx.x; it < V * rem; it += kFftThreads) {
     const int v = it / rem, irem = it - v * rem;
     const T *av = a + v * n;
     T *bv = b + v * n;
@@ -77,7 +80,7 @@ __device__ __forceinline__ void fwd_r2(const T *__restrict__ a, T *__restrict__ 
     B(0, 1) = a00 - a10;
   }
   const int per = rem * nh;
-  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+  for (int it = threadIdx.x; it < V * per; it += kFftThreads) {
     const int v = it / per, r = it - v * per;
     const int kq = r / rem, irem = r - kq * rem;
     const T *av = a + v * n;
@@ -106,7 +109,7 @@ __device__ __forceinline__ void fwd_r4(const T *__restrict__ a, T *__restrict__ 
   const int even = (lv & 1) == 0;
 #define A(j, k) av[irem + rem * ((j) + 4 * (k))]
 #define B(k, j) bv[irem + rem * ((k) + lv * (j))]
-  for (int it = threadIdx.x; it < V * rem; it += blockDim.x) {
+  for (int it = threadIdx.x; it < V * rem; it += kFftThreads) {
     const int v = it / rem, irem = it - v * rem;
     const T *av = a + v * n;
     T *bv = b + v * n;
@@ -119,7 +122,7 @@ __device__ __forceinline__ void fwd_r4(const T *__restrict__ a, T *__restrict__ 
     B(0, 3) = a3 - a1;
   }
   const int per = rem * nh;
-  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+  for (int it = threadIdx.x; it < V * per; it += kFftThreads) {
     const int v = it / per, r = it - v * per;
     const int kq = r / rem, irem = r - kq * rem;
     const T *av = a + v * n;
@@ -155,7 +158,7 @@ __device__ __forceinline__ void fwd_r4(const T *__restrict__ a, T *__restrict__ 
   }
   if (even) {
     const int l2 = lv / 2;
-    for (int it = threadIdx.x; it < V * rem; it += blockDim.x) {
+    for (int it = threadIdx.x; it < V * rem; it += kFftThreads) {
       const int v = it / rem, irem = it - v * rem;
       const T *av = a + v * n;
       T *bv = b + v * n;
@@ -185,7 +188,7 @@ __device__ __forceinline__ void rev_r4(const T *__restrict__ a, T *__restrict__ 
   const int even = (lv & 1) == 0;
 #define A(k, j) av[ic + cum * ((k) + lv * (j))]
 #define B(j, k) bv[ic + cum * ((j) + 4 * (k))]
-  for (int it = threadIdx.x; it < V * cum; it += blockDim.x) {
+  for (int it = threadIdx.x; it < V * cum; it += kFftThreads) {
     const int v = it / cum, ic = it - v * cum;
     const T *av = a + v * n;
     T *bv = b + v * n;
@@ -200,7 +203,7 @@ __device__ __forceinline__ void rev_r4(const T *__restrict__ a, T *__restrict__ 
     B(3, 0) = t3r - t4r;
   }
   const int per = cum * nh;
-  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+  for (int it = threadIdx.x; it < V * per; it += kFftThreads) {
     const int v = it / per, r = it - v * per;
     const int kq = r / cum, ic = r - kq * cum;
     const T *av = a + v * n;
@@ -236,7 +239,7 @@ __device__ __forceinline__ void rev_r4(const T *__restrict__ a, T *__restrict__ 
   }
   if (even) {
     const int l2 = lv / 2;
-    for (int it = threadIdx.x; it < V * cum; it += blockDim.x) {
+    for (int it = threadIdx.x; it < V * cum; it += kFftThreads) {
       const int v = it / cum, ic = it - v * cum;
       const T *av = a + v * n;
       T *bv = b + v * n;
@@ -261,7 +264,7 @@ __device__ __forceinline__ void rev_r2(const T *__restrict__ a, T *__restrict__ 
   const int nh = (lv - 1) / 2;
 #define A(k, j) av[ic + cum * ((k) + lv * (j))]
 #define B(j, k) bv[ic + cum * ((j) + 2 * (k))]
-  for (int it = threadIdx.x; it < V * cum; it += blockDim.x) {
+  for (int it = threadIdx.x; it < V * cum; it += kFftThreads) {
     const int v = it / cum, ic = it - v * cum;
     const T *av = a + v * n;
     T *bv = b + v * n;
@@ -270,7 +273,7 @@ __device__ __forceinline__ void rev_r2(const T *__restrict__ a, T *__restrict__ 
     B(1, 0) = a0 - a1;
   }
   const int per = cum * nh;
-  for (int it = threadIdx.x; it < V * per; it += blockDim.x) {
+  for (int it = threadIdx.x; it < V * per; it += kFftThreads) {
     const int v = it / per, r = it - v * per;
     const int kq = r / cum, ic = r - kq * cum;
     const T *av = a + v * n;
@@ -297,7 +300,7 @@ __device__ __forceinline__ void rev_r2(const T *__restrict__ a, T *__restrict__ 
 // so that the blocks one XCD runs hold neighbouring vectors (shared L2 lines
 // on the strided side).
 template <int N, bool FWD, class TI, class T, class TO, int VT>
-__global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
+__global__ void __launch_bounds__(kFftThreads) fft_kernel(FftArgs g) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *lds = (T *)lds_raw;
   const int f = blockIdx.y;
@@ -318,9 +321,9 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
   const int tot = V * N;
   constexpr int nhalf = N / 2;
   if (kTrigInLds)
-    for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) lds[2 * V * N + i] = gtrig[i];
+    for (int i = threadIdx.x; i < 2 * N; i += kFftThreads) lds[2 * V * N + i] = gtrig[i];
   // ---- load
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < tot; idx += kFftThreads) {
     int v, e;
     if (g.in_rows) {
       v = idx / N;
@@ -411,7 +414,7 @@ __global__ void __launch_bounds__(256) fft_kernel(FftArgs g) {
   }
   // ---- store (normalisation folded in, stafft.f90:275-284 / 394-403)
   const T post = (T)g.post[f];
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < tot; idx += kFftThreads) {
     int v, e;
     if (g.out_rows) {
       v = idx / N;
@@ -441,15 +444,15 @@ static int launch_nv(bool fwd, dim3 grid, size_t shm, hipStream_t s, const FftAr
   // and end; without, an ordinary launch (the one stream capture records)
   if (ea && eb) {
     if (fwd)
-      hipExtLaunchKernelGGL((fft_kernel<N, true, TI, T, TO, VT>), grid, dim3(256), (uint32_t)shm,
+      hipExtLaunchKernelGGL((fft_kernel<N, true, TI, T, TO, VT>), grid, dim3(kFftThreads), (uint32_t)shm,
                             s, ea, eb, 0u, g);
     else
-      hipExtLaunchKernelGGL((fft_kernel<N, false, TI, T, TO, VT>), grid, dim3(256), (uint32_t)shm,
+      hipExtLaunchKernelGGL((fft_kernel<N, false, TI, T, TO, VT>), grid, dim3(kFftThreads), (uint32_t)shm,
                             s, ea, eb, 0u, g);
   } else if (fwd) {
-    hipLaunchKernelGGL((fft_kernel<N, true, TI, T, TO, VT>), grid, dim3(256), shm, s, g);
+    hipLaunchKernelGGL((fft_kernel<N, true, TI, T, TO, VT>), grid, dim3(kFftThreads), shm, s, g);
   } else {
-    hipLaunchKernelGGL((fft_kernel<N, false, TI, T, TO, VT>), grid, dim3(256), shm, s, g);
+    hipLaunchKernelGGL((fft_kernel<N, false, TI, T, TO, VT>), grid, dim3(kFftThreads), shm, s, g);
   }
   return HYD_OK;
 }
