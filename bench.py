@@ -160,6 +160,11 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision timing and its error against the FP64 step")
     ap.add_argument("--err-steps", type=int, default=100)
+    ap.add_argument("--ensemble", type=int, default=0, metavar="FLOWS",
+                    help="also time an ensemble of FLOWS independent mixed-precision flows "
+                         "(ng from {256,512,1024}, rms in [2pi,6pi], 40-120 levels, seeded) "
+                         "run together (hyd_run_batch) and one after another")
+    ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--precision", type=int, default=0,
                     help="0 plain FP64 (bit-exact), 1 mixed FP32 grid / FP64 state")
     ap.add_argument("--copy-calib", action="store_true",
@@ -308,6 +313,8 @@ def main():
     }
     if mixed is not None:
         out["mixed"] = mixed
+    if args.ensemble > 0:
+        out["ensemble"] = ensemble_report(args.ensemble, args.ensemble_steps, args.seed)
     print(json.dumps(out), flush=True)
 
 
@@ -367,6 +374,56 @@ def mixed_report(qq, ng, steps, warmup, err_steps, plain_value):
                               "max_node_displacement": float(np.abs(d).max()),
                               "qs_max_rel_diff": float(np.abs(qs64 - qs32).max()
                                                        / np.abs(qs64).max())}}
+
+
+def ensemble_report(nflows, steps, seed):
+    """Aggregate node-updates/s of an ensemble of independent mixed-precision
+    flows of uneven sizes: all advanced together by hyd_run_batch, and the
+    same flows advanced one after another with hyd_run.  Both start each
+    flow with a recontouring (hyd_run's loop), included in the time.  Nodes
+    per step are taken as the mean of each flow's counts before and after."""
+    from hydra_amd import Hydra
+    from hydra_amd.ic import random_pv
+    rng = np.random.default_rng(seed + 1)
+    spec = [(int(rng.choice([256, 512, 1024])), float(rng.uniform(2 * np.pi, 6 * np.pi)),
+             int(rng.integers(40, 121)), int(rng.integers(1 << 30))) for _ in range(nflows)]
+
+    def make():
+        hs = []
+        for ng, rms, lev, sd in spec:
+            h = Hydra(ng, ncontq=lev, precision=1, node_capacity=max(8 * ng * ng, 1 << 20))
+            h.set_field(random_pv(ng, seed=sd, rms=rms))
+            hs.append(h)
+        return hs
+
+    def timed(run):
+        hs = make()
+        run(hs, 2)  # warm-up: first contours, graph capture
+        for h in hs:
+            h.sync()
+        p0 = [h.counts()[0] for h in hs]
+        t0 = time.perf_counter()
+        run(hs, steps)
+        for h in hs:
+            h.sync()
+        wall = time.perf_counter() - t0
+        p1 = [h.counts()[0] for h in hs]
+        for h in hs:
+            h.close()
+        return sum(steps * (a + b) / 2 for a, b in zip(p0, p1)) / wall, wall, p1
+
+    def one_by_one(hs, k):
+        for h in hs:
+            h.run(k)
+
+    v_batch, w_batch, nodes = timed(lambda hs, k: Hydra.run_batch(hs, k))
+    v_seq, w_seq, _ = timed(one_by_one)
+    log(f"ensemble: {nflows} flows, {steps} steps: batched {w_batch:.3f}s, one by one {w_seq:.3f}s")
+    return {"flows": [{"ng": ng, "rms": round(rms, 3), "levels": lev, "nodes": p}
+                      for (ng, rms, lev, _), p in zip(spec, nodes)],
+            "steps": steps, "value_batched": round(v_batch, 1),
+            "value_one_by_one": round(v_seq, 1), "unit": "node-updates/s",
+            "precision": "mixed-fp32-grid"}
 
 
 def cpu_baseline(h, qq, ng, nsteps):

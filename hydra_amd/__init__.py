@@ -90,6 +90,7 @@ def lib():
         L.hyd_renode.argtypes = [C.c_void_p, _D, _D, C.c_int32, _D, _D, C.c_int32,
                                  C.POINTER(C.c_int32)]
         L.hyd_run.argtypes = [C.c_void_p, C.c_int32]
+        L.hyd_run_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int32]
         L.hyd_get_tuning.argtypes = [C.c_void_p, _I]
         L.hyd_copy_calibration.argtypes = [C.c_void_p, C.c_int64]
         L.hyd_set_timing.argtypes = [C.c_void_p, C.c_int32]
@@ -280,6 +281,13 @@ class Hydra:
     def run(self, nsteps):
         """caps.f90:83-91 for nsteps advances (recontour, init, advect)."""
         _check(self.L.hyd_run(self.h, int(nsteps)))
+
+    @staticmethod
+    def run_batch(flows, nsteps):
+        """hyd_run for each flow of an ensemble (distinct Hydra objects),
+        their steps running concurrently; same results as flow.run(nsteps)."""
+        arr = (C.c_void_p * len(flows))(*[f.h for f in flows])
+        _check(lib().hyd_run_batch(arr, len(flows), int(nsteps)))
 
     def tuning(self):
         """FFT batch per layout class picked by the autotuner at init."""

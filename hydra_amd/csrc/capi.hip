@@ -791,6 +791,76 @@ int hyd_run(hyd_ctx *h, int32_t nsteps) {
   return HYD_OK;
 }
 
+// An ensemble of independent flows, each running hyd_run's loop for nsteps
+// advances.  Rounds: every flow that needs it recontours / regularises /
+// does surgery (host-orchestrated, one flow at a time), then the RK4 steps
+// of all flows are enqueued on their own streams and run concurrently (a
+// mixed-precision flow replays its graph), then one wait per flow reads its
+// scalars back.  Each flow's sequence of operations, hence its result, is
+// the one hyd_run would produce.
+int hyd_run_batch(hyd_ctx **hs, int32_t n, int32_t nsteps) {
+  if (!hs || n < 0 || nsteps < 0) {
+    set_error("run_batch: bad arguments");
+    return HYD_EARG;
+  }
+  for (int i = 0; i < n; i++) {
+    if (!hs[i] || !C(hs[i])->have_field) {
+      set_error("run_batch: every flow needs a context with a field");
+      return HYD_ESTATE;
+    }
+    for (int j = 0; j < i; j++)
+      if (hs[j] == hs[i]) {
+        set_error("run_batch: a context appears twice");
+        return HYD_EARG;
+      }
+  }
+  const double twistmax = 2.5;
+  const int nregmax = 20;
+  std::vector<int> done(n, 0), start(n, 1), stepped(n, 0);
+  for (;;) {
+    bool any = false;
+    for (int i = 0; i < n; i++) {
+      stepped[i] = 0;
+      if (done[i] >= nsteps) continue;
+      any = true;
+      Ctx &c = *C(hs[i]);
+      if (start[i]) {
+        HYD_CHECK(recontour(c, c.qr));
+        HYD_CHECK(advect_start(c));
+        start[i] = 0;
+      }
+      if (c.hsc->twist > twistmax) {
+        c.ireg = c.ireg + 1;
+        if (c.ireg == nregmax) {
+          HYD_CHECK(prepare(c));
+          HYD_HIP(hipStreamSynchronize(c.s));
+          start[i] = 1;  // recontour next round; this round has no advance
+          continue;
+        }
+        HYD_CHECK(surgery(c));
+        HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+        HYD_HIP(hipStreamSynchronize(c.s));
+        c.hsc->twist = c.hsc->twist - twistmax;
+        HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
+      }
+      if (c.p.precision == 1)
+        HYD_CHECK(advance_mixed_graph(c));
+      else
+        HYD_CHECK(advance(c));
+      HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+      stepped[i] = 1;
+    }
+    if (!any) break;
+    for (int i = 0; i < n; i++) {
+      if (!stepped[i]) continue;
+      HYD_CHECK(c2g_status(*C(hs[i])));  // waits for the flow's stream
+      done[i]++;
+    }
+  }
+  for (int i = 0; i < n; i++) HYD_HIP(hipStreamSynchronize(C(hs[i])->s));
+  return HYD_OK;
+}
+
 __global__ void k_copy_calib(const double *__restrict__ a, double *__restrict__ b, int64_t n) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x)

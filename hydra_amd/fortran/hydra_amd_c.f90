@@ -203,6 +203,13 @@ interface
     type(c_ptr), value :: ctx
     integer(c_int32_t), value :: nsteps
   end function
+
+  ! ensemble of independent flows (array of contexts)
+  integer(c_int) function hyd_run_batch(ctxs, n, nsteps) bind(c, name="hyd_run_batch")
+    import :: c_int, c_ptr, c_int32_t
+    type(c_ptr), intent(in) :: ctxs(*)
+    integer(c_int32_t), value :: n, nsteps
+  end function
 end interface
 
 contains

@@ -162,6 +162,14 @@ int hyd_advect_step(hyd_ctx *ctx);
    nsteps advances have been taken. */
 int hyd_run(hyd_ctx *ctx, int32_t nsteps);
 
+/* An ensemble of n independent flows (distinct contexts, any sizes and
+ * precisions), each advanced by hyd_run's loop for nsteps steps.  The flows'
+ * RK4 steps of one round run concurrently on their own streams; each flow's
+ * result equals hyd_run on that flow alone.  No reference counterpart: the
+ * batched ensemble of many flows (one flow per program in the reference,
+ * caps.f90). */
+int hyd_run_batch(hyd_ctx **ctxs, int32_t n, int32_t nsteps);
+
 /* Synchronise the context's stream (timing helpers). */
 int hyd_sync(hyd_ctx *ctx);
 /* Kernel-time accounting of the dominant kernels, measured with HIP events
