@@ -97,6 +97,20 @@ def lib():
         L.hyd_kernel_times.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
                                        np.ctypeslib.ndpointer(dtype=np.float64),
                                        np.ctypeslib.ndpointer(dtype=np.int64)]
+        L.hyd_cnn_create.argtypes = [C.c_int32, C.c_double, C.c_uint64, C.c_int32,
+                                     C.POINTER(C.c_void_p)]
+        L.hyd_cnn_destroy.argtypes = [C.c_void_p]
+        _F = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.hyd_cnn_forward.argtypes = [C.c_void_p, _F, _D, _D, C.c_int32, _F, _F]
+        L.hyd_cnn_gradients.argtypes = [C.c_void_p, _F, _D, _D, C.c_int32, _F, _F, _F,
+                                        C.POINTER(C.c_float)]
+        L.hyd_cnn_train_step.argtypes = [C.c_void_p, _F, _D, _D, C.c_int32, _F, _F, C.c_float,
+                                         C.POINTER(C.c_float)]
+        L.hyd_cnn_num_weights.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.hyd_cnn_get_weights.argtypes = [C.c_void_p, _F]
+        L.hyd_cnn_set_weights.argtypes = [C.c_void_p, _F]
+        _U16 = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+        L.hyd_cnn_mfma_probe.argtypes = [_U16, _U16, _F]
         _lib = L
     return _lib
 
@@ -315,3 +329,76 @@ class Hydra:
         raw = names.raw
         return [(raw[32 * i:32 * i + 32].split(b"\0")[0].decode(), ms[i], int(by[i]))
                 for i in range(n)]
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class VelocityCNN:
+    """The learned sub-grid velocity correction (hyd_cnn_*): a 4-layer 3x3
+    CNN on the n x n coarse grid (inputs u, v, q), output (du, dv) sampled
+    at node positions; BF16 matrix-core kernels, FP32 accumulation, Adam."""
+
+    def __init__(self, n, ell=2 * np.pi, seed=1, device=0):
+        self.L = lib()
+        self.n = n
+        h = C.c_void_p()
+        _check(self.L.hyd_cnn_create(int(n), float(ell), int(seed), int(device), C.byref(h)))
+        self.h = h
+        nw = C.c_int64()
+        _check(self.L.hyd_cnn_num_weights(self.h, C.byref(nw)))
+        self.nw = nw.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hyd_cnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _in(self, in3):
+        a = _f32(in3)
+        if a.shape != (self.n * self.n, 3):
+            a = a.reshape(self.n * self.n, 3)
+        return a
+
+    def forward(self, in3, xn, yn):
+        m = len(xn)
+        du, dv = np.empty(m, np.float32), np.empty(m, np.float32)
+        _check(self.L.hyd_cnn_forward(self.h, self._in(in3), _f64(xn), _f64(yn), m, du, dv))
+        return du, dv
+
+    def gradients(self, in3, xn, yn, tu, tv):
+        g = np.empty(self.nw, np.float32)
+        loss = C.c_float()
+        _check(self.L.hyd_cnn_gradients(self.h, self._in(in3), _f64(xn), _f64(yn), len(xn),
+                                        _f32(tu), _f32(tv), g, C.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, in3, xn, yn, tu, tv, lr=1e-3):
+        loss = C.c_float()
+        _check(self.L.hyd_cnn_train_step(self.h, self._in(in3), _f64(xn), _f64(yn), len(xn),
+                                         _f32(tu), _f32(tv), float(lr), C.byref(loss)))
+        return loss.value
+
+    def weights(self):
+        w = np.empty(self.nw, np.float32)
+        _check(self.L.hyd_cnn_get_weights(self.h, w))
+        return w
+
+    def set_weights(self, w):
+        _check(self.L.hyd_cnn_set_weights(self.h, _f32(w)))
+
+
+def mfma_probe(a_bf16, b_bf16):
+    """D = A B of one 16x16x16 BF16 matrix-core block (A row-major,
+    b_bf16[n][k]); returns D (16x16 FP32)."""
+    d = np.empty(256, np.float32)
+    _check(lib().hyd_cnn_mfma_probe(np.ascontiguousarray(a_bf16, np.uint16).ravel(),
+                                    np.ascontiguousarray(b_bf16, np.uint16).ravel(), d))
+    return d.reshape(16, 16)

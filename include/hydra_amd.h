@@ -190,6 +190,34 @@ int hyd_copy_calibration(hyd_ctx *ctx, int64_t nbytes);
 int hyd_kernel_times(hyd_ctx *ctx, int32_t max, char *names, double *ms, int64_t *bytes);
 int hyd_set_timing(hyd_ctx *ctx, int32_t level);
 
+/* ---- learned velocity correction (no reference counterpart) ------------
+ * A 4-layer 3x3 convolutional network (inputs u, v, q on an n x n doubly
+ * periodic grid of side ell, 32 channels, GELU; BF16 weights and
+ * activations on the matrix cores with FP32 accumulation) whose 2-channel
+ * output (du, dv) is sampled bilinearly at node positions: a sub-grid
+ * correction to velint's velocities.  Plain bilinear velint stays the
+ * fallback.  Inputs in3 are FP32 [ix*n + iy][3] (channel fastest). */
+typedef struct hyd_cnn hyd_cnn;
+int hyd_cnn_create(int32_t n, double ell, uint64_t seed, int32_t device, hyd_cnn **out);
+int hyd_cnn_destroy(hyd_cnn *cnn);
+/* corrections at m nodes (host arrays) */
+int hyd_cnn_forward(hyd_cnn *cnn, const float *in3, const double *xn, const double *yn,
+                    int32_t m, float *du, float *dv);
+/* loss mean_i |c_i - t_i|^2 and its gradient w.r.t. every weight (FP32,
+ * layer by layer: W[co][ci][tap] then bias[co]) */
+int hyd_cnn_gradients(hyd_cnn *cnn, const float *in3, const double *xn, const double *yn,
+                      int32_t m, const float *tu, const float *tv, float *grad, float *loss);
+/* one Adam step (lr) on that loss; *loss is the loss before the step */
+int hyd_cnn_train_step(hyd_cnn *cnn, const float *in3, const double *xn, const double *yn,
+                       int32_t m, const float *tu, const float *tv, float lr, float *loss);
+int hyd_cnn_num_weights(hyd_cnn *cnn, int64_t *nw);
+int hyd_cnn_get_weights(hyd_cnn *cnn, float *w);
+int hyd_cnn_set_weights(hyd_cnn *cnn, const float *w);
+/* One 16x16x16 BF16 matrix-core product D = A B (A row-major, B column
+ * by column, D row-major; 256 BF16 / 256 FP32 values): the operand layout
+ * check of the network's kernels. */
+int hyd_cnn_mfma_probe(const uint16_t *a, const uint16_t *b, float *d);
+
 #ifdef __cplusplus
 }
 #endif
