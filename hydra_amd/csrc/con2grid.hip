@@ -90,6 +90,7 @@ __device__ __forceinline__ int64_t bk_find(const Buckets &b, int32_t key) {
 }
 
 // Node phase: contours.f90:206-231 and the crossing loop 249-269.
+static constexpr int kCrossBatch = 4;
 __global__ void c2g_nodes(C2GArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
@@ -124,15 +125,41 @@ __global__ void c2g_nodes(C2GArgs a) {
     // contours.f90:251-253 vs congen.f90:398-400
     const int ixbeg = a.ultra ? ixci + (1 + jump) / 2 + a.nxf : ixci + (jump - 1) / 2 + a.nxf;
     const int sgn = a.ultra ? jump : (copysign(1.0, dx) > 0.0 ? 1 : -1);
-    for (int ncr = 0; ncr != nxc; ncr += jump) {
-      const int ix = 1 + (ixbeg + ncr) % a.nxf;
-      const double x2 = a.xgf[ix - 1] - xi;
-      const double px0 = (x2 - a.ellx * (double)(int)(x2 * a.hlxi)) / dx;
-      const double y2 = yi + px0 * dy;
-      const int iy = 2 + (int)(a.glyfi * (y2 - a.elly * (double)(int)(y2 * a.hlyi) - a.ymin));
-      // row nyf+1 is never swept (contours.f90:274): its jumps are dropped
-      if (iy >= 2 && iy <= a.nyf)
-        add_count(&a.cnt[(int64_t)(ix - 1) * a.nyf + (iy - 1)], sgn, &a.flags[0]);
+    // crossings in groups of kCrossBatch: the cells of a group are found
+    // first and their counter atomics issued together, so a node's
+    // crossings cost one atomic round trip per group, not one each
+    const int32_t d = sgn > 0 ? 1 : (1 << 16);
+    int ncr = 0;
+    while (ncr != nxc) {
+      int64_t cell[kCrossBatch];
+      int ixs[kCrossBatch];
+      double xg[kCrossBatch];
+      const int nb = min(kCrossBatch, jump * (nxc - ncr));
+#pragma unroll
+      for (int b = 0; b < kCrossBatch; b++) {
+        ixs[b] = 1 + (ixbeg + ncr + jump * b) % a.nxf;
+        xg[b] = b < nb ? a.xgf[ixs[b] - 1] : 0.0;
+      }
+#pragma unroll
+      for (int b = 0; b < kCrossBatch; b++) {
+        cell[b] = -1;
+        if (b < nb) {
+          const int ix = ixs[b];
+          const double x2 = xg[b] - xi;
+          const double px0 = (x2 - a.ellx * (double)(int)(x2 * a.hlxi)) / dx;
+          const double y2 = yi + px0 * dy;
+          const int iy = 2 + (int)(a.glyfi * (y2 - a.elly * (double)(int)(y2 * a.hlyi) - a.ymin));
+          // row nyf+1 is never swept (contours.f90:274): its jumps are dropped
+          if (iy >= 2 && iy <= a.nyf) cell[b] = (int64_t)(ix - 1) * a.nyf + (iy - 1);
+        }
+      }
+      ncr += jump * nb;
+      int32_t old[kCrossBatch];
+#pragma unroll
+      for (int b = 0; b < kCrossBatch; b++) old[b] = cell[b] >= 0 ? atomicAdd(&a.cnt[cell[b]], d) : 0;
+#pragma unroll
+      for (int b = 0; b < kCrossBatch; b++)
+        if (cell[b] >= 0 && !is_flagged(old[b]) && is_flagged(old[b] + d)) atomicAdd(&a.flags[0], 1);
     }
   }
 }

@@ -320,12 +320,64 @@ __global__ void __launch_bounds__(kFftThreads) fft_kernel(FftArgs g) {
   const T *tcs = kTrigInLds ? lds + 2 * V * N : gtrig, *tsn = tcs + N;
   const int tot = V * N;
   constexpr int nhalf = N / 2;
-  if (kTrigInLds)
-    for (int i = threadIdx.x; i < 2 * N; i += kFftThreads) lds[2 * V * N + i] = gtrig[i];
-  // ---- load
-  for (int idx = threadIdx.x; idx < tot; idx += kFftThreads) {
+  // ---- load.  Every value a thread needs is fetched before any is used
+  // (branch-free, from clamped in-bounds addresses), so each thread has all
+  // of its global loads in flight at once.
+  constexpr int PER = (V * N + kFftThreads - 1) / kFftThreads;  // values per thread
+  constexpr bool FULL = (V * N) % kFftThreads == 0;
+  const int tid = threadIdx.x;
+  TI raw[PER];
+  double dkr[PER];
+  const bool rows_in = g.in_rows != 0;
+  const int mh = m / 2;
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const int idx = tid + kFftThreads * j;
+    const bool live = FULL || idx < tot;
     int v, e;
-    if (g.in_rows) {
+    if (rows_in) {
+      v = idx / N;
+      e = idx - v * N;
+    } else {
+      e = idx >> lv;
+      v = idx & (V - 1);
+    }
+    const int vg = v0 + v;
+    // the source element: mirrored for a fused derivative (sta2dfft.f90:169-245)
+    int vs = vg, es = e;
+    if (!FWD && gpre == 2) es = (e == 0) ? 0 : N - e;
+    if (!FWD && gpre == 1) vs = (vg == 0) ? 0 : m - vg;
+    raw[j] = live ? (rows_in ? gin[(int64_t)vs * N + es] : gin[vs + (int64_t)m * es]) : (TI)0;
+    // the derivative's wavenumber, filtered (index 0 where the result is 0)
+    if (!FWD && gpre == 2) {
+      const int kk = (e < nhalf) ? e : N - e;
+      dkr[j] = gdk[(e == 0 || e == nhalf) ? 0 : 2 * kk - 1];
+    } else if (!FWD && gpre == 1) {
+      const int kk = (vg < mh) ? vg : m - vg;
+      dkr[j] = gdk[(vg == 0 || vg == mh) ? 0 : 2 * kk - 1];
+    }
+  }
+  if (kTrigInLds) {
+    constexpr int TPER = (2 * N + kFftThreads - 1) / kFftThreads;
+    T tr[TPER];
+#pragma unroll
+    for (int j = 0; j < TPER; j++) {
+      const int i = tid + kFftThreads * j;
+      tr[j] = (2 * N % kFftThreads == 0 || i < 2 * N) ? gtrig[i] : (T)0;
+    }
+#pragma unroll
+    for (int j = 0; j < TPER; j++) {
+      const int i = tid + kFftThreads * j;
+      if (2 * N % kFftThreads == 0 || i < 2 * N) lds[2 * V * N + i] = tr[j];
+    }
+  }
+  const double subv = (FWD && g.sub) ? *g.sub / g.subdiv : 0.0;
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const int idx = tid + kFftThreads * j;
+    if (!(FULL || idx < tot)) continue;
+    int v, e;
+    if (rows_in) {
       v = idx / N;
       e = idx - v * N;
     } else {
@@ -335,32 +387,21 @@ __global__ void __launch_bounds__(kFftThreads) fft_kernel(FftArgs g) {
     const int vg = v0 + v;
     T x;
     if (FWD) {
-      const TI raw = g.in_rows ? gin[(int64_t)vg * N + e] : gin[vg + (int64_t)m * e];
       // con2grid's mean removal (contours.f90:338-345) fused into the load
-      x = g.sub ? (T)(raw - *g.sub / g.subdiv) : (T)raw;
+      x = g.sub ? (T)(raw[j] - subv) : (T)raw[j];
     } else {
-      // optional fused derivative (sta2dfft.f90:169-245)
       if (gpre == 2) {
         // along elements: der(e) from var(n-e)
-        if (e == 0 || e == nhalf) {
-          x = 0.0;
-        } else {
-          const int em = N - e;
-          const T var = (T)g.in_rows ? gin[(int64_t)vg * N + em] : gin[vg + (int64_t)m * em];
-          x = (e < nhalf) ? -(T)gdk[2 * e - 1] * var : (T)gdk[2 * em - 1] * var;
-        }
+        const T dkv = (T)dkr[j];
+        const T var = (T)raw[j];
+        x = (e == 0 || e == nhalf) ? (T)0.0 : ((e < nhalf) ? -dkv * var : dkv * var);
       } else if (gpre == 1) {
         // across vectors (vector index is the derivative's wavenumber, m = its length)
-        const int mh = m / 2;
-        if (vg == 0 || vg == mh) {
-          x = 0.0;
-        } else {
-          const int vm = m - vg;
-          const T var = (T)g.in_rows ? gin[(int64_t)vm * N + e] : gin[vm + (int64_t)m * e];
-          x = (vg < mh) ? -(T)gdk[2 * vg - 1] * var : (T)gdk[2 * vm - 1] * var;
-        }
+        const T dkv = (T)dkr[j];
+        const T var = (T)raw[j];
+        x = (vg == 0 || vg == mh) ? (T)0.0 : ((vg < mh) ? -dkv * var : dkv * var);
       } else {
-        x = (T)(g.in_rows ? gin[(int64_t)vg * N + e] : gin[vg + (int64_t)m * e]);
+        x = (T)raw[j];
       }
       // revfft pre-processing, stafft.f90:311-325
       if (e >= nhalf + 1) x = -x;
@@ -414,9 +455,13 @@ __global__ void __launch_bounds__(kFftThreads) fft_kernel(FftArgs g) {
   }
   // ---- store (normalisation folded in, stafft.f90:275-284 / 394-403)
   const T post = (T)g.post[f];
-  for (int idx = threadIdx.x; idx < tot; idx += kFftThreads) {
+  const bool rows_out = g.out_rows != 0;
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const int idx = tid + kFftThreads * j;
+    if (!(FULL || idx < tot)) continue;
     int v, e;
-    if (g.out_rows) {
+    if (rows_out) {
       v = idx / N;
       e = idx - v * N;
       gout[(int64_t)(v0 + v) * N + e] = (TO)(a[v * N + e] * post);
