@@ -71,6 +71,9 @@ def lib():
             getattr(L, fn).argtypes = [C.c_void_p, _D, _D]
         L.hyd_main_invert.argtypes = [C.c_void_p, _D, _D, _D, _D]
         L.hyd_gradient.argtypes = [C.c_void_p, _D, _D, _D]
+        L.hyd_spec1d.argtypes = [C.c_void_p, _D, _D, C.c_int32]
+        L.hyd_norms.argtypes = [C.c_void_p, _D, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.hyd_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.hyd_get_table.argtypes = [C.c_void_p, C.c_int32, _D]
         L.hyd_velint.argtypes = [C.c_void_p, _D, _D, _D, _D, C.c_int32, _D, _D]
         L.hyd_con2grid.argtypes = [C.c_void_p, _D, _D, _I, C.c_int32, C.c_double, _D, C.c_void_p]
@@ -180,6 +183,27 @@ class Hydra:
 
     def gradient(self, ff):
         return self._grid_op(self.L.hyd_gradient, ff, 2)
+
+    # -- diagnostics ---------------------------------------------------------
+    def spec1d(self, ss, iopt=0):
+        """spectral.f90:370-424: (spec[0..max(nx,ny)], ss after the call);
+        entries above kmax are 0 here."""
+        ss = _f64(ss, (self.ng, self.ng)).copy()
+        spec = np.zeros(self.ng + 1)
+        _check(self.L.hyd_spec1d(self.h, ss, spec, int(iopt)))
+        return spec, ss
+
+    def norms(self, qq):
+        """generic.f90:30-64: (l1norm, l2norm) of a physical field"""
+        a, b = C.c_double(), C.c_double()
+        _check(self.L.hyd_norms(self.h, _f64(qq, (self.ng, self.ng)), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def energy(self):
+        """evolution.f90:612-644: (ekin, epot) from the current uu, vv, pp"""
+        a, b = C.c_double(), C.c_double()
+        _check(self.L.hyd_energy(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def table(self, name):
         n = self.ng

@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HYDRA_AMD_ARCH", "gfx950")
 
 SOURCES = ["tables.cpp", "fft.hip", "con2grid.hip", "seqsum.hip", "nodes.hip", "step.hip",
-           "renode.hip", "congen.hip", "surgery.hip", "capi.hip", "cnn.hip"]
+           "renode.hip", "congen.hip", "surgery.hip", "capi.hip", "cnn.hip", "diag.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

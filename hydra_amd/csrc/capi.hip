@@ -109,6 +109,7 @@ static void free_ctx(Ctx *c) {
   for (hipEvent_t e : {c->ev_fork, c->ev_join})
     if (e) hipEventDestroy(e);
   free_congen(*c);
+  diag_free(*c);
   for (auto &r : c->krec) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
@@ -370,6 +371,42 @@ int hyd_main_invert(hyd_ctx *h, const double *qq, double *uu, double *vv, double
   HYD_CHECK(d2h(c, uu, c.wb, c.N));
   HYD_CHECK(d2h(c, vv, c.wc, c.N));
   return d2h(c, pp, c.wd, c.N);
+}
+
+int hyd_spec1d(hyd_ctx *h, double *ss, double *spec, int32_t iopt) {
+  Ctx &c = *C(h);
+  if (!ss || !spec || (iopt != 0 && iopt != 1)) {
+    set_error("spec1d: null array or iopt not 0/1");
+    return HYD_EARG;
+  }
+  HYD_CHECK(h2d(c, c.wb, ss, c.N));
+  HYD_CHECK(diag_spec1d(c, c.wb, c.wc, iopt));
+  HYD_CHECK(d2h(c, spec, c.wc, c.ht.kmax + 1));
+  if (iopt == 1) HYD_CHECK(d2h(c, ss, c.wb, c.N));
+  return HYD_OK;
+}
+
+int hyd_norms(hyd_ctx *h, const double *qq, double *ql1, double *ql2) {
+  Ctx &c = *C(h);
+  if (!qq || !ql1 || !ql2) {
+    set_error("norms: null argument");
+    return HYD_EARG;
+  }
+  HYD_CHECK(h2d(c, c.wb, qq, c.N));
+  return diag_norms(c, c.wb, ql1, ql2);
+}
+
+int hyd_energy(hyd_ctx *h, double *ekin, double *epot) {
+  Ctx &c = *C(h);
+  if (!ekin || !epot) {
+    set_error("energy: null argument");
+    return HYD_EARG;
+  }
+  if (!c.have_field || c.p.precision != 0) {
+    set_error("energy: needs a field and the plain FP64 fields (precision 0)");
+    return HYD_ESTATE;
+  }
+  return diag_energy(c, ekin, epot);
 }
 
 int hyd_gradient(hyd_ctx *h, const double *ff, double *ffx, double *ffy) {

@@ -68,6 +68,11 @@ struct HostTables {
   // host-evaluated integrating factor, evolution.f90:603-605)
   std::vector<double> qdiss_u;
   std::vector<int32_t> qdiss_map;
+  // spectrum shells, spectral.f90:110-133: kmag (nx,ny), kmax, kmaxred,
+  // spmf and alk (0..max(nx,ny))
+  std::vector<int32_t> kmag;
+  int kmax = 0, kmaxred = 0;
+  std::vector<double> spmf, alk;
 };
 
 int build_consts(const hyd_params &p, Consts &c);
@@ -159,6 +164,11 @@ struct Ctx {
   double *exp_host = nullptr;    // pinned: host-evaluated exp table of the step
   hipEvent_t ev_sc = nullptr;    // adapt's scalars have reached the host mirror
   // second stream of the plain step (default priority) and its fork/join events
+  // diagnostics (diag.hip): spectrum shells in the reference's mode order
+  int32_t *shell_ptr = nullptr;  // kmax + 2
+  int32_t *shell_ent = nullptr;  // N: mode index | weight code << 29
+  double *rksq = nullptr;        // (nx,ny), spec1d's iopt = 1
+  double *diag_out = nullptr;    // 8 scratch results
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void *seq_work = nullptr; // chunk table of seqsum
@@ -204,6 +214,11 @@ int launch_xderiv(Ctx &c, const double *var, double *der);
 int launch_yderiv(Ctx &c, const double *var, double *der);
 
 int seqsum(Ctx &c, const double *a, int64_t n, double *dout);
+// diagnostics (diag.hip)
+int diag_energy(Ctx &c, double *ekin, double *epot);
+int diag_norms(Ctx &c, const double *qq, double *ql1, double *ql2);
+int diag_spec1d(Ctx &c, double *ss, double *spec, int iopt);
+void diag_free(Ctx &c);
 int64_t seqsum_work_bytes(int64_t n);
 int64_t seqsum_work_chunks(int64_t n);
 

@@ -187,6 +187,26 @@ int build_tables(const hyd_params &p, const Consts &c, HostTables &t) {
       t.fhi[k] = 1.0 - t.flo[k];
       t.rksq[k] = rkxf[kx] * rkxf[kx] + rkyf[ky] * rkyf[ky];
     }
+  // spectral.f90:110-133 (spectrum shells)
+  const int nm = std::max(nx, ny);
+  const double delk = sqrt(scx * scx + scy * scy), delki = 1.0 / delk;
+  t.kmax = (int)lround(sqrt(a2 + b2) * delki);
+  t.kmaxred = (int)lround(sqrt((a2 + b2) / 2.0) * delki);
+  t.kmag.assign(N, 0);
+  t.spmf.assign((size_t)nm + 1, 0.0);
+  t.alk.assign((size_t)nm + 1, 0.0);
+  for (int ky = 0; ky < ny; ky++)
+    for (int kx = 0; kx < nx; kx++) {
+      const int k = (int)lround(sqrt(t.rkx[kx] * t.rkx[kx] + t.rky[ky] * t.rky[ky]) * delki);
+      t.kmag[(size_t)ky * nx + kx] = k;
+      t.spmf[k] = t.spmf[k] + 1.0;
+    }
+  const double snorm = 4.0 * kPi / (double)(nx * ny);
+  t.spmf[0] = 0.0;
+  for (int k = 1; k <= t.kmax; k++) {
+    t.spmf[k] = snorm * (double)k / t.spmf[k];
+    t.alk[k] = log10(delk * (double)k);
+  }
   // spectral.f90:136-147
   t.green[0] = 0.0;
   for (int kx = 2; kx <= nx; kx++) t.green[kx - 1] = -1.0 / (t.rkx[kx - 1] * t.rkx[kx - 1] + c.kdsq);

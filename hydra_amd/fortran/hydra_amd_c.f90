@@ -82,6 +82,30 @@ interface
     real(c_double), intent(out) :: ffx(*), ffy(*)
   end function
 
+  ! spectral.f90:370-424 (ss modified when iopt = 1)
+  integer(c_int) function hyd_spec1d(ctx, ss, spec, iopt) bind(c, name="hyd_spec1d")
+    import :: c_int, c_int32_t, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(inout) :: ss(*)
+    real(c_double), intent(inout) :: spec(*)
+    integer(c_int32_t), value :: iopt
+  end function
+
+  ! generic.f90:30-64
+  integer(c_int) function hyd_norms(ctx, qq, ql1, ql2) bind(c, name="hyd_norms")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(in) :: qq(*)
+    real(c_double), intent(out) :: ql1, ql2
+  end function
+
+  ! evolution.f90:612-644
+  integer(c_int) function hyd_energy(ctx, ekin, epot) bind(c, name="hyd_energy")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(out) :: ekin, epot
+  end function
+
   ! contours.f90:136-169
   integer(c_int) function hyd_velint(ctx, uu, vv, xq, yq, nptq, uq, vq) &
       bind(c, name="hyd_velint")

@@ -84,6 +84,23 @@ int hyd_yderiv(hyd_ctx *ctx, const double *var, double *der);
 int hyd_main_invert(hyd_ctx *ctx, const double *qq, double *uu, double *vv, double *pp);
 /* gradient(ff,ffx,ffy) spectral.f90:345-366 */
 int hyd_gradient(hyd_ctx *ctx, const double *ff, double *ffx, double *ffy);
+/* spec1d(ss,spec,iopt) spectral.f90:370-424: the 1D spectrum of the
+   spectral field ss (nx,ny), each shell summed in the reference's mode
+   order; spec[0..kmax] written (spec holds max(nx,ny)+1 values, the rest
+   untouched); iopt = 1 first multiplies ss by rksq and returns it so. */
+int hyd_spec1d(hyd_ctx *ctx, double *ss, double *spec, int32_t iopt);
+
+/* ---- ca/plane/qg/caps/generic.f90 ---------------------------------------- */
+/* l1norm(qq,ql1) generic.f90:30-46 and l2norm(qq,ql2) generic.f90:48-64 of a
+   physical field qq (ny,nx): garea*sum(|qq|), garea*sum(qq**2), the sums
+   left to right as the reference's. */
+int hyd_norms(hyd_ctx *ctx, const double *qq, double *ql1, double *ql2);
+
+/* ---- ca/plane/qg/caps/evolution.f90 (diagnostics) ------------------------ */
+/* energy(ekin,epot) evolution.f90:612-644 from the context's uu, vv, pp as
+   left by the last inversion (plain precision only: HYD_ESTATE otherwise). */
+int hyd_energy(hyd_ctx *ctx, double *ekin, double *epot);
+
 /* Spectral tables of init_spectral (spectral.f90:14-22): id 1 green, 2 flo,
    3 fhi, 4 qdiss, 5 rksq (ng*ng each); 6 hrkx, 7 rkx, 8 hrky, 9 rky (ng);
    10 xtrig, 11 ytrig (2*ng). */

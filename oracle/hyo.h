@@ -129,6 +129,8 @@ void hyo_adapt(hyo_model *m);
 void hyo_source(hyo_model *m, double *sqs, double *sqd, int lev);
 int hyo_advance(hyo_model *m);
 void hyo_energy(const hyo_model *m, double *ekin, double *epot);
+double hyo_l1norm(const hyo_model *m, const double *qq);
+double hyo_l2norm(const hyo_model *m, const double *qq);
 /* One pass of the advect loop body (evolution.f90:55-80): surgery when
    twist>twistmax, then advance.  Returns 1 when nregmax regularisations were
    reached and prepare() has been called (caller must recontour). */

@@ -84,6 +84,10 @@ def lib():
         L.hyo_advect_step.argtypes = [C.c_void_p]
         L.hyo_run.argtypes = [C.c_void_p, C.c_int]
         L.hyo_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.hyo_spec1d.argtypes = [C.c_void_p, _D, _D, C.c_int]
+        for fn in ("hyo_l1norm", "hyo_l2norm"):
+            getattr(L, fn).argtypes = [C.c_void_p, _D]
+            getattr(L, fn).restype = C.c_double
         _lib = L
     return _lib
 
@@ -243,6 +247,17 @@ class Oracle:
         self.L.hyo_energy(self.m, C.byref(a), C.byref(b))
         return a.value, b.value
 
+    def spec1d(self, ss, iopt=0):
+        """(spec[0..max(nx,ny)], ss after the call); entries above kmax stay 0."""
+        ss = np.ascontiguousarray(ss, dtype=np.float64).copy()
+        spec = np.zeros(max(ss.shape) + 1)
+        self.L.hyo_spec1d(self.m, ss, spec, int(iopt))
+        return spec, ss
+
+    def norms(self, qq):
+        qq = np.ascontiguousarray(qq, dtype=np.float64)
+        return self.L.hyo_l1norm(self.m, qq), self.L.hyo_l2norm(self.m, qq)
+
 
 # ---------------------------------------------------------------------------
 # The reference itself (amdflang build of its modules + ref_shim.f90).
@@ -392,6 +407,26 @@ class Ref:
         fx, fy = np.empty_like(ff), np.empty_like(ff)
         run_big_stack(self.L.ref_gradient, ff, fx, fy)
         return fx, fy
+
+    def spec1d(self, ss, iopt=0):
+        """As Oracle.spec1d; entries above kmax are left 0."""
+        self.L.ref_spec1d.argtypes = [_D, _D, C.c_int]
+        ss = np.ascontiguousarray(ss, dtype=np.float64).copy()
+        spec = np.zeros(max(ss.shape) + 1)
+        run_big_stack(self.L.ref_spec1d, ss, spec, int(iopt))
+        return spec, ss
+
+    def norms(self, qq):
+        self.L.ref_norms.argtypes = [_D, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        qq = np.ascontiguousarray(qq, dtype=np.float64).copy()
+        a, b = C.c_double(), C.c_double()
+        run_big_stack(self.L.ref_norms, qq, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def kmax(self):
+        k = C.c_int()
+        self.L.ref_kmax(C.byref(k))
+        return k.value
 
     def recontour(self, qq):
         run_big_stack(self.L.ref_recontour, np.ascontiguousarray(qq, dtype=np.float64).copy())

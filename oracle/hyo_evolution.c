@@ -307,6 +307,26 @@ void hyo_energy(const hyo_model *m, double *ekin, double *epot) {
   free(zz);
 }
 
+/* generic.f90:30-46 and 48-64: garea * sum(|qq|) and garea * sum(qq**2),
+ * sequential sums over the (ny,nx) field in storage order. */
+double hyo_l1norm(const hyo_model *m, const double *qq) {
+  const long N = (long)m->nx * m->ny;
+  double *z = (double *)malloc(sizeof(double) * N);
+  for (long k = 0; k < N; k++) z[k] = fabs(qq[k]);
+  const double r = m->garea * hyo_sum_seq(z, N);
+  free(z);
+  return r;
+}
+
+double hyo_l2norm(const hyo_model *m, const double *qq) {
+  const long N = (long)m->nx * m->ny;
+  double *z = (double *)malloc(sizeof(double) * N);
+  for (long k = 0; k < N; k++) z[k] = qq[k] * qq[k];
+  const double r = m->garea * hyo_sum_seq(z, N);
+  free(z);
+  return r;
+}
+
 /* evolution.f90:55-80, one iteration of the time loop. */
 int hyo_advect_step(hyo_model *m) {
   const double twistmax = 2.5;

@@ -15,6 +15,7 @@ use constants
 use contours
 use spectral
 use congen
+use generic
 
 implicit none
 
@@ -172,6 +173,25 @@ subroutine ref_consts(out) bind(c, name='ref_consts')
   out(10) = oms
   out(11) = small3
   out(12) = kdsq
+end subroutine
+
+! spectral.f90:370-424 (ss is modified when iopt = 1)
+subroutine ref_spec1d(ss, spec, iopt) bind(c, name='ref_spec1d')
+  real(c_double):: ss(nx,ny), spec(0:max(nx,ny))
+  integer(c_int), value:: iopt
+  call spec1d(ss, spec, iopt)
+end subroutine
+
+! generic.f90:30-64
+subroutine ref_norms(qq, ql1, ql2) bind(c, name='ref_norms')
+  real(c_double):: qq(ny,nx), ql1, ql2
+  call l1norm(qq, ql1)
+  call l2norm(qq, ql2)
+end subroutine
+
+subroutine ref_kmax(k) bind(c, name='ref_kmax')
+  integer(c_int):: k
+  k = kmax
 end subroutine
 
 ! congen.f90 recontour: qqr is the residual (or the full field when there are

@@ -51,6 +51,11 @@ def main():
                       (9, "hrky"), (11, "xtrig")]:
         out["tab_" + name] = r.table(tid)
     out["d4small"] = np.float64(r.consts()[2])
+    # diagnostics: spec1d spectral.f90:370-424, l1norm/l2norm generic.f90:30-64
+    out["spec_kmax"] = np.int64(r.kmax())
+    for iopt in (0, 1):
+        out[f"spec{iopt}"], out[f"spec{iopt}_ss"] = r.spec1d(s, iopt)
+    out["norm_l1"], out["norm_l2"] = (np.float64(v) for v in r.norms(f))
     # renode of the first three contours, contours.f90:687-933
     for j in range(3):
         i1, i2 = c["i1q"][j], c["i2q"][j]
