@@ -36,7 +36,7 @@ FIELDS = {"qs": 0, "qd": 1, "qc": 2, "qq": 3, "pp": 4, "uu": 5, "vv": 6, "qr": 7
 SCALARS = {"t": 0, "dt": 1, "twist": 2, "zzrms": 3, "zzmax": 4, "uumax": 5, "qjump": 6,
            "dtmax": 7}
 TABLES = {"green": 1, "flo": 2, "fhi": 3, "qdiss": 4, "rksq": 5, "hrkx": 6, "rkx": 7, "hrky": 8,
-          "rky": 9, "xtrig": 10, "ytrig": 11}
+          "rky": 9, "xtrig": 10, "ytrig": 11, "spmf": 12, "alk": 13, "shells": 14, "bety": 15}
 
 
 class HydraError(RuntimeError):
@@ -207,8 +207,8 @@ class Hydra:
 
     def table(self, name):
         n = self.ng
-        size = {1: n * n, 2: n * n, 3: n * n, 4: n * n, 5: n * n, 10: 2 * n, 11: 2 * n}.get(
-            TABLES[name], n)
+        size = {1: n * n, 2: n * n, 3: n * n, 4: n * n, 5: n * n, 10: 2 * n, 11: 2 * n,
+                12: n + 1, 13: n + 1, 14: 3}.get(TABLES[name], n)
         out = np.empty(size)
         _check(self.L.hyd_get_table(self.h, TABLES[name], out))
         return out

@@ -433,6 +433,14 @@ int hyd_get_table(hyd_ctx *h, int32_t id, double *out) {
     case 9: v = &t.rky; break;
     case 10: v = &t.xtrig; break;
     case 11: v = &t.ytrig; break;
+    case 12: v = &t.spmf; break;
+    case 13: v = &t.alk; break;
+    case 14:
+      out[0] = (double)t.kmax;
+      out[1] = (double)t.kmaxred;
+      out[2] = c.c.dsumi;
+      return HYD_OK;
+    case 15: v = &t.bety; break;
     default: set_error("unknown table id"); return HYD_EARG;
   }
   memcpy(out, v->data(), sizeof(double) * v->size());

@@ -1,0 +1,238 @@
+"""The reference's files, read and written byte for byte (host code).
+
+Formats (all little-endian; gfortran/amdflang conventions: direct-access
+records have no markers and RECL counts bytes; unformatted sequential
+records are framed by a 4-byte byte count before and after):
+
+* ``qq_init.r8`` -- direct access, RECL = 2*nbytes = 8 (N+1): record 1 holds
+  t (FP64) then the PV field qq(ny,nx) (FP64) (caps.f90:150-153,
+  constants.f90:19).
+* ``qq.r4`` and ``cont/qqresi.r4`` -- direct access, RECL = nbytes = 4 (N+1):
+  record r holds t (real*4) then a field (ny,nx) in real*4 (evolution.f90:
+  703-704 and 753-756, read back by common.f90:68-72).
+* ``cont/qqsynopsis.asc`` -- one line per contour save,
+  '(i9,1x,i10,1x,f12.5,1x,f16.12)' nq, nptq, t, qjump (evolution.f90:750).
+* ``cont/qqindexNNN`` -- one sequential record: npq, i1q, indq (int32, nq
+  each) (evolution.f90:760-762); ``cont/qqnodesNNN`` -- one record: xq, yq
+  (FP64, nptq each) (evolution.f90:764-766).
+* ``ene.asc`` '(f9.2,3(1x,f17.11))' t, ekin, epot, ekin+epot; ``norms.asc``
+  '(f9.2,1x,f16.9)' t, <q^2>; ``spectra.asc`` a '(f9.2,2(1x,f16.9),1x,i5)'
+  header t, sum, <q^2>, kmaxred then kmaxred lines '(2(1x,f12.8))' log10 k,
+  log10 spectrum (evolution.f90:676-728).
+
+Fields are numpy arrays of shape (ng, ng) whose C order is the reference's
+column-major (ny,nx) (index [ix, iy]), as everywhere in hydra_amd.
+
+``readcont`` / ``restart`` follow common.f90:30-100: the contour table is
+rebuilt from npq and i1q (i2q = i1q + npq - 1, nextq closing each contour on
+itself), the residual is widened from real*4, and the PV field is
+qs = ptospc(con2grid(contours) + qr), computed by the library.
+"""
+import os
+
+import numpy as np
+
+
+# ---- Fortran record primitives ------------------------------------------------
+
+def _direct_write(path, rec, recl, payload):
+    """Write record rec (1-based) of a direct-access file of record length
+    recl bytes (the file is extended as needed)."""
+    if len(payload) > recl:
+        raise ValueError("record larger than RECL")
+    mode = "r+b" if os.path.exists(path) else "w+b"
+    with open(path, mode) as f:
+        f.seek((rec - 1) * recl)
+        f.write(payload + b"\0" * (recl - len(payload)))
+
+
+def _direct_read(path, rec, recl):
+    with open(path, "rb") as f:
+        f.seek((rec - 1) * recl)
+        b = f.read(recl)
+    if len(b) != recl:
+        raise ValueError(f"{path}: record {rec} missing")
+    return b
+
+
+def write_record(f, *arrays):
+    """One unformatted sequential record holding the arrays in order."""
+    payload = b"".join(np.ascontiguousarray(a).tobytes() for a in arrays)
+    n = np.array([len(payload)], dtype="<i4").tobytes()
+    f.write(n + payload + n)
+
+
+def read_record(f):
+    head = f.read(4)
+    if len(head) != 4:
+        raise ValueError("end of file")
+    n = int(np.frombuffer(head, dtype="<i4")[0])
+    payload = f.read(n)
+    tail = f.read(4)
+    if len(payload) != n or tail != head:
+        raise ValueError("corrupt sequential record")
+    return payload
+
+
+# ---- fields --------------------------------------------------------------------
+
+def write_qq_init(path, t, qq):
+    """qq_init.r8 (caps.f90:150-153)."""
+    qq = np.ascontiguousarray(qq, dtype="<f8")
+    N = qq.size
+    _direct_write(path, 1, 8 * (N + 1), np.array([t], "<f8").tobytes() + qq.tobytes())
+
+
+def read_qq_init(path, ng):
+    N = ng * ng
+    b = _direct_read(path, 1, 8 * (N + 1))
+    a = np.frombuffer(b, dtype="<f8")
+    return float(a[0]), a[1:].reshape(ng, ng).copy()
+
+
+def write_r4_record(path, rec, t, field):
+    """Record rec of qq.r4 / qqresi.r4: real(t) then real(field)."""
+    f = np.ascontiguousarray(field, dtype="<f4")
+    _direct_write(path, rec, 4 * (f.size + 1), np.array([t], "<f4").tobytes() + f.tobytes())
+
+
+def read_r4_record(path, rec, ng):
+    b = _direct_read(path, rec, 4 * (ng * ng + 1))
+    a = np.frombuffer(b, dtype="<f4")
+    return float(a[0]), a[1:].reshape(ng, ng).copy()
+
+
+# ---- formatted lines -------------------------------------------------------------
+
+def _f(x, w, d):
+    s = f"{x:{w}.{d}f}"
+    return s if len(s) <= w else "*" * w
+
+
+def _i(n, w):
+    s = f"{n:{w}d}"
+    return s if len(s) <= w else "*" * w
+
+
+def fmt_synopsis(nq, nptq, t, qjump):
+    return f"{_i(nq, 9)} {_i(nptq, 10)} {_f(t, 12, 5)} {_f(qjump, 16, 12)}"
+
+
+def fmt_ene(t, ekin, epot):
+    return f"{_f(t, 9, 2)} {_f(ekin, 17, 11)} {_f(epot, 17, 11)} {_f(ekin + epot, 17, 11)}"
+
+
+def fmt_norms(t, ql2):
+    return f"{_f(t, 9, 2)} {_f(ql2, 16, 9)}"
+
+
+def fmt_spectra_head(t, sumqspec, ql2, kmaxred):
+    return f"{_f(t, 9, 2)} {_f(sumqspec, 16, 9)} {_f(ql2, 16, 9)} {_i(kmaxred, 5)}"
+
+
+def fmt_spectra_line(alk, lspec):
+    return f" {_f(alk, 12, 8)} {_f(lspec, 12, 8)}"
+
+
+# ---- contours ---------------------------------------------------------------------
+
+def savecont(outdir, icrec, t, contours, residual):
+    """evolution.f90:735-778 for save icrec (1-based): synopsis line,
+    residual record, and (when there are contours) index and node files.
+    residual is qr = spctop(qq - qc) in physical space."""
+    cdir = os.path.join(outdir, "cont")
+    os.makedirs(cdir, exist_ok=True)
+    nq, nptq = len(contours["i1q"]), len(contours["x"])
+    with open(os.path.join(cdir, "qqsynopsis.asc"), "a") as f:
+        f.write(fmt_synopsis(nq, nptq, t, contours["qjump"]) + "\n")
+    write_r4_record(os.path.join(cdir, "qqresi.r4"), icrec, t, residual)
+    if nq > 0:
+        pind = f"{icrec:03d}"
+        with open(os.path.join(cdir, "qqindex" + pind), "wb") as f:
+            write_record(f, *(np.asarray(contours[k], dtype="<i4") for k in ("npq", "i1q", "indq")))
+        with open(os.path.join(cdir, "qqnodes" + pind), "wb") as f:
+            write_record(f, np.asarray(contours["x"], dtype="<f8"),
+                         np.asarray(contours["y"], dtype="<f8"))
+
+
+def readcont(outdir, iind, ng):
+    """common.f90:30-72: the contours, qjump, t and the FP64-widened residual
+    of save iind."""
+    cdir = os.path.join(outdir, "cont")
+    with open(os.path.join(cdir, "qqsynopsis.asc")) as f:
+        lines = f.read().splitlines()
+    if len(lines) < iind:
+        raise ValueError(f"qqsynopsis.asc has {len(lines)} saves, not {iind}")
+    w = lines[iind - 1].split()
+    nq, nptq, t, qjump = int(w[0]), int(w[1]), float(w[2]), float(w[3])
+    pind = f"{iind:03d}"
+    npq = i1q = indq = np.zeros(0, np.int32)
+    x = y = np.zeros(0)
+    if nq > 0:
+        with open(os.path.join(cdir, "qqindex" + pind), "rb") as f:
+            a = np.frombuffer(read_record(f), dtype="<i4")
+        npq, i1q, indq = (a[k * nq:(k + 1) * nq].astype(np.int32) for k in range(3))
+        with open(os.path.join(cdir, "qqnodes" + pind), "rb") as f:
+            a = np.frombuffer(read_record(f), dtype="<f8")
+        x, y = a[:nptq].copy(), a[nptq:2 * nptq].copy()
+    _, qr4 = read_r4_record(os.path.join(cdir, "qqresi.r4"), iind, ng)
+    # common.f90:78-88: i2q and nextq from npq and i1q
+    i2q = (i1q + npq - 1).astype(np.int32)
+    nextq = np.zeros(nptq, np.int32)
+    for j in range(nq):
+        b, e = int(i1q[j]), int(i2q[j])
+        nextq[b - 1:e - 1] = np.arange(b + 1, e + 1, dtype=np.int32)
+        nextq[e - 1] = b
+    c = dict(x=x, y=y, nextq=nextq, i1q=i1q.astype(np.int32), i2q=i2q, npq=npq.astype(np.int32),
+             indq=indq.astype(np.int32), qjump=qjump)
+    return c, qr4.astype(np.float64), t
+
+
+def restart(h, outdir, iind):
+    """caps.f90:185-189 + common.f90:30-100 on a Hydra context: loads the
+    contours, residual and time of save iind and sets qs = ptospc(con2grid +
+    qr).  Returns t."""
+    c, qr, t = readcont(outdir, iind, h.ng)
+    qa = h.con2grid(c["x"], c["y"], c["nextq"], c["qjump"]) if len(c["x"]) else np.zeros((h.ng, h.ng))
+    qa = qa + qr
+    h.set_field(qa, t)           # qs = ptospc(qa)
+    h.put_field("qr", qr)        # the residual, not the full field
+    h.set_contours(c)            # contours and qjump of the save
+    return t
+
+
+# ---- grid saves ---------------------------------------------------------------------
+
+def savegrid(h, outdir, igrec):
+    """evolution.f90:646-731 at the current time (no interpolation between
+    steps: the save time is the state's own t): appends to ene.asc,
+    norms.asc and spectra.asc and writes record igrec+1 of qq.r4.  Needs the
+    plain precision (energy reads the FP64 fields)."""
+    os.makedirs(outdir, exist_ok=True)
+    t = h.scalar("t")
+    qqs = h.field("qq")
+    ekin, epot = h.energy()
+    with open(os.path.join(outdir, "ene.asc"), "a") as f:
+        f.write(fmt_ene(t, ekin, epot) + "\n")
+    kmax, kmaxred, dsumi = h.table("shells")
+    kmax, kmaxred = int(kmax), int(kmaxred)
+    spmf, alk = h.table("spmf"), h.table("alk")
+    qspec, _ = h.spec1d(qqs, 0)
+    sumqspec = 0.0
+    for k in range(1, kmax + 1):
+        sumqspec = sumqspec + qspec[k]
+        qspec[k] = spmf[k] * qspec[k]
+    sumqspec = 8.0 * sumqspec * dsumi
+    wka = h.spctop(qqs)
+    bety = h.table("bety")
+    qqr4 = (wka + bety[None, :]).astype(np.float32)
+    write_r4_record(os.path.join(outdir, "qq.r4"), igrec + 1, t, qqr4)
+    _, qql2 = h.norms(wka)
+    with open(os.path.join(outdir, "norms.asc"), "a") as f:
+        f.write(fmt_norms(t, qql2) + "\n")
+    with open(os.path.join(outdir, "spectra.asc"), "a") as f:
+        f.write(fmt_spectra_head(t, sumqspec, qql2, kmaxred) + "\n")
+        with np.errstate(divide="ignore"):
+            for k in range(1, kmaxred + 1):
+                f.write(fmt_spectra_line(alk[k], np.log10(qspec[k])) + "\n")
+    return igrec + 1

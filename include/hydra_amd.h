@@ -103,7 +103,9 @@ int hyd_energy(hyd_ctx *ctx, double *ekin, double *epot);
 
 /* Spectral tables of init_spectral (spectral.f90:14-22): id 1 green, 2 flo,
    3 fhi, 4 qdiss, 5 rksq (ng*ng each); 6 hrkx, 7 rkx, 8 hrky, 9 rky (ng);
-   10 xtrig, 11 ytrig (2*ng). */
+   10 xtrig, 11 ytrig (2*ng); spectrum shells (spectral.f90:110-133): 12 spmf,
+   13 alk (ng+1 each, alk(0) unused), 14 kmax, kmaxred, dsumi (3 values);
+   15 bety (ng). */
 int hyd_get_table(hyd_ctx *ctx, int32_t id, double *out);
 
 /* ---- ca/plane/qg/caps/contours.f90 ---------------------------------------- */

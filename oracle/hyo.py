@@ -428,5 +428,43 @@ class Ref:
         self.L.ref_kmax(C.byref(k))
         return k.value
 
+    def readcont(self, directory, iind):
+        """common.f90 readcont(iind) run in directory: (qs, qr, t); the
+        contours are then the module's (contours())."""
+        self.L.ref_readcont.argtypes = [C.c_int, _D, _D, C.POINTER(C.c_double)]
+        n = self.ng
+        qs, qr, t = np.empty((n, n)), np.empty((n, n)), C.c_double()
+        cwd = os.getcwd()
+        os.chdir(directory)
+        try:
+            run_big_stack(self.L.ref_readcont, int(iind), qs, qr, C.byref(t))
+        finally:
+            os.chdir(cwd)
+        return qs, qr, t.value
+
+    def read_qq_init(self, directory):
+        """caps.f90:150-153's open/read of directory/qq_init.r8: (t, qq)."""
+        self.L.ref_read_qq_init.argtypes = [C.POINTER(C.c_double), _D]
+        n = self.ng
+        qq, t = np.empty((n, n)), C.c_double()
+        cwd = os.getcwd()
+        os.chdir(directory)
+        try:
+            run_big_stack(self.L.ref_read_qq_init, C.byref(t), qq)
+        finally:
+            os.chdir(cwd)
+        return t.value, qq
+
+    def format(self, kind, v=(), iv=()):
+        """The reference's output formats (see ref_shim.f90 ref_format)."""
+        self.L.ref_format.argtypes = [C.c_int, _D, _I, C.c_char_p]
+        va = np.zeros(4)
+        va[:len(v)] = v
+        ia = np.zeros(2, np.int32)
+        ia[:len(iv)] = iv
+        buf = C.create_string_buffer(128)
+        self.L.ref_format(int(kind), va, ia, buf)
+        return buf.raw.decode().rstrip()
+
     def recontour(self, qq):
         run_big_stack(self.L.ref_recontour, np.ascontiguousarray(qq, dtype=np.float64).copy())

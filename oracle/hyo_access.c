@@ -12,7 +12,7 @@ double *hyo_dfield(hyo_model *m, const char *name) {
   if (!strcmp(name, #n)) return m->n;
   D(rksq) D(qdiss) D(green) D(flo) D(fhi) D(rk) D(rkx) D(hrkx) D(rky) D(hrky) D(bety)
   D(xtrig) D(ytrig) D(xq) D(yq) D(qs) D(qd) D(qc) D(qq) D(pp) D(uu) D(vv) D(qr) D(emq)
-  D(epq) D(qspre) D(xgf) D(ygf) D(xgu) D(ygu)
+  D(epq) D(qspre) D(xgf) D(ygf) D(xgu) D(ygu) D(spmf) D(alk)
 #undef D
   return 0;
 }
@@ -46,6 +46,7 @@ long hyo_get_i(const hyo_model *m, const char *name) {
 #define I(n) \
   if (!strcmp(name, #n)) return (long)m->n;
   I(nx) I(ny) I(nq) I(nptq) I(npm) I(nm) I(contex) I(ireg) I(err) I(nxf) I(nyf) I(nxu) I(nyu)
+  I(kmax) I(kmaxred)
 #undef I
   return -1;
 }

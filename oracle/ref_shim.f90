@@ -16,6 +16,8 @@ use contours
 use spectral
 use congen
 use generic
+use common, only: readcont, qs_common => qs, qr_common => qr
+use variables, only: t_var => t
 
 implicit none
 
@@ -192,6 +194,53 @@ end subroutine
 subroutine ref_kmax(k) bind(c, name='ref_kmax')
   integer(c_int):: k
   k = kmax
+end subroutine
+
+! common.f90:30-100, readcont(iind) in the current working directory
+! (cont/qqsynopsis.asc, cont/qqindexNNN, cont/qqnodesNNN, cont/qqresi.r4)
+subroutine ref_readcont(iind, qs_out, qr_out, tout) bind(c, name='ref_readcont')
+  integer(c_int), value:: iind
+  real(c_double):: qs_out(nx,ny), qr_out(ny,nx), tout
+  call readcont(iind)
+  qs_out = qs_common
+  qr_out = qr_common
+  tout = t_var
+end subroutine
+
+! The open and read statements of caps.f90:150-153 (qq_init.r8 in the
+! current working directory).
+subroutine ref_read_qq_init(tout, qq) bind(c, name='ref_read_qq_init')
+  real(c_double):: tout, qq(ny,nx)
+  double precision:: tt, qa(ny,nx)
+  open(11,file='qq_init.r8',form='unformatted', &
+      & access='direct',status='old',recl=2*nbytes)
+  read(11,rec=1) tt,qa
+  close(11)
+  tout = tt
+  qq = qa
+end subroutine
+
+! The formats of evolution.f90:676-728 and 750, written to line (len 128):
+! kind 1 ene.asc, 2 norms.asc, 3 spectra.asc header, 4 spectra.asc line,
+! 5 qqsynopsis.asc.  v holds the reals, iv the integers in statement order.
+subroutine ref_format(kind, v, iv, line) bind(c, name='ref_format')
+  integer(c_int), value:: kind
+  real(c_double):: v(4)
+  integer(c_int):: iv(2)
+  character(kind=c_char):: line(128)
+  character(len=128):: buf
+  integer:: i
+  buf = ' '
+  select case (kind)
+  case (1); write(buf,'(f9.2,3(1x,f17.11))') v(1),v(2),v(3),v(2)+v(3)
+  case (2); write(buf,'(f9.2,1x,f16.9)') v(1),v(2)
+  case (3); write(buf,'(f9.2,2(1x,f16.9),1x,i5)') v(1),v(2),v(3),iv(1)
+  case (4); write(buf,'(2(1x,f12.8))') v(1),v(2)
+  case (5); write(buf,'(i9,1x,i10,1x,f12.5,1x,f16.12)') iv(1),iv(2),v(1),v(2)
+  end select
+  do i = 1, 128
+    line(i) = buf(i:i)
+  enddo
 end subroutine
 
 ! congen.f90 recontour: qqr is the residual (or the full field when there are

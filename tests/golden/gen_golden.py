@@ -56,6 +56,10 @@ def main():
     for iopt in (0, 1):
         out[f"spec{iopt}"], out[f"spec{iopt}_ss"] = r.spec1d(s, iopt)
     out["norm_l1"], out["norm_l2"] = (np.float64(v) for v in r.norms(f))
+    # the output formats of evolution.f90:676-728 and 750, written by the
+    # reference's format statements (cases listed in tests/test_io.py)
+    from tests.test_io import FMT_CASES
+    out["fmt_lines"] = np.array([r.format(k, v, iv) for k, v, iv in FMT_CASES])
     # renode of the first three contours, contours.f90:687-933
     for j in range(3):
         i1, i2 = c["i1q"][j], c["i2q"][j]
