@@ -428,14 +428,20 @@ int advance(Ctx &c) {
   HYD_CHECK(fork_s2(c));
   HYD_CHECK(on_s2(c, [&]() { return source(c, 0); }));
   HYD_CHECK(rk_nodes(c, 1));
-  HYD_CHECK(adapt_finish(c));
-  // the field update needs the integrating factors adapt_finish uploads
-  HYD_CHECK(fork_s2(c));
+  // stage 2's contour grid needs only the stage-1 nodes: it is queued
+  // before the host evaluates the integrating factors, so that the GPU has
+  // it to run meanwhile; the factors are uploaded and expanded on the
+  // second stream, ahead of the stage-1 field update that needs them
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
+  HYD_CHECK(contour_grid(c));
+  HYD_CHECK(on_s2(c, [&]() { return adapt_finish(c); }));
   HYD_CHECK(fields(1));
   // stages 2-4
   for (int stage = 2; stage <= 4; stage++) {
-    if (stage != 3) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
-    HYD_CHECK(contour_grid(c));
+    if (stage > 2) {
+      if (stage != 3) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
+      HYD_CHECK(contour_grid(c));
+    }
     HYD_HIP(hipStreamWaitEvent(c.s, c.ev_join, 0));
     HYD_CHECK(inversion_fields(c));
     HYD_CHECK(fork_s2(c));
