@@ -13,6 +13,8 @@
 // reference, once per distinct qdiss value, and expanded on the device.
 #include <math.h>
 
+#include <condition_variable>
+#include <mutex>
 #include <thread>
 
 #include "hyd_internal.h"
@@ -266,26 +268,92 @@ int gradient_dev(Ctx &c, const double *ff, double *fx, double *fy, double *tmp) 
   return HYD_OK;
 }
 
-// exp(dfac*qdiss_u[j]) on the host, split over a few threads.
+// exp(dfac*qdiss_u[j]) on the host (the reference's libm exp), split over
+// persistent worker threads: creating threads every step would cost more
+// than the table itself.  One process-wide pool; concurrent callers take
+// turns.
+class ExpPool {
+ public:
+  static ExpPool &get() {
+    static ExpPool p;
+    return p;
+  }
+  void run(const double *q, size_t n, double dfac, double *out) {
+    std::lock_guard<std::mutex> turn(submit_);
+    if (nw_ == 0 || n < 4096) {
+      for (size_t j = 0; j < n; j++) out[j] = exp(dfac * q[j]);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> l(m_);
+      q_ = q;
+      n_ = n;
+      dfac_ = dfac;
+      out_ = out;
+      pending_ = nw_;
+      gen_++;
+    }
+    cv_.notify_all();
+    slice(nw_, q, n, dfac, out);  // the caller takes the last share
+    std::unique_lock<std::mutex> l(m_);
+    done_.wait(l, [&] { return pending_ == 0; });
+  }
+
+ private:
+  ExpPool() {
+    const unsigned hc = std::thread::hardware_concurrency();
+    nw_ = hc > 8 ? 7u : (hc > 1 ? hc - 1 : 0u);
+    for (unsigned i = 0; i < nw_; i++) th_.emplace_back([this, i] { loop(i); });
+  }
+  ~ExpPool() {
+    {
+      std::lock_guard<std::mutex> l(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  void slice(unsigned i, const double *q, size_t n, double dfac, double *out) const {
+    const size_t parts = nw_ + 1, per = (n + parts - 1) / parts;
+    const size_t b = i * per, e = std::min(n, b + per);
+    for (size_t j = b; j < e; j++) out[j] = exp(dfac * q[j]);
+  }
+  void loop(unsigned i) {
+    uint64_t seen = 0;
+    for (;;) {
+      const double *q;
+      double *out;
+      size_t n;
+      double dfac;
+      {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        q = q_;
+        n = n_;
+        dfac = dfac_;
+        out = out_;
+      }
+      slice(i, q, n, dfac, out);
+      std::lock_guard<std::mutex> l(m_);
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  std::mutex submit_, m_;
+  std::condition_variable cv_, done_;
+  std::vector<std::thread> th_;
+  unsigned nw_ = 0, pending_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+  const double *q_ = nullptr;
+  double *out_ = nullptr;
+  size_t n_ = 0;
+  double dfac_ = 0.0;
+};
+
 static void host_exp_table(const std::vector<double> &qu, double dfac, double *out) {
-  const size_t n = qu.size();
-  unsigned nt = std::thread::hardware_concurrency();
-  if (nt > 8) nt = 8;
-  if (nt < 1 || n < 4096) nt = 1;
-  auto work = [&](size_t b, size_t e) {
-    for (size_t j = b; j < e; j++) out[j] = exp(dfac * qu[j]);
-  };
-  if (nt == 1) {
-    work(0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  const size_t per = (n + nt - 1) / nt;
-  for (unsigned t = 0; t < nt; t++) {
-    const size_t b = t * per, e = std::min(n, b + per);
-    if (b < e) th.emplace_back(work, b, e);
-  }
-  for (auto &x : th) x.join();
+  ExpPool::get().run(qu.data(), qu.size(), dfac, out);
 }
 
 int adapt(Ctx &c) {
