@@ -12,7 +12,8 @@ node and spectral updates).  Inputs are resident in HBM before timing.
 
 Besides the JSON line the script prints nothing on stdout.  The roofline
 entry is for the kernel with the largest share of the timed region, timed
-with HIP events on the library's stream; cpu_baseline is the oracle (C
+with per-launch HIP events on the library's stream (in steps run on one
+stream, so that each launch runs alone); cpu_baseline is the oracle (C
 restatement of the reference, one core) on a bounded sample of the same
 state.
 """
@@ -266,6 +267,9 @@ def main():
         if name.startswith("k:"):
             kbytes.setdefault(name, []).append(by)
     dom = max(kbytes, key=lambda k: tot[k])
+    # for reference: the interval the same per-launch events report for an
+    # empty kernel (launch and stamping cost; not subtracted)
+    ev_ms = h.event_overhead(200)
     avg_ms = tot[dom] / cnt[dom]
     alg = float(np.mean(kbytes[dom]))
     achieved = alg / (avg_ms * 1e-3) / 1e9
@@ -273,7 +277,8 @@ def main():
     roofline = {"bound": "hbm", "kernel": ksym, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profile(ksym, alg),
-                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": cnt[dom] / kr,
+                "avg_launch_ms": round(avg_ms, 4), "empty_launch_ms": round(ev_ms, 4),
+                "launches_per_step": cnt[dom] / kr,
                 "algorithmic_bytes_per_launch": int(alg)}
 
     if args.copy_calib:

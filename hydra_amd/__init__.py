@@ -96,6 +96,7 @@ def lib():
         L.hyd_run_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int32]
         L.hyd_get_tuning.argtypes = [C.c_void_p, _I]
         L.hyd_copy_calibration.argtypes = [C.c_void_p, C.c_int64]
+        L.hyd_event_overhead.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         L.hyd_set_timing.argtypes = [C.c_void_p, C.c_int32]
         L.hyd_kernel_times.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
                                        np.ctypeslib.ndpointer(dtype=np.float64),
@@ -336,6 +337,12 @@ class Hydra:
     def copy_calibration(self, nbytes):
         """One streaming copy kernel of nbytes (traffic-counter reference)."""
         _check(self.L.hyd_copy_calibration(self.h, int(nbytes)))
+
+    def event_overhead(self, reps=200):
+        """ms the per-launch timing events report for an empty kernel"""
+        v = C.c_double()
+        _check(self.L.hyd_event_overhead(self.h, int(reps), C.byref(v)))
+        return v.value
 
     def sync(self):
         _check(self.L.hyd_sync(self.h))

@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <mutex>
 
+#include <hip/hip_ext.h>
+
 #include "hyd_internal.h"
 
 namespace hyd {
@@ -910,6 +912,42 @@ __global__ void k_copy_calib(const double *__restrict__ a, double *__restrict__ 
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x)
     b[k] = a[k];
+}
+
+__global__ void k_empty() {}
+
+// The cost of the per-launch timing itself: reps launches of an empty
+// kernel, each with its own start/stop events as the timed kernels have
+// them (hipExtLaunchKernel); the average event interval in ms.
+int hyd_event_overhead(hyd_ctx *h, int32_t reps, double *avg_ms) {
+  Ctx &c = *C(h);
+  if (reps < 1 || !avg_ms) {
+    set_error("event_overhead: reps >= 1 and a result pointer");
+    return HYD_EARG;
+  }
+  std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+  int rc = HYD_OK;
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) rc = HYD_EHIP;
+  for (int r = 0; r < reps && rc == HYD_OK; r++) {
+    hipExtLaunchKernelGGL(k_empty, dim3(1), dim3(64), 0u, c.s, ev[2 * r], ev[2 * r + 1], 0u);
+    if (hipGetLastError() != hipSuccess) rc = HYD_EHIP;
+  }
+  double tot = 0.0;
+  if (rc == HYD_OK && hipStreamSynchronize(c.s) != hipSuccess) rc = HYD_EHIP;
+  for (int r = 0; r < reps && rc == HYD_OK; r++) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]) != hipSuccess) rc = HYD_EHIP;
+    tot += ms;
+  }
+  for (auto &e : ev)
+    if (e) hipEventDestroy(e);
+  if (rc != HYD_OK) {
+    set_error("event_overhead: HIP event or launch failed");
+    return rc;
+  }
+  *avg_ms = tot / reps;
+  return HYD_OK;
 }
 
 int hyd_copy_calibration(hyd_ctx *h, int64_t nbytes) {

@@ -466,7 +466,10 @@ __global__ void k_tick(Scalars *sc) {
 template <class Fn>
 static int on_s2(Ctx &c, Fn fn) {
   hipStream_t main = c.s;
-  c.s = c.s2;
+  // with per-launch timing on, everything stays on the main stream, in
+  // program order (a valid order of the same work), so that each timed
+  // launch runs alone
+  c.s = c.timing ? main : c.s2;
   const int rc = fn();
   c.s = main;
   return rc;

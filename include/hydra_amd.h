@@ -202,6 +202,11 @@ int hyd_get_tuning(hyd_ctx *ctx, int32_t *v);
    kernel of nbytes (<= 128 ng^2) read and nbytes written, the reference
    point that scales hardware traffic counters. */
 int hyd_copy_calibration(hyd_ctx *ctx, int64_t nbytes);
+/* No reference counterpart (measurement aid): the average interval the
+   per-launch timing events report for an empty kernel launched the same
+   way (reps launches): the launch and stamping cost that every record of
+   hyd_kernel_times includes. */
+int hyd_event_overhead(hyd_ctx *ctx, int32_t reps, double *avg_ms);
 /* No reference counterpart: per-launch timing with HIP events on the
    library's stream.  Level 0 off, 1 the single kernels whose records start
    with "k:" (one record per launch), 2 every timed group as well.
