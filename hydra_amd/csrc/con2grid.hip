@@ -304,45 +304,84 @@ __device__ __forceinline__ double cell_value(int32_t w, int64_t key, double qjum
   return nb >= 0 ? b.vals[nb] : 0.0 / 0.0;
 }
 
-// qbot (contours.f90:234-238): one wavefront, chain over the non-empty
-// bottom-row counters.
-// The counters and their values are staged in LDS by the whole block; wave 0
-// runs the chain.  nxf <= kQbotMax.
+// Inclusive wavefront scan of ints (row shifts, then row broadcasts).
+template <int CTRL>
+__device__ __forceinline__ int c2g_dpp(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int wave_scan_i32(int v) {
+  const int l = threadIdx.x & 63, rl = l & 15;
+  int t;
+  t = c2g_dpp<0x111>(v); if (rl >= 1) v += t;
+  t = c2g_dpp<0x112>(v); if (rl >= 2) v += t;
+  t = c2g_dpp<0x114>(v); if (rl >= 4) v += t;
+  t = c2g_dpp<0x118>(v); if (rl >= 8) v += t;
+  t = c2g_dpp<0x142>(v); if ((l & 31) >= 16) v += t;
+  t = c2g_dpp<0x143>(v); if (l >= 32) v += t;
+  return v;
+}
+
+// qbot (contours.f90:234-238): qbot(1) = 0, qbot(ix+1) = qbot(ix) + qjx(ix)
+// over the non-empty bottom-row counters, in order.  Per pass of kQbotPart
+// counters the block evaluates the counters' values and compacts the
+// non-empty ones into LDS in order (a block scan of their count); wave 0
+// then runs the chain over the compacted list 64 values at a time (fixed
+// lane indices, so each step is one dependent add), recording the running
+// value after each; every counter then reads the value after the last
+// non-empty one at or before it.  The additions are the reference's, in its
+// order.  nxf <= kQbotMax.
 static constexpr int kQbotMax = 32768;
+static constexpr int kQbotPart = 2048;  // counters per pass, two per thread
 __global__ void __launch_bounds__(1024) c2g_qbot(const int32_t *qjx, double *qbot, int nxf,
                                                  int64_t nkey, double qjump, Buckets bk) {
-  __shared__ double cvs[kQbotMax / 4];   // values of one quarter at a time
-  __shared__ uint8_t nz[kQbotMax / 4];
-  const int q4 = kQbotMax / 4;
-  double v = 0.0;  // qbot(1)
-  if (threadIdx.x == 0) qbot[0] = 0.0;
-  // qbot(ix+1) = qbot(ix) + qjx(ix), ix = 1..nxf-1
-  for (int part = 0; part < nxf - 1; part += q4) {
-    const int m = min(q4, nxf - 1 - part);
-    for (int t = threadIdx.x; t < m; t += blockDim.x) {
-      const int32_t w = qjx[part + t];
-      nz[t] = w != 0;
-      cvs[t] = w ? cell_value(w, nkey + part + t, qjump, bk) : 0.0;
-    }
+  __shared__ double comp[kQbotPart];  // values of the non-empty counters, in order
+  __shared__ double pref[kQbotPart];  // the chain after each of them
+  __shared__ int wsum[16];
+  __shared__ double vin;              // the chain's value entering the pass
+  const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
+  double v = 0.0;  // qbot(1); wave 0 carries the chain across passes
+  if (tid == 0) qbot[0] = 0.0;
+  for (int part = 0; part < nxf - 1; part += kQbotPart) {
+    const int m = min(kQbotPart, nxf - 1 - part);
+    const int t0 = 2 * tid, t1 = t0 + 1;
+    const int32_t w0 = t0 < m ? qjx[part + t0] : 0;
+    const int32_t w1 = t1 < m ? qjx[part + t1] : 0;
+    const double c0 = w0 ? cell_value(w0, nkey + part + t0, qjump, bk) : 0.0;
+    const double c1 = w1 ? cell_value(w1, nkey + part + t1, qjump, bk) : 0.0;
+    const int n0 = w0 != 0, n1 = w1 != 0, cnt = n0 + n1;
+    const int incl = wave_scan_i32(cnt);
+    if (l == 63) wsum[wv] = incl;
+    if (tid == 0) vin = v;
     __syncthreads();
-    if (threadIdx.x < 64) {
-      const int l = threadIdx.x;
-      for (int base = 0; base < m; base += 64) {
-        const int t = base + l;
-        const bool ok = t < m;
-        const bool w = ok && nz[t];
-        const double cv = w ? cvs[t] : 0.0;
-        uint64_t mask = __ballot(w);
-        double mine = v;
-        while (mask) {
-          const int b = __ffsll((unsigned long long)mask) - 1;
-          mask &= mask - 1;
-          v = readlane_f64(cv, b) + v;
-          if (l >= b) mine = v;
+    int off = 0, total = 0;
+    for (int k = 0; k < 16; k++) {
+      const int ws = wsum[k];
+      if (k < wv) off += ws;
+      total += ws;
+    }
+    const int ex = off + incl - cnt;  // non-empty counters before t0
+    if (n0) comp[ex] = c0;
+    if (n1) comp[ex + n0] = c1;
+    __syncthreads();
+    if (wv == 0) {
+      for (int j0 = 0; j0 < total; j0 += 64) {
+        const int nk = min(64, total - j0);
+        const double val = l < nk ? comp[j0 + l] : 0.0;
+        double mine = 0.0;
+#pragma unroll
+        for (int k = 0; k < 64; k++) {
+          if (k >= nk) break;
+          v = readlane_f64(val, k) + v;
+          if (l == k) mine = v;
         }
-        if (ok) qbot[part + t + 1] = mine;
+        if (l < nk) pref[j0 + l] = mine;
       }
     }
+    __syncthreads();
+    const double v0 = vin;
+    const int i0 = ex + n0, i1 = i0 + n1;  // non-empty counters up to t0, t1
+    if (t0 < m) qbot[part + t0 + 1] = i0 > 0 ? pref[i0 - 1] : v0;
+    if (t1 < m) qbot[part + t1 + 1] = i1 > 0 ? pref[i1 - 1] : v0;
     __syncthreads();
   }
 }
@@ -712,22 +751,6 @@ __global__ void c2g_colscan_int(const int32_t *cnt, const int32_t *qb, float *fq
   }
 }
 
-// Inclusive wavefront scan of ints (row shifts, then row broadcasts).
-template <int CTRL>
-__device__ __forceinline__ int c2g_dpp(int v) {
-  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false);
-}
-__device__ __forceinline__ int wave_scan_i32(int v) {
-  const int l = threadIdx.x & 63, rl = l & 15;
-  int t;
-  t = c2g_dpp<0x111>(v); if (rl >= 1) v += t;
-  t = c2g_dpp<0x112>(v); if (rl >= 2) v += t;
-  t = c2g_dpp<0x114>(v); if (rl >= 4) v += t;
-  t = c2g_dpp<0x118>(v); if (rl >= 8) v += t;
-  t = c2g_dpp<0x142>(v); if ((l & 31) >= 16) v += t;
-  t = c2g_dpp<0x143>(v); if (l >= 32) v += t;
-  return v;
-}
 
 // Integer y sweep fused with the first FP32 nine-point average (4ng ->
 // 2ng): the blocking of c2g_colscan_avg9, with each 64-row tile of a column
