@@ -105,6 +105,11 @@ def read_r4_record(path, rec, ng):
 # ---- formatted lines -------------------------------------------------------------
 
 def _f(x, w, d):
+    if x != x or x in (float("inf"), float("-inf")):
+        # non-finite values as the amdflang runtime writes them (gfortran
+        # spells out Infinity where the field is wide enough)
+        s = "NaN" if x != x else ("Inf" if x > 0 else "-Inf")
+        return s.rjust(w) if len(s) <= w else "*" * w
     s = f"{x:{w}.{d}f}"
     return s if len(s) <= w else "*" * w
 

@@ -25,6 +25,9 @@ FMT_CASES = [
     (4, (1.0, -3.5e-9), ()),
     (5, (0.125, 0.1234567), (264, 7805)),
     (5, (123.456789, 2.5e-7), (0, 0)),
+    (4, (1.0, float("-inf")), ()),
+    (4, (0.5, float("nan")), ()),
+    (2, (12345678.0, 1.0), ()),
 ]
 
 
