@@ -81,6 +81,8 @@ def lib():
         L.hyd_set_field.argtypes = [C.c_void_p, _D, C.c_double]
         L.hyd_set_contours.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _D, _D, _I, _I, _I, _I,
                                        _I, C.c_double]
+        L.hyd_check_contours.argtypes = [C.POINTER(HydParams), C.c_int32, C.c_int32, _D, _D, _I,
+                                         _I, _I, _I, _I]
         L.hyd_get_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.hyd_get_contours.argtypes = [C.c_void_p, _D, _D, _I, _I, _I, _I, _I]
         L.hyd_get_field.argtypes = [C.c_void_p, C.c_int32, _D]
@@ -131,6 +133,30 @@ def _f64(a, shape=None):
     return a
 
 
+def _params(ng, ncontq=80, ellx=2 * np.pi, elly=None, kd=0.0, beta=0.0, rekman=0.0, cdamp=2.0,
+            nnu=3.0, tgsave=1000.0, tcsave=1000.0, dt_fixed=0.0, node_capacity=0, device=0,
+            precision=0):
+    return HydParams(ng=ng, ncontq=ncontq, ellx=ellx, elly=ellx if elly is None else elly, kd=kd,
+                     beta=beta, rtherm=0.0, rekman=rekman, cdamp=cdamp, nnu=nnu, tgsave=tgsave,
+                     tcsave=tcsave, dt_fixed=dt_fixed, node_capacity=node_capacity,
+                     precision=precision, device=device)
+
+
+def _contour_args(c):
+    ii = {k: np.ascontiguousarray(c[k], dtype=np.int32) for k in ("nextq", "i1q", "i2q", "npq", "indq")}
+    return (len(c["x"]), len(c["i1q"]), _f64(c["x"]), _f64(c["y"]), ii["nextq"], ii["i1q"],
+            ii["i2q"], ii["npq"], ii["indq"])
+
+
+def check_contours(c, ng, **kw):
+    """The contour-table checks of set_contours, on the host without a
+    device (contours numbered consecutively from node 1, ascending levels,
+    one nextq cycle per contour, nodes inside the domain); raises
+    HydraError naming the first that fails."""
+    p = _params(ng, **kw)
+    _check(lib().hyd_check_contours(C.byref(p), *_contour_args(c)))
+
+
 class Hydra:
     """One flow on one GPU: the reference's module state, device resident."""
 
@@ -139,11 +165,10 @@ class Hydra:
                  node_capacity=0, device=0, precision=0):
         self.L = lib()
         self.ng = ng
-        self.params = HydParams(ng=ng, ncontq=ncontq, ellx=ellx, elly=ellx if elly is None else elly,
-                                kd=kd, beta=beta, rtherm=0.0, rekman=rekman, cdamp=cdamp, nnu=nnu,
-                                tgsave=tgsave, tcsave=tcsave, dt_fixed=dt_fixed,
-                                node_capacity=node_capacity, precision=precision,
-                                device=device)
+        self.params = _params(ng, ncontq=ncontq, ellx=ellx, elly=elly, kd=kd, beta=beta,
+                              rekman=rekman, cdamp=cdamp, nnu=nnu, tgsave=tgsave, tcsave=tcsave,
+                              dt_fixed=dt_fixed, node_capacity=node_capacity, device=device,
+                              precision=precision)
         h = C.c_void_p()
         _check(self.L.hyd_init(C.byref(self.params), C.byref(h)))
         self.h = h
@@ -260,10 +285,7 @@ class Hydra:
         _check(self.L.hyd_set_field(self.h, _f64(qq, (self.ng, self.ng)), float(t0)))
 
     def set_contours(self, c):
-        n, nc = len(c["x"]), len(c["i1q"])
-        ii = {k: np.ascontiguousarray(c[k], dtype=np.int32) for k in ("nextq", "i1q", "i2q", "npq", "indq")}
-        _check(self.L.hyd_set_contours(self.h, n, nc, _f64(c["x"]), _f64(c["y"]), ii["nextq"],
-                                       ii["i1q"], ii["i2q"], ii["npq"], ii["indq"], float(c["qjump"])))
+        _check(self.L.hyd_set_contours(self.h, *_contour_args(c), float(c["qjump"])))
 
     def counts(self):
         a, b = C.c_int32(), C.c_int32()

@@ -29,22 +29,31 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    objs = []
+def _compile(src: str, objdir: str, verbose: bool) -> str:
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(objdir, src + ".o")
+    if src.endswith(".cpp"):
+        cmd = [HIPCC, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+               "-c", path, "-o", obj]
+    else:
+        cmd = [HIPCC, "-x", "hip", *FLAGS, "-c", path, "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile every source (in parallel, at most 8 at a time) and link."""
+    from concurrent.futures import ThreadPoolExecutor
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    for src in SOURCES:
-        path = os.path.join(CSRC, src)
-        obj = os.path.join(objdir, src + ".o")
-        lang = [] if src.endswith(".cpp") else ["-x", "hip"]
-        cmd = [HIPCC, *lang, *FLAGS, "-c", path, "-o", obj]
-        if src.endswith(".cpp"):
-            cmd = [HIPCC, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
-                   "-c", path, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    # longest compiles first so the pool finishes together
+    order = sorted(SOURCES, key=lambda f: -os.path.getsize(os.path.join(CSRC, f)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        done = dict(zip(order, ex.map(lambda f: _compile(f, objdir, verbose), order)))
+    objs = [done[f] for f in SOURCES]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

@@ -449,6 +449,84 @@ int hyd_get_table(hyd_ctx *h, int32_t id, double *out) {
   return HYD_OK;
 }
 
+static int check_links(const int32_t *nextq, int32_t n) {
+  for (int32_t i = 0; i < n; i++)
+    if (nextq[i] < 1 || nextq[i] > n) {
+      set_error("nextq holds a node number outside 1..nptq");
+      return HYD_EARG;
+    }
+  return HYD_OK;
+}
+
+// Every cell index the kernels form from a position stays on its grid:
+// 1+int(glxi*(x-xmin)) (velint, contours.f90:146), 1+int(glxfi*(x-xmin))
+// (con2grid, :207) and int(glxui*(x-xmin)) (con2ugrid, congen.f90:354) all
+// lie in 0..n-1 (+1) exactly when these do; NaN and infinities fail.
+static bool position_ok(double x, double xmin, double g0, int n0, double g1, int n1, double g2,
+                        int n2) {
+  const double d = x - xmin;
+  if (!(d >= 0.0)) return false;
+  const double a = g0 * d, b = g1 * d, e = g2 * d;
+  return a < (double)n0 && b < (double)n1 && e < (double)n2;
+}
+
+static int check_positions(const Consts &k, const double *xq, const double *yq, int64_t n) {
+  for (int64_t i = 0; i < n; i++)
+    if (!position_ok(xq[i], k.xmin, k.glxi, k.nx, k.glxfi, k.nxf, k.glxui, k.nxu) ||
+        !position_ok(yq[i], k.ymin, k.glyi, k.ny, k.glyfi, k.nyf, k.glyui, k.nyu)) {
+      set_error("a node lies outside the periodic domain [xmin,xmax) x [ymin,ymax)");
+      return HYD_EARG;
+    }
+  return HYD_OK;
+}
+
+// The contour store's invariants (contours.f90:12-15, common.f90:78-88):
+// contours numbered consecutively from node 1 (i1q(1) = 1, i1q(j+1) =
+// i2q(j)+1, i2q(nq) = nptq, npq = i2q-i1q+1 >= 1), levels ascending (indq,
+// which surgery relies on, contours.f90:372-394), and nextq one closed
+// cycle through each contour's own nodes.  Surgery, renode and congen index
+// device arrays with these numbers, so a table breaking them is rejected
+// before anything reaches the device.
+static int check_table(const Consts &k, int32_t nptq, int32_t nq, const double *xq,
+                       const double *yq, const int32_t *nextq, const int32_t *i1q,
+                       const int32_t *i2q, const int32_t *npq, const int32_t *indq) {
+  if (nptq < 0 || nq < 0 || (nq == 0) != (nptq == 0)) {
+    set_error("contour table: nq and nptq must both be zero or both positive");
+    return HYD_EARG;
+  }
+  if (nq == 0) return HYD_OK;
+  if (!xq || !yq || !nextq || !i1q || !i2q || !npq || !indq) {
+    set_error("contour table: null array");
+    return HYD_EARG;
+  }
+  int64_t expect = 1;
+  for (int32_t j = 0; j < nq; j++) {
+    if (i1q[j] != expect || i2q[j] < i1q[j] || i2q[j] > nptq || npq[j] != i2q[j] - i1q[j] + 1 ||
+        (j > 0 && indq[j] < indq[j - 1])) {
+      set_error("contour table: i1q/i2q/npq not consecutive from node 1, or indq descending");
+      return HYD_EARG;
+    }
+    expect = (int64_t)i2q[j] + 1;
+  }
+  if (expect != (int64_t)nptq + 1) {
+    set_error("contour table: i2q(nq) differs from nptq");
+    return HYD_EARG;
+  }
+  HYD_CHECK(check_links(nextq, nptq));
+  for (int32_t j = 0; j < nq; j++) {
+    int32_t i = i1q[j];
+    for (int32_t m = 0; m < npq[j]; m++) {
+      const int32_t nx = nextq[i - 1];
+      if (nx < i1q[j] || nx > i2q[j] || (nx == i1q[j]) != (m == npq[j] - 1)) {
+        set_error("contour table: nextq is not one closed cycle through each contour");
+        return HYD_EARG;
+      }
+      i = nx;
+    }
+  }
+  return check_positions(k, xq, yq, nptq);
+}
+
 int hyd_velint(hyd_ctx *h, const double *uu, const double *vv, const double *xq, const double *yq,
                int32_t nptq, double *uq, double *vq) {
   Ctx &c = *C(h);
@@ -457,6 +535,7 @@ int hyd_velint(hyd_ctx *h, const double *uu, const double *vv, const double *xq,
     return HYD_ECAPACITY;
   }
   if (nptq == 0) return HYD_OK;
+  HYD_CHECK(check_positions(c.c, xq, yq, nptq));
   HYD_CHECK(h2d(c, c.wa, uu, c.N));
   HYD_CHECK(h2d(c, c.wb, vv, c.N));
   HYD_CHECK(h2d(c, c.xqi, xq, nptq));
@@ -464,15 +543,6 @@ int hyd_velint(hyd_ctx *h, const double *uu, const double *vv, const double *xq,
   HYD_CHECK(velint(c, c.wa, c.wb, c.xqi, c.yqi, nptq, c.uq, c.vq));
   HYD_CHECK(d2h(c, uq, c.uq, nptq));
   return d2h(c, vq, c.vq, nptq);
-}
-
-static int check_links(const int32_t *nextq, int32_t n) {
-  for (int32_t i = 0; i < n; i++)
-    if (nextq[i] < 1 || nextq[i] > n) {
-      set_error("nextq holds a node number outside 1..nptq");
-      return HYD_EARG;
-    }
-  return HYD_OK;
 }
 
 int hyd_con2grid(hyd_ctx *h, const double *xq, const double *yq, const int32_t *nextq,
@@ -483,6 +553,7 @@ int hyd_con2grid(hyd_ctx *h, const double *xq, const double *yq, const int32_t *
     return HYD_ECAPACITY;
   }
   HYD_CHECK(check_links(nextq, nptq));
+  HYD_CHECK(check_positions(c.c, xq, yq, nptq));
   if (nptq > 0) {
     HYD_CHECK(h2d(c, c.xqi, xq, nptq));
     HYD_CHECK(h2d(c, c.yqi, yq, nptq));
@@ -536,6 +607,7 @@ int hyd_set_field(hyd_ctx *h, const double *qq, double t0) {
   HYD_HIP(hipStreamSynchronize(c.s));
   c.nptq = 0;
   c.nq = 0;
+  c.phase = 0;
   c.have_field = true;
   return HYD_OK;
 }
@@ -549,7 +621,7 @@ int hyd_set_contours(hyd_ctx *h, int32_t nptq, int32_t nq, const double *xq, con
     set_error("set_contours: counts outside capacity");
     return HYD_ECAPACITY;
   }
-  HYD_CHECK(check_links(nextq, nptq));
+  HYD_CHECK(check_table(c.c, nptq, nq, xq, yq, nextq, i1q, i2q, npq, indq));
   if (nptq > 0) {
     HYD_CHECK(h2d(c, c.xq, xq, nptq));
     HYD_CHECK(h2d(c, c.yq, yq, nptq));
@@ -561,12 +633,25 @@ int hyd_set_contours(hyd_ctx *h, int32_t nptq, int32_t nq, const double *xq, con
   c.indq.assign(indq, indq + nq);
   c.nptq = nptq;
   c.nq = nq;
+  c.phase = 0;  // a restart recontours first (caps.f90 recont)
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
   HYD_HIP(hipStreamSynchronize(c.s));
   c.hsc->qjump = qjump;
   HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
   HYD_HIP(hipStreamSynchronize(c.s));
   return HYD_OK;
+}
+
+int hyd_check_contours(const hyd_params *p, int32_t nptq, int32_t nq, const double *xq,
+                       const double *yq, const int32_t *nextq, const int32_t *i1q,
+                       const int32_t *i2q, const int32_t *npq, const int32_t *indq) {
+  if (!p) {
+    set_error("check_contours: null parameters");
+    return HYD_EARG;
+  }
+  Consts k;
+  HYD_CHECK(build_consts(*p, k));
+  return check_table(k, nptq, nq, xq, yq, nextq, i1q, i2q, npq, indq);
 }
 
 int hyd_get_counts(hyd_ctx *h, int32_t *nptq, int32_t *nq) {
@@ -609,6 +694,7 @@ int hyd_get_field(hyd_ctx *h, int32_t id, double *out) {
 
 int hyd_put_field(hyd_ctx *h, int32_t id, const double *in) {
   Ctx &c = *C(h);
+  c.graph_nptq = -1;  // state changed outside the captured step
   double *f = field_ptr(c, id);
   if (!f) {
     set_error("unknown field id");
@@ -639,6 +725,7 @@ int hyd_get_scalar(hyd_ctx *h, int32_t id, double *out) {
 
 int hyd_set_scalar(hyd_ctx *h, int32_t id, double val) {
   Ctx &c = *C(h);
+  c.graph_nptq = -1;  // the captured step holds scalars (qjump) as launch arguments
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
   HYD_HIP(hipStreamSynchronize(c.s));
   double *v = scalar_slot(*c.hsc, id);
@@ -661,6 +748,7 @@ static int advect_start(Ctx &c) {
   c.hsc->twist = 0.0;
   HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
   c.ireg = 0;
+  c.phase = 2;
   return HYD_OK;
 }
 
@@ -679,6 +767,7 @@ int hyd_prepare(hyd_ctx *h) {
   Ctx &c = *C(h);
   c.graph_nptq = -1;  // state changed outside the captured step
   HYD_CHECK(prepare(c));
+  c.phase = 0;
   return c2g_status(c);
 }
 
@@ -690,6 +779,7 @@ int hyd_recontour(hyd_ctx *h) {
     return HYD_ESTATE;
   }
   HYD_CHECK(recontour(c, c.qr));
+  c.phase = 1;
   return c2g_status(c);
 }
 
@@ -794,6 +884,7 @@ static int advect_step(Ctx &c) {
     if (c.ireg == nregmax) {
       HYD_CHECK(prepare(c));
       HYD_HIP(hipStreamSynchronize(c.s));
+      c.phase = 0;
       return 1;
     }
     HYD_CHECK(surgery(c));
@@ -816,7 +907,19 @@ int hyd_advect_step(hyd_ctx *h) {
   return advect_step(c);
 }
 
-// caps.f90:83-91 with a step count in place of the final time
+// Where a run resumes (c.phase): contours rebuilt from qr first (caps.f90
+// recont), then advect's start, or straight on inside advect's loop.
+static int resume_cycle(Ctx &c) {
+  if (c.phase == 0) {
+    HYD_CHECK(recontour(c, c.qr));
+    c.phase = 1;
+  }
+  if (c.phase == 1) HYD_CHECK(advect_start(c));
+  return HYD_OK;
+}
+
+// caps.f90:83-91 with a step count in place of the final time.  A run
+// resumes where the previous call stopped, so run(a) then run(b) is run(a+b).
 int hyd_run(hyd_ctx *h, int32_t nsteps) {
   Ctx &c = *C(h);
   if (!c.have_field) {
@@ -825,8 +928,7 @@ int hyd_run(hyd_ctx *h, int32_t nsteps) {
   }
   int done = 0;
   while (done < nsteps) {
-    HYD_CHECK(recontour(c, c.qr));
-    HYD_CHECK(advect_start(c));
+    HYD_CHECK(resume_cycle(c));
     while (done < nsteps) {
       const int r = advect_step(c);
       if (r < 0) return r;
@@ -863,7 +965,7 @@ int hyd_run_batch(hyd_ctx **hs, int32_t n, int32_t nsteps) {
   }
   const double twistmax = 2.5;
   const int nregmax = 20;
-  std::vector<int> done(n, 0), start(n, 1), stepped(n, 0);
+  std::vector<int> done(n, 0), stepped(n, 0);
   for (;;) {
     bool any = false;
     for (int i = 0; i < n; i++) {
@@ -871,17 +973,13 @@ int hyd_run_batch(hyd_ctx **hs, int32_t n, int32_t nsteps) {
       if (done[i] >= nsteps) continue;
       any = true;
       Ctx &c = *C(hs[i]);
-      if (start[i]) {
-        HYD_CHECK(recontour(c, c.qr));
-        HYD_CHECK(advect_start(c));
-        start[i] = 0;
-      }
+      HYD_CHECK(resume_cycle(c));
       if (c.hsc->twist > twistmax) {
         c.ireg = c.ireg + 1;
         if (c.ireg == nregmax) {
           HYD_CHECK(prepare(c));
           HYD_HIP(hipStreamSynchronize(c.s));
-          start[i] = 1;  // recontour next round; this round has no advance
+          c.phase = 0;  // recontour next round; this round has no advance
           continue;
         }
         HYD_CHECK(surgery(c));

@@ -156,6 +156,10 @@ struct Ctx {
   int nptq = 0, nq = 0;
   bool contex = false;
   int ireg = 0;  // regularisations since the last recontouring
+  // where a step-count run resumes: 0 = contours to rebuild from qr
+  // (caps.f90 recont), 1 = contours built, advect's start to run,
+  // 2 = inside advect's loop
+  int phase = 0;
   bool have_field = false;
   // scalars
   Scalars *dsc = nullptr;   // device

@@ -672,7 +672,19 @@ int surgery(Ctx &c) {
     }
   }
   if (npt > c.cap) {
-    set_error("surgery: node count exceeds the context capacity");
+    // The reconnections above already rewired xq, yq and nextq in place, so
+    // the store no longer matches its contour table: the context drops its
+    // state and every step call reports HYD_ESTATE until a new field or
+    // contour set is loaded.
+    c.have_field = false;
+    c.nptq = 0;
+    c.nq = 0;
+    c.i1q.clear();
+    c.i2q.clear();
+    c.npq.clear();
+    c.indq.clear();
+    set_error("surgery: node count exceeds the context capacity; the context's contours are "
+              "dropped (set_field or set_contours to continue)");
     return HYD_ECAPACITY;
   }
   const int nqn = (int)np.size();

@@ -210,10 +210,15 @@ def restart(h, outdir, iind):
 
 def savegrid(h, outdir, igrec):
     """evolution.f90:646-731 at the current time (no interpolation between
-    steps: the save time is the state's own t): appends to ene.asc,
+    steps: the save time is the state's own t), after an inversion of the
+    current state: appends to ene.asc,
     norms.asc and spectra.asc and writes record igrec+1 of qq.r4.  Needs the
     plain precision (energy reads the FP64 fields)."""
     os.makedirs(outdir, exist_ok=True)
+    # the fields at t: after an advance, qq/uu/vv/pp still hold the last RK4
+    # stage's; the reference inverts before every save (evolution.f90:87-88,
+    # and 193 ahead of 203 inside advance)
+    h.inversion()
     t = h.scalar("t")
     qqs = h.field("qq")
     ekin, epot = h.energy()

@@ -150,6 +150,16 @@ int hyd_set_field(hyd_ctx *ctx, const double *qq, double t0);
 int hyd_set_contours(hyd_ctx *ctx, int32_t nptq, int32_t nq, const double *xq, const double *yq,
                      const int32_t *nextq, const int32_t *i1q, const int32_t *i2q,
                      const int32_t *npq, const int32_t *indq, double qjump);
+/* The checks hyd_set_contours makes, without a context or a device (a host
+   can validate a restart file before it opens one): contours numbered
+   consecutively from node 1 (i1q, i2q, npq as common.f90:78-88 rebuilds
+   them), levels ascending in indq (contours.f90:372-394 relies on it),
+   nextq one closed cycle through each contour's nodes, every node inside
+   [xmin,xmax) x [ymin,ymax).  HYD_EARG, with the reason in
+   hyd_last_error(), when one fails. */
+int hyd_check_contours(const hyd_params *p, int32_t nptq, int32_t nq, const double *xq,
+                       const double *yq, const int32_t *nextq, const int32_t *i1q,
+                       const int32_t *i2q, const int32_t *npq, const int32_t *indq);
 int hyd_get_counts(hyd_ctx *ctx, int32_t *nptq, int32_t *nq);
 int hyd_get_contours(hyd_ctx *ctx, double *xq, double *yq, int32_t *nextq, int32_t *i1q,
                      int32_t *i2q, int32_t *npq, int32_t *indq);

@@ -78,6 +78,9 @@ typedef struct hyo_model {
   double t, dt, dt2, dt3, dt6, twist, dtmax, tinit;
   int contex;
   int ireg;
+  /* where a step-count run resumes: 0 = contours to rebuild (recont),
+   * 1 = contours built, advect's start to run, 2 = inside advect's loop */
+  int phase;
   /* diagnostics of the last adapt call */
   double zzrms, zzmax, uumax, dtacc;
   int err;
@@ -119,6 +122,7 @@ int hyo_renode(const hyo_model *m, const double *xd, const double *yd, int npd,
 
 /* ---- congen.f90 ---------------------------------------------------------- */
 int hyo_recontour(hyo_model *m, double *qq);
+void hyo_advect_start(hyo_model *m);
 
 /* ---- evolution.f90 ------------------------------------------------------- */
 int hyo_init_from_field(hyo_model *m, const double *qq_init, double t0);

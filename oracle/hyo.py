@@ -77,7 +77,8 @@ def lib():
         L.hyo_renode.argtypes = [C.c_void_p, _D, _D, C.c_int, _D, _D, C.POINTER(C.c_int)]
         L.hyo_recontour.argtypes = [C.c_void_p, _D]
         L.hyo_init_from_field.argtypes = [C.c_void_p, _D, C.c_double]
-        for fn in ("hyo_evolution_init", "hyo_prepare", "hyo_inversion", "hyo_adapt"):
+        for fn in ("hyo_evolution_init", "hyo_advect_start", "hyo_prepare", "hyo_inversion",
+                   "hyo_adapt"):
             getattr(L, fn).argtypes = [C.c_void_p]
         L.hyo_source.argtypes = [C.c_void_p, _D, _D, C.c_int]
         L.hyo_advance.argtypes = [C.c_void_p]
@@ -137,7 +138,7 @@ class Oracle:
         return self.L.hyo_get_d(self.m, name.encode())
 
     def set(self, name, v):
-        if isinstance(v, (int, np.integer)) and name in ("nq", "nptq", "contex", "ireg"):
+        if isinstance(v, (int, np.integer)) and name in ("nq", "nptq", "contex", "ireg", "phase"):
             self.L.hyo_set_i(self.m, name.encode(), int(v))
         else:
             self.L.hyo_set_d(self.m, name.encode(), float(v))
@@ -160,6 +161,7 @@ class Oracle:
         self.set("nptq", n)
         self.set("nq", nc)
         self.set("qjump", c["qjump"])
+        self.set("phase", 0)  # a restart recontours (caps.f90 recont) first
 
     # -- routines ----------------------------------------------------------
     def ptospc(self, r):
@@ -229,6 +231,19 @@ class Oracle:
 
     def evolution_init(self):
         self.L.hyo_evolution_init(self.m)
+
+    def advect_start(self):
+        """evolution.f90:44-51: init, twist = 0, ireg = 0 (run resumes
+        inside the advect loop afterwards)."""
+        self.L.hyo_advect_start(self.m)
+
+    def prepare(self):
+        """evolution.f90:135-162: qs folded, qd = qs - qc, qr = spctop(qd)."""
+        self.L.hyo_prepare(self.m)
+
+    def inversion(self):
+        """evolution.f90:494-520."""
+        self.L.hyo_inversion(self.m)
 
     def advance(self):
         rc = self.L.hyo_advance(self.m)
@@ -422,6 +437,14 @@ class Ref:
         a, b = C.c_double(), C.c_double()
         run_big_stack(self.L.ref_norms, qq, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def contint(self, qq, n):
+        """generic.f90:66-79: (max(qq) - min(qq)) / n."""
+        self.L.ref_contint.argtypes = [_D, C.c_int, C.POINTER(C.c_double)]
+        dq = C.c_double()
+        run_big_stack(self.L.ref_contint, np.ascontiguousarray(qq, dtype=np.float64), int(n),
+                      C.byref(dq))
+        return dq.value
 
     def kmax(self):
         k = C.c_int()

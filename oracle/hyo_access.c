@@ -45,7 +45,7 @@ void hyo_set_d(hyo_model *m, const char *name, double v) {
 long hyo_get_i(const hyo_model *m, const char *name) {
 #define I(n) \
   if (!strcmp(name, #n)) return (long)m->n;
-  I(nx) I(ny) I(nq) I(nptq) I(npm) I(nm) I(contex) I(ireg) I(err) I(nxf) I(nyf) I(nxu) I(nyu)
+  I(nx) I(ny) I(nq) I(nptq) I(npm) I(nm) I(contex) I(ireg) I(phase) I(err) I(nxf) I(nyf) I(nxu) I(nyu)
   I(kmax) I(kmaxred)
 #undef I
   return -1;
@@ -54,6 +54,6 @@ long hyo_get_i(const hyo_model *m, const char *name) {
 void hyo_set_i(hyo_model *m, const char *name, long v) {
 #define I(n) \
   if (!strcmp(name, #n)) { m->n = (int)v; return; }
-  I(nq) I(nptq) I(contex) I(ireg)
+  I(nq) I(nptq) I(contex) I(ireg) I(phase)
 #undef I
 }

@@ -282,6 +282,7 @@ int hyo_recontour(hyo_model *m, double *qq) {
     double qql1 = m->garea * s;
     if (qql1 < small) {
       free(qa);
+      m->phase = 1;
       return HYO_OK;
     }
     if (m->beffect)
@@ -321,6 +322,7 @@ int hyo_recontour(hyo_model *m, double *qq) {
     }
     m->nq = w.na;
     m->nptq = w.npta;
+    m->phase = 1;
   } else {
     m->err = status;
   }

@@ -65,6 +65,7 @@ int hyo_init_from_field(hyo_model *m, const double *qq_init, double t0) {
   m->tinit = t0;
   m->t = t0;
   m->twist = 0.0;
+  m->phase = 0;
   return HYO_OK;
 }
 
@@ -94,6 +95,16 @@ void hyo_prepare(hyo_model *m) {
   }
   hyo_spctop(m, m->qd, m->qr);
   free(qa);
+  m->phase = 0;
+}
+
+/* Start of advect, evolution.f90:44-51: init, twist = 0, no regularisations
+ * counted yet; the run is then inside advect's loop. */
+void hyo_advect_start(hyo_model *m) {
+  hyo_evolution_init(m);
+  m->twist = 0.0;
+  m->ireg = 0;
+  m->phase = 2;
 }
 
 /* evolution.f90:494-520 */
@@ -344,14 +355,15 @@ int hyo_advect_step(hyo_model *m) {
   return 0;
 }
 
-/* caps.f90:83-91 loop driven by a step count instead of tsim. */
+/* caps.f90:83-91 loop driven by a step count instead of tsim.  A run
+ * resumes where the previous one stopped (phase): inside advect's loop it
+ * continues the loop, so run(a) then run(b) equals run(a + b); recont
+ * (recontour of qr) happens only at the start of a cycle. */
 int hyo_run(hyo_model *m, int nsteps) {
   int done = 0;
   while (done < nsteps) {
-    if (hyo_recontour(m, m->qr)) return -1;
-    hyo_evolution_init(m);
-    m->twist = 0.0;
-    m->ireg = 0;
+    if (m->phase == 0 && hyo_recontour(m, m->qr)) return -1;
+    if (m->phase <= 1) hyo_advect_start(m);
     while (done < nsteps) {
       int r = hyo_advect_step(m);
       if (r < 0) return -1;

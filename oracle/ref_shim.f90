@@ -191,6 +191,13 @@ subroutine ref_norms(qq, ql1, ql2) bind(c, name='ref_norms')
   call l2norm(qq, ql2)
 end subroutine
 
+! generic.f90:66-79
+subroutine ref_contint(qq, n, dq) bind(c, name='ref_contint')
+  real(c_double):: qq(ny,nx), dq
+  integer(c_int), value:: n
+  call contint(qq, n, dq)
+end subroutine
+
 subroutine ref_kmax(k) bind(c, name='ref_kmax')
   integer(c_int):: k
   k = kmax

@@ -78,10 +78,24 @@ def main():
     s2 = r.contours()
     for k, v in s2.items():
         out["sg_" + k] = v
+    # contint, generic.f90:66-79 (the contour interval of the initial field)
+    out["contint"] = np.float64(r.contint(qq, NCONTQ))
+    # recontouring with contours present (congen.f90:40-61: con2ugrid of the
+    # contours, congen.f90:338-441, plus the bilinear residual): the sheared
+    # contours above and a seeded residual of about half a contour interval
+    res = random_pv(NG, seed=77)
+    res = 0.5 * qjump * res / np.abs(res).max()
+    out["rx_res"] = res
+    r.set_contours(sc)
+    r.recontour(res)
+    rx = r.contours()
+    for k, v in rx.items():
+        out["rx_" + k] = v
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"ref_ng{NG}.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes;", len(c["x"]), "nodes,", len(c["i1q"]), "contours;",
-          "surgery ->", len(s2["x"]), len(s2["i1q"]))
+          "surgery ->", len(s2["x"]), len(s2["i1q"]), "; recontour with contours ->",
+          len(rx["x"]), len(rx["i1q"]))
 
 
 if __name__ == "__main__":

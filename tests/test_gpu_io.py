@@ -73,6 +73,9 @@ def test_savegrid_matches_oracle(tmp_path):
     assert bits_equal(h.table("alk")[1:kmax + 1], o.d("alk", NG + 1)[1:kmax + 1])
     out = str(tmp_path / "run")
     assert io.savegrid(h, out, 0) == 1
+    o.inversion()   # savegrid saves the state at t (evolution.f90:87-88)
+    for name in ("qq", "uu", "vv", "pp"):
+        assert bits_equal(h.field(name), o.d(name)), name
     t = h.scalar("t")
     ek, ep = o.energy()
     assert open(os.path.join(out, "ene.asc")).read() == io.fmt_ene(t, ek, ep) + "\n"

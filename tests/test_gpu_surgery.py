@@ -80,3 +80,35 @@ def test_run_with_surgery_and_recontouring_bitwise():
     for name in ("qs", "qd", "qc"):
         assert bits_equal(h.field(name), o.d(name)), name
     h.close()
+
+
+@pytest.mark.parametrize("split", [37, 160])
+def test_run_in_two_calls_bitwise(split):
+    """run(a) then run(b) equals one run of a+b steps, on the device and in
+    the oracle alike (the second call resumes inside advect's loop; split
+    160 ends the first call just before the 20th regularisation hands over
+    to recontouring), and a batch run split the same way agrees too."""
+    from hydra_amd import Hydra
+    from hydra_amd.ic import random_pv
+    from oracle.hyo import Oracle
+    ng, n = 32, 170
+    qq = random_pv(ng, seed=7)
+    o = Oracle(ng)
+    o.init_from_field(qq)
+    o.run(n)
+    h = Hydra(ng)
+    h.set_field(qq)
+    h.run(split)
+    h.run(n - split)
+    assert h.scalar("t") == o.getd("t")
+    _same_contours(h.contours(), o.contours())
+    for name in ("qs", "qd", "qc"):
+        assert bits_equal(h.field(name), o.d(name)), name
+    h.close()
+    b = Hydra(ng)
+    b.set_field(qq)
+    Hydra.run_batch([b], split)
+    Hydra.run_batch([b], n - split)
+    assert b.scalar("t") == o.getd("t")
+    _same_contours(b.contours(), o.contours())
+    b.close()

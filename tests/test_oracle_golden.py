@@ -83,3 +83,23 @@ def test_surgery(gold, oracle):
     ref = contours_of(gold, "sg_")
     for k in ref:
         assert bits_equal(out[k], ref[k]), k
+
+
+def test_contint(gold, oracle):
+    """generic.f90:66-79 through the oracle's set-up (caps.f90:166)."""
+    oracle.init_from_field(gold["qq"])
+    # init_from_field divides by the oracle's ncontq (20 here, as the golden run)
+    assert bits_equal(np.float64(oracle.getd("qjump")), gold["contint"])
+
+
+def test_recontour_with_contours(gold, oracle):
+    """congen.f90:40-61 (nq > 0): the contours gridded on the ultra-fine
+    grid (con2ugrid, congen.f90:338-441) plus the bilinear residual, then
+    new contours."""
+    c = contours_of(gold, "rc_")
+    oracle.set_contours(dict(c, x=gold["sg_in_x"], y=gold["sg_in_y"], qjump=float(gold["qjump"])))
+    oracle.recontour(gold["rx_res"])
+    out = oracle.contours()
+    ref = contours_of(gold, "rx_")
+    for k in ref:
+        assert bits_equal(out[k], ref[k]), k

@@ -7,17 +7,24 @@ import pytest
 from tests.helpers import TANGLES, bits_equal, tangle_contours
 
 NG = 32
+_PAIRS = {}
+
+
+def make_pair(ng):
+    from oracle.hyo import Oracle, Ref, ref_available
+    if not ref_available(ng):
+        pytest.skip("reference build oracle/_ref not present")
+    if ng not in _PAIRS:
+        r = Ref(ng)
+        o = Oracle(ng)
+        o.set("d4small", float(r.consts()[2]))
+        _PAIRS[ng] = (o, r)
+    return _PAIRS[ng]
 
 
 @pytest.fixture(scope="module")
 def pair():
-    from oracle.hyo import Oracle, Ref, ref_available
-    if not ref_available(NG):
-        pytest.skip("reference build oracle/_ref not present")
-    r = Ref(NG)
-    o = Oracle(NG)
-    o.set("d4small", float(r.consts()[2]))
-    return o, r
+    return make_pair(NG)
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
@@ -76,3 +83,57 @@ def test_surgery_tangles(pair, seed, ncirc, rmin, rmax):
     a, b = o.contours(), r.contours()
     for k in ("x", "y", "nextq", "i1q", "i2q", "npq", "indq"):
         assert bits_equal(a[k], b[k]), k
+
+
+def same_contours(a, b):
+    return all(bits_equal(a[k], b[k]) for k in ("x", "y", "nextq", "i1q", "i2q", "npq", "indq"))
+
+
+@pytest.mark.parametrize("ng", [32, 64])
+def test_recontour_with_contours(ng):
+    """congen.f90:40-61 with contours present: con2ugrid (congen.f90:338-441,
+    its own index conventions: int() without +1, its own ixbeg) plus the
+    bilinear residual, then ugrid2con.  Two inputs: freshly made contours
+    sheared by a smooth displacement with a seeded residual, and the
+    contours and residual qr = spctop(qs - qc) that prepare
+    (evolution.f90:135-162) leaves after the oracle advanced the flow."""
+    from oracle.hyo import Oracle
+    from tests.helpers import oracle_state
+    o, r = make_pair(ng)
+    src, qq, c0 = oracle_state(ng)
+    qj = src.getd("qjump")
+    x = c0["x"] + 0.04 * np.sin(2 * c0["y"] + 1.0)
+    y = c0["y"] + 0.04 * np.cos(3 * c0["x"])
+    L = 2 * np.pi
+    x = (1 - 1e-12) * (x - L * np.trunc(x / np.pi))
+    y = (1 - 1e-12) * (y - L * np.trunc(y / np.pi))
+    res = np.random.default_rng(ng).standard_normal((ng, ng)) * 0.3 * qj
+    cases = [(dict(c0, x=x, y=y, qjump=qj), res)]
+    # advected: 6 RK4 steps from the seeded field, then prepare
+    a = Oracle(ng)
+    a.init_from_field(qq)
+    a.recontour(a.d("qr").copy())
+    a.advect_start()
+    for _ in range(6):
+        a.advance()
+    a.prepare()
+    cases.append((dict(a.contours(), qjump=a.getd("qjump")), a.d("qr").copy()))
+    for c, qr in cases:
+        o.set_contours(c)
+        r.set_contours(c)
+        o.recontour(qr)
+        r.recontour(qr)
+        assert o.get("nq") > 0
+        assert same_contours(o.contours(), r.contours())
+
+
+@pytest.mark.parametrize("ng", [32, 64])
+def test_contint(ng):
+    """generic.f90:66-79 against the oracle's set-up, caps.f90:166."""
+    from oracle.hyo import Oracle
+    _, r = make_pair(ng)
+    for seed in (1, 2):
+        qq = np.random.default_rng(seed).standard_normal((ng, ng)) * 7.0
+        o = Oracle(ng, ncontq=80)
+        o.init_from_field(qq)
+        assert bits_equal(np.float64(o.getd("qjump")), np.float64(r.contint(qq, 80)))
