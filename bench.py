@@ -5,17 +5,25 @@
 Workload: the doubly periodic 2D Euler CAPS step at ng=1024 (contour grid
 16ng for recontouring, 4ng inside con2grid), a seeded random PV field with
 1D spectrum ~ k^3 exp(-2 (k/6)^2) and rms 4*pi, contoured into 80 levels
-on the GPU (congen), then K RK4 steps (evolution.f90 advance) of the
+on the GPU (congen), surgery scale dm = 0.01*(2*pi/ng) (ell = glx; the
+reference's own dm = 0.0625*glx is timed beside it), then K passes of the
+advect loop (evolution.f90:55-80: surgery when the twist exceeds 2.5,
+recontouring after 20 regularisations, one RK4 advance) on the
 device-resident state.  One node-update = one contour node advanced through
 one full RK4 step (four stages: con2grid, FFT inversion, sources, velint,
 node and spectral updates).  Inputs are resident in HBM before timing.
 
-Besides the JSON line the script prints nothing on stdout.  The roofline
-entry is for the kernel with the largest share of the timed region, timed
-with per-launch HIP events on the library's stream (in steps run on one
-stream, so that each launch runs alone); cpu_baseline is the oracle (C
-restatement of the reference, one core) on a bounded sample of the same
-state.
+Besides the JSON line the script prints nothing on stdout.  Roofline
+entries count algorithmic bytes as the step's dataflow reads and writes
+each logical field once per use (a 2D transform: its field once in and once
+out, shared by its two 1D passes; node arrays at FP64):
+  * roofline: the kernel with the largest share of the step (the inverse
+    FFT pass), timed per launch with HIP events on the library's stream in
+    steps run on one stream, so that each launch runs alone;
+  * step_roofline: 137 s_g N + 336 P bytes per RK4 step (N = ng^2, s_g = 8)
+    over the measured ms per step.
+cpu_baseline is the oracle (C restatement of the reference, gcc -O3, one
+core) on a bounded sample of the same state.
 """
 from __future__ import annotations
 
@@ -149,14 +157,130 @@ def stats_summary(db, out, title, command):
         f.write("\n".join(lines) + "\n")
 
 
+def step_bytes(ng, nodes, s_g=8):
+    """Algorithmic HBM bytes of one RK4 step: 137 field passes of s_g-byte
+    grid elements (33 per stage x 4, adapt 5) and 42 FP64 node words."""
+    return 137 * s_g * ng * ng + 336 * nodes
+
+
+def make_flow(ng, qq, precision=0, ell_cells=0.0, ncontq=80):
+    from hydra_amd import Hydra
+    h = Hydra(ng, ncontq=ncontq, precision=precision, ell_cells=ell_cells,
+              node_capacity=max(16 * ng * ng, 1 << 22))
+    t0 = time.perf_counter()
+    h.set_field(qq)
+    h.recontour()
+    h.evolution_init()
+    h.sync()
+    return h, time.perf_counter() - t0
+
+
+def advect_steps(h, k):
+    """k passes of the advect loop (evolution.f90:55-80): surgery when the
+    twist exceeds 2.5, recontouring after 20 regularisations (caps.f90:
+    83-91), then one RK4 advance.  Returns the node-updates."""
+    updates = 0
+    for _ in range(k):
+        h.run(1)
+        updates += h.counts()[0]
+    return updates
+
+
+def timed_run(h, steps, warmup):
+    for _ in range(warmup):
+        advect_steps(h, 1)
+    h.sync()
+    h.phase_times(reset=True)
+    t0 = time.perf_counter()
+    updates = advect_steps(h, steps)
+    h.sync()
+    wall = time.perf_counter() - t0
+    return updates, wall, h.phase_times()
+
+
+def fft_roofline(h, steps, ng):
+    """Per-launch timing of the FFT passes (the step's largest kernel by
+    time) and the step's breakdown by kernel group."""
+    kr = min(steps, 10)
+    h.set_timing(1)
+    advect_steps(h, kr)
+    h.sync()
+    recs = h.kernel_times()
+    h.set_timing(0)
+    tot, cnt, pbytes = {}, {}, {}
+    for name, ms, by in recs:
+        tot[name] = tot.get(name, 0.0) + ms
+        cnt[name] = cnt.get(name, 0) + 1
+        if name.startswith("k:"):
+            pbytes.setdefault(name, []).append(by)
+    nb = 2
+    h.set_timing(2)
+    advect_steps(h, nb)
+    h.sync()
+    gt, gc, gb = {}, {}, {}
+    for name, ms, by in h.kernel_times():
+        gt[name] = gt.get(name, 0.0) + ms
+        gc[name] = gc.get(name, 0) + 1
+        gb[name] = gb.get(name, 0) + by
+    h.set_timing(0)
+    groups = {}
+    for name, ms in sorted(gt.items(), key=lambda kv: -kv[1]):
+        log(f"  {name:16s} {ms / nb:8.3f} ms/step  {gc[name] // nb} calls/step")
+        groups[name] = {"ms_per_step": round(ms / nb, 4), "calls_per_step": gc[name] // nb,
+                        "algorithmic_gbs": round(gb[name] / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+    dom = max(pbytes, key=lambda k: tot[k])
+    ev_ms = h.event_overhead(200)
+    avg_ms = tot[dom] / cnt[dom]
+    # a launch is one 1D pass of F fields; the 2D transform's fields count
+    # once in and once out, shared by its two passes: half the pass's bytes
+    pass_bytes = float(np.mean(pbytes[dom]))
+    alg = pass_bytes / 2
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    ksym = KERNEL_SYMBOL.get(dom, dom).replace("{N}", str(ng))
+    traffic = traffic_from_profile(ksym, pass_bytes)
+    roof = {"bound": "hbm", "kernel": ksym, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": int(alg),
+            "counting": "2D transform = its fields once in + once out, half per 1D-pass launch",
+            "avg_launch_ms": round(avg_ms, 4), "empty_launch_ms": round(ev_ms, 4),
+            "launches_per_step": cnt[dom] / kr}
+    if traffic is not None:
+        roof["traffic_over_algorithmic"] = round(traffic / alg, 3)
+        roof["hbm_gbs_measured"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+    return roof, groups
+
+
+def regularisation_costs(h):
+    """One surgery and one prepare + recontouring of the current state
+    (the nq > 0 branch of congen.f90:40-61) timed on their own, then the
+    loop restarted (advect's start): the amortised extras of a run."""
+    h.phase_times(reset=True)
+    p0 = h.counts()[0]
+    h.surgery()
+    h.prepare()
+    h.recontour()
+    h.evolution_init()
+    h.sync()
+    pt = h.phase_times(reset=True)
+    return {"nodes": p0, "surgery_s": round(pt["surgery"][1], 4),
+            "recontour_with_contours_s": round(pt["recontour"][1], 4),
+            "nodes_after": h.counts()[0]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ng", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=12345)
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--ell-cells", type=float, default=1.0,
+                    help="surgery/renode scale ell in grid lengths: 1.0 gives dm = 0.01 glx "
+                         "(the workload's), 6.25 the reference's dm = 0.0625 glx")
+    ap.add_argument("--ref-dm-steps", type=int, default=100,
+                    help="steps of the same loop with the reference's dm (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision timing and its error against the FP64 step")
@@ -189,110 +313,61 @@ def main():
               "(use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    from hydra_amd import Hydra
     from hydra_amd.ic import random_pv
 
     ng = args.ng
     qq = random_pv(ng, seed=args.seed)
-    h = Hydra(ng, ncontq=80, precision=args.precision)
-    t0 = time.perf_counter()
-    h.set_field(qq)
-    h.recontour()
-    h.evolution_init()
-    h.sync()
-    t_setup = time.perf_counter() - t0
+    h, t_setup = make_flow(ng, qq, args.precision, args.ell_cells)
     P, nq = h.counts()
     tuning = h.tuning()
-    log(f"setup: ng={ng} contours={nq} nodes={P} recontour+init {t_setup:.2f}s")
-    N = ng * ng
-
-    def step():
-        """One pass of the advect loop (evolution.f90:55-80): surgery when the
-        twist exceeds 2.5, recontouring after 20 regularisations
-        (caps.f90:83-91), then one RK4 advance.  Returns the nodes advanced."""
-        while h.advect_step() == 1:
-            h.recontour()
-            h.evolution_init()
-        return h.counts()[0]
-
-    for _ in range(args.warmup):
-        step()
-    h.sync()
-    # timed region: K steps, no per-kernel events (the mixed-precision step
-    # replays its captured graph here)
-    h.sync()
-    t0 = time.perf_counter()
-    updates = 0
-    for _ in range(args.steps):
-        updates += step()
-    h.sync()
-    wall = time.perf_counter() - t0
+    dm = 0.04 * (args.ell_cells if args.ell_cells > 0 else 6.25) * (2 * np.pi / ng) / 4
+    log(f"setup: ng={ng} ell={args.ell_cells} glx contours={nq} nodes={P} "
+        f"recontour+init {t_setup:.2f}s")
+    updates, wall, ph = timed_run(h, args.steps, args.warmup)
     P_end, nq_end = h.counts()
     ms_step = 1e3 * wall / args.steps
     value = updates / wall
-    log(f"timed: {args.steps} steps {wall:.3f}s, nodes {P} -> {P_end}, contours {nq} -> {nq_end}")
-
-    # kernel durations: HIP events around each launch of the single kernels
-    # ("k:" records) on the library stream, over the next steps (eager)
-    kr = min(args.steps, 10)
-    h.set_timing(1)
-    for _ in range(kr):
-        step()
-    h.sync()
-    recs = h.kernel_times()
-    h.set_timing(0)
-    tot = {}
-    cnt = {}
-    for name, ms, by in recs:
-        tot[name] = tot.get(name, 0.0) + ms
-        cnt[name] = cnt.get(name, 0) + 1
-    # breakdown by kernel group (after the timed region, all records on)
-    nb = 2
-    h.set_timing(2)
-    for _ in range(nb):
-        step()
-    h.sync()
-    gt, gc = {}, {}
-    for name, ms, by in h.kernel_times():
-        gt[name] = gt.get(name, 0.0) + ms
-        gc[name] = gc.get(name, 0) + 1
-    h.set_timing(0)
-    for name, ms in sorted(gt.items(), key=lambda kv: -kv[1]):
-        log(f"  {name:16s} {ms / nb:8.3f} ms/step  {gc[name] // nb} calls/step")
-    # roofline: the single kernel ("k:" records, one record per launch) with
-    # the largest share of the timed region; its algorithmic bytes per launch
-    # are recorded with each launch
-    kbytes = {}
-    for name, ms, by in recs:
-        if name.startswith("k:"):
-            kbytes.setdefault(name, []).append(by)
-    dom = max(kbytes, key=lambda k: tot[k])
-    # for reference: the interval the same per-launch events report for an
-    # empty kernel (launch and stamping cost; not subtracted)
-    ev_ms = h.event_overhead(200)
-    avg_ms = tot[dom] / cnt[dom]
-    alg = float(np.mean(kbytes[dom]))
-    achieved = alg / (avg_ms * 1e-3) / 1e9
-    ksym = KERNEL_SYMBOL.get(dom, dom).replace("{N}", str(ng))
-    roofline = {"bound": "hbm", "kernel": ksym, "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profile(ksym, alg),
-                "avg_launch_ms": round(avg_ms, 4), "empty_launch_ms": round(ev_ms, 4),
-                "launches_per_step": cnt[dom] / kr,
-                "algorithmic_bytes_per_launch": int(alg)}
-
+    p_avg = updates / args.steps
+    log(f"timed: {args.steps} steps {wall:.3f}s, nodes {P} -> {P_end}, contours {nq} -> {nq_end}, "
+        f"surgeries {ph['surgery']}, recontours {ph['recontour']}")
+    s_g = 8 if args.precision == 0 else 4
+    bstep = step_bytes(ng, p_avg, s_g)
+    step_roof = {"bound": "hbm", "bytes_per_step": int(bstep),
+                 "formula": "137*s_g*N + 336*P (N=ng^2, s_g=%d, P=mean nodes)" % s_g,
+                 "achieved": round(bstep / (ms_step * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "frac": round(bstep / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "ms_per_step_at_peak": round(bstep / (HBM_PEAK_GBS * 1e9) * 1e3, 4)}
+    roofline, groups = fft_roofline(h, args.steps, ng)
+    run = {"setup_recontour_init_s": round(t_setup, 4), "timed_s": round(wall, 4),
+           "surgeries": ph["surgery"][0], "surgery_s": round(ph["surgery"][1], 4),
+           "recontours": ph["recontour"][0], "recontour_s": round(ph["recontour"][1], 4),
+           "whole_run_s": round(t_setup + wall, 4),
+           "whole_run_node_updates_per_s": round(updates / (t_setup + wall), 1)}
     if args.copy_calib:
         copy_calibration(h)
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(h, qq, ng, args.cpu_steps)
+            cpu = cpu_baseline(h, qq, ng, args.cpu_steps, args.ell_cells)
         except Exception as e:  # the baseline is a report, not the product
             log(f"cpu baseline failed: {e}")
+    regs = regularisation_costs(h)
+    log(f"regularisation: {regs}")
     h.close()
     mixed = None
     if args.precision == 0 and not args.no_mixed:
-        mixed = mixed_report(qq, ng, args.steps, args.warmup, args.err_steps, value)
+        mixed = mixed_report(qq, ng, args.steps, args.warmup, args.err_steps, value,
+                             args.ell_cells)
+    refdm = None
+    if args.ref_dm_steps > 0 and args.ell_cells != 6.25:
+        g, ts = make_flow(ng, qq, args.precision, 6.25)
+        u, w, phr = timed_run(g, args.ref_dm_steps, args.warmup)
+        refdm = {"ell_cells": 6.25, "dm": "0.0625*glx (contours.f90:38)", "nodes": g.counts()[0],
+                 "steps": args.ref_dm_steps, "value": round(u / w, 1),
+                 "ms_per_step": round(1e3 * w / args.ref_dm_steps, 4),
+                 "surgeries": phr["surgery"][0], "setup_s": round(ts, 3)}
+        log(f"reference dm: {refdm}")
+        g.close()
     out = {
         "metric": "contour node-updates/s at ng=1024: plain FP64 vs FP32/batched at matched error",
         "value": round(value, 1),
@@ -307,78 +382,63 @@ def main():
         "dtype": "f64" if args.precision == 0 else "f32/f64",
         "data": "synthetic",
         "config": {"workload": ("plain FP64" if args.precision == 0 else "mixed FP32-grid/FP64-state")
-                   + f" CAPS RK4 step, 2D Euler, ng={ng}, 80 PV levels, "
-                               f"seeded k^3 exp(-2(k/6)^2) field (rms 4pi)",
-                   "ng": ng, "nodes": P, "nodes_end": P_end, "contours": nq,
+                   + f" CAPS advect loop (RK4 step + surgery + recontouring), 2D Euler, ng={ng}, "
+                     f"80 PV levels, seeded k^3 exp(-2(k/6)^2) field (rms 4pi), "
+                     f"dm={dm / (2 * np.pi / ng):.4g}*glx",
+                   "ng": ng, "nodes": P, "nodes_end": P_end, "nodes_mean": round(p_avg, 1),
+                   "contours": nq, "ell_cells": args.ell_cells,
                    "precision": "plain-fp64" if args.precision == 0 else "mixed-fp32-grid",
                    "fft_batch_autotuned": tuning,
                    "dt": "adaptive (reference adapt)", "seed": args.seed},
         "roofline": roofline,
+        "step_roofline": step_roof,
+        "run": run,
+        "regularisation": regs,
+        "kernel_groups": groups,
         "cpu_baseline": cpu,
     }
     if mixed is not None:
         out["mixed"] = mixed
+    if refdm is not None:
+        out["reference_dm"] = refdm
     if args.ensemble > 0:
         out["ensemble"] = ensemble_report(args.ensemble, args.ensemble_steps, args.seed)
     print(json.dumps(out), flush=True)
 
 
-def mixed_report(qq, ng, steps, warmup, err_steps, plain_value):
+def mixed_report(qq, ng, steps, warmup, err_steps, plain_value, ell_cells):
     """The mixed-precision step (FP32 grids and transforms, FP64 state and
     nodes) on the same workload: its throughput over the same loop, and its
     error against the FP64 step after err_steps RK4 steps from the same
-    contours (no surgery in between, so the nodes correspond one to one):
-    RMS and maximum node displacement (periodic, domain units) and the
-    largest spectral PV difference relative to the largest value."""
-    from hydra_amd import Hydra
-    h = Hydra(ng, ncontq=80, precision=1)
-    h.set_field(qq)
-    h.recontour()
-    h.evolution_init()
-
-    def step():
-        while h.advect_step() == 1:
-            h.recontour()
-            h.evolution_init()
-        return h.counts()[0]
-
-    for _ in range(warmup):
-        step()
-    h.sync()
-    t0 = time.perf_counter()
-    updates = 0
-    for _ in range(steps):
-        updates += step()
-    h.sync()
-    wall = time.perf_counter() - t0
+    contours (no surgery in between, so the nodes correspond one to one),
+    evaluated with the FP64 diagnostics (hydra_amd.diagnostics)."""
+    from hydra_amd.diagnostics import error_vs_fp64, fp64_diagnostics
+    h, _ = make_flow(ng, qq, 1, ell_cells)
+    updates, wall, ph = timed_run(h, steps, warmup)
     h.close()
-    runs = []
-    for prec in (0, 1):
-        g = Hydra(ng, ncontq=80, precision=prec)
-        g.set_field(qq)
-        g.recontour()
-        g.evolution_init()
-        for _ in range(err_steps):
-            g.advance()
-        g.sync()
-        runs.append((g.contours(), g.field("qs")))
-        g.close()
-    (c64, qs64), (c32, qs32) = runs
-    same = all(np.array_equal(c64[k], c32[k]) for k in ("nextq", "i1q", "i2q", "npq", "indq"))
-    L = 2 * np.pi
-    d = np.r_[c64["x"] - c32["x"], c64["y"] - c32["y"]]
-    d = d - L * np.round(d / L)
-    log(f"mixed: {steps} steps {wall:.3f}s; after {err_steps} steps rms node displacement "
-        f"{np.sqrt(np.mean(d * d)):.3e}")
+    h64, _ = make_flow(ng, qq, 0, ell_cells)
+    c = h64.contours()
+    h32, _ = make_flow(ng, qq, 1, ell_cells)
+    h32.set_contours(c)
+    h32.evolution_init()
+    _, e0, z0 = fp64_diagnostics(h64)
+    h64.evolution_init()
+    for _ in range(err_steps):
+        h64.advance()
+        h32.advance()
+    err = error_vs_fp64(h64, h32, e0, z0)
+    err["steps"] = err_steps
+    h32.close()
+    h64.close()
+    log(f"mixed: {steps} steps {wall:.3f}s; after {err_steps} steps {err}")
     return {"value": round(updates / wall, 1), "unit": "node-updates/s",
             "ms_per_step": round(1e3 * wall / steps, 4),
             "speedup_vs_plain": round(updates / wall / plain_value, 3),
+            "surgeries": ph["surgery"][0],
             "dtype": "f32 grids/f64 nodes and state",
-            "error_vs_fp64": {"steps": err_steps, "topology_equal": bool(same),
-                              "rms_node_displacement": float(np.sqrt(np.mean(d * d))),
-                              "max_node_displacement": float(np.abs(d).max()),
-                              "qs_max_rel_diff": float(np.abs(qs64 - qs32).max()
-                                                       / np.abs(qs64).max())}}
+            "step_roofline_frac": round(step_bytes(ng, updates / steps, 4)
+                                        / (wall / steps) / 1e9 / HBM_PEAK_GBS, 4),
+            "error_vs_fp64": err}
 
 
 def ensemble_report(nflows, steps, seed):
@@ -431,36 +491,30 @@ def ensemble_report(nflows, steps, seed):
             "precision": "mixed-fp32-grid"}
 
 
-def cpu_baseline(h, qq, ng, nsteps):
-    """The oracle (C restatement, one core) advancing the same state for a
-    bounded number of steps; returns node-updates/s."""
-    import subprocess
+def cpu_baseline(h, qq, ng, nsteps, ell_cells):
+    """The oracle (C restatement, gcc -O3, one core) advancing the same
+    state through the same loop for a bounded number of steps; returns
+    node-updates/s."""
     from oracle.hyo import Oracle, build
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
         build()
     c = h.contours()
-    o = Oracle(ng)
+    o = Oracle(ng, ell_cells=ell_cells)
     o.init_from_field(qq)
     o.set_contours(c)
-    o.evolution_init()
-    o.set("twist", 0.0)
-    o.set("ireg", 0)
+    o.advect_start()
     P = len(c["x"])
     updates = 0
     t0 = time.perf_counter()
     for _ in range(nsteps):
-        while o.advect_step() == 1:
-            o.recontour(o.d("qr").copy())
-            o.evolution_init()
-            o.set("twist", 0.0)
-            o.set("ireg", 0)
+        o.run(1)
         updates += o.get("nptq")
     wall = time.perf_counter() - t0
     return {"value": round(updates / wall, 1), "unit": "node-updates/s", "cores": 1,
             "kind": "port",
             "sample": f"{nsteps} steps of the advect loop (surgery + RK4 advance) of the oracle "
-                      f"(C restatement of evolution.f90/contours.f90, gcc -O2, one thread) from "
-                      f"the same ng={ng} contours ({P} nodes), {wall:.1f} s"}
+                      f"(C restatement of evolution.f90/contours.f90, gcc -O3 -ffp-contract=off, "
+                      f"one thread) from the same ng={ng} contours ({P} nodes), {wall:.1f} s"}
 
 
 if __name__ == "__main__":

@@ -50,6 +50,7 @@ class HydParams(C.Structure):
         ("rtherm", C.c_double), ("rekman", C.c_double), ("cdamp", C.c_double), ("nnu", C.c_double),
         ("tgsave", C.c_double), ("tcsave", C.c_double), ("dt_fixed", C.c_double),
         ("node_capacity", C.c_int64), ("precision", C.c_int32), ("device", C.c_int32),
+        ("ell_cells", C.c_double),
     ]
 
 
@@ -95,6 +96,7 @@ def lib():
         L.hyd_renode.argtypes = [C.c_void_p, _D, _D, C.c_int32, _D, _D, C.c_int32,
                                  C.POINTER(C.c_int32)]
         L.hyd_run.argtypes = [C.c_void_p, C.c_int32]
+        L.hyd_phase_times.argtypes = [C.c_void_p, _D, C.c_int32]
         L.hyd_run_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int32]
         L.hyd_get_tuning.argtypes = [C.c_void_p, _I]
         L.hyd_copy_calibration.argtypes = [C.c_void_p, C.c_int64]
@@ -135,11 +137,11 @@ def _f64(a, shape=None):
 
 def _params(ng, ncontq=80, ellx=2 * np.pi, elly=None, kd=0.0, beta=0.0, rekman=0.0, cdamp=2.0,
             nnu=3.0, tgsave=1000.0, tcsave=1000.0, dt_fixed=0.0, node_capacity=0, device=0,
-            precision=0):
+            precision=0, ell_cells=0.0):
     return HydParams(ng=ng, ncontq=ncontq, ellx=ellx, elly=ellx if elly is None else elly, kd=kd,
                      beta=beta, rtherm=0.0, rekman=rekman, cdamp=cdamp, nnu=nnu, tgsave=tgsave,
                      tcsave=tcsave, dt_fixed=dt_fixed, node_capacity=node_capacity,
-                     precision=precision, device=device)
+                     precision=precision, device=device, ell_cells=ell_cells)
 
 
 def _contour_args(c):
@@ -162,13 +164,13 @@ class Hydra:
 
     def __init__(self, ng, ncontq=80, ellx=2 * np.pi, elly=None, kd=0.0, beta=0.0, rekman=0.0,
                  cdamp=2.0, nnu=3.0, tgsave=1000.0, tcsave=1000.0, dt_fixed=0.0,
-                 node_capacity=0, device=0, precision=0):
+                 node_capacity=0, device=0, precision=0, ell_cells=0.0):
         self.L = lib()
         self.ng = ng
         self.params = _params(ng, ncontq=ncontq, ellx=ellx, elly=elly, kd=kd, beta=beta,
                               rekman=rekman, cdamp=cdamp, nnu=nnu, tgsave=tgsave, tcsave=tcsave,
                               dt_fixed=dt_fixed, node_capacity=node_capacity, device=device,
-                              precision=precision)
+                              precision=precision, ell_cells=ell_cells)
         h = C.c_void_p()
         _check(self.L.hyd_init(C.byref(self.params), C.byref(h)))
         self.h = h
@@ -319,6 +321,13 @@ class Hydra:
 
     def evolution_init(self):
         _check(self.L.hyd_evolution_init(self.h))
+
+    def phase_times(self, reset=False):
+        """Surgery and recontouring calls and wall seconds so far:
+        {"surgery": (calls, s), "recontour": (calls, s)}."""
+        out = np.zeros(4)
+        _check(self.L.hyd_phase_times(self.h, out, int(bool(reset))))
+        return {"surgery": (int(out[0]), float(out[1])), "recontour": (int(out[2]), float(out[3]))}
 
     def inversion(self):
         _check(self.L.hyd_inversion(self.h))

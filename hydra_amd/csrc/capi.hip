@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <mutex>
 
 #include <hip/hip_ext.h>
@@ -322,6 +323,32 @@ static int d2h(Ctx &c, double *h, const double *d, int64_t n) {
   return HYD_OK;
 }
 
+// Surgery and recontouring with their wall time (each ends with the stream
+// drained) accumulated per context: the regularisation costs that a run
+// amortises over its steps.
+static double wall_now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+static int timed_surgery(Ctx &c) {
+  const double t0 = wall_now();
+  HYD_CHECK(surgery(c));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  c.phase_n[0] += 1;
+  c.phase_s[0] += wall_now() - t0;
+  return HYD_OK;
+}
+
+static int timed_recontour(Ctx &c) {
+  const double t0 = wall_now();
+  HYD_CHECK(recontour(c, c.qr));
+  HYD_HIP(hipStreamSynchronize(c.s));
+  c.phase_n[1] += 1;
+  c.phase_s[1] += wall_now() - t0;
+  return HYD_OK;
+}
+
 }  // namespace hyd
 
 using namespace hyd;
@@ -329,6 +356,21 @@ using namespace hyd;
 extern "C" {
 
 const char *hyd_last_error(void) { return g_err.c_str(); }
+
+int hyd_phase_times(hyd_ctx *h, double *out, int32_t reset) {
+  Ctx &c = *C(h);
+  if (out) {
+    out[0] = c.phase_n[0];
+    out[1] = c.phase_s[0];
+    out[2] = c.phase_n[1];
+    out[3] = c.phase_s[1];
+  }
+  if (reset) {
+    c.phase_n[0] = c.phase_n[1] = 0;
+    c.phase_s[0] = c.phase_s[1] = 0.0;
+  }
+  return HYD_OK;
+}
 
 int hyd_init(const hyd_params *p, hyd_ctx **ctx) { return create(p, ctx); }
 
@@ -778,7 +820,7 @@ int hyd_recontour(hyd_ctx *h) {
     set_error("recontour before set_field");
     return HYD_ESTATE;
   }
-  HYD_CHECK(recontour(c, c.qr));
+  HYD_CHECK(timed_recontour(c));
   c.phase = 1;
   return c2g_status(c);
 }
@@ -870,7 +912,7 @@ int hyd_advance(hyd_ctx *h) {
 int hyd_surgery(hyd_ctx *h) {
   Ctx &c = *C(h);
   c.graph_nptq = -1;  // state changed outside the captured step
-  HYD_CHECK(surgery(c));
+  HYD_CHECK(timed_surgery(c));
   HYD_HIP(hipStreamSynchronize(c.s));
   return HYD_OK;
 }
@@ -887,7 +929,7 @@ static int advect_step(Ctx &c) {
       c.phase = 0;
       return 1;
     }
-    HYD_CHECK(surgery(c));
+    HYD_CHECK(timed_surgery(c));
     HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
     HYD_HIP(hipStreamSynchronize(c.s));
     c.hsc->twist = c.hsc->twist - twistmax;
@@ -911,7 +953,7 @@ int hyd_advect_step(hyd_ctx *h) {
 // recont), then advect's start, or straight on inside advect's loop.
 static int resume_cycle(Ctx &c) {
   if (c.phase == 0) {
-    HYD_CHECK(recontour(c, c.qr));
+    HYD_CHECK(timed_recontour(c));
     c.phase = 1;
   }
   if (c.phase == 1) HYD_CHECK(advect_start(c));
@@ -982,7 +1024,7 @@ int hyd_run_batch(hyd_ctx **hs, int32_t n, int32_t nsteps) {
           c.phase = 0;  // recontour next round; this round has no advance
           continue;
         }
-        HYD_CHECK(surgery(c));
+        HYD_CHECK(timed_surgery(c));
         HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
         HYD_HIP(hipStreamSynchronize(c.s));
         c.hsc->twist = c.hsc->twist - twistmax;

@@ -160,6 +160,9 @@ struct Ctx {
   // (caps.f90 recont), 1 = contours built, advect's start to run,
   // 2 = inside advect's loop
   int phase = 0;
+  // surgery (0) and recontouring (1) calls and their wall seconds
+  int64_t phase_n[2] = {0, 0};
+  double phase_s[2] = {0.0, 0.0};
   bool have_field = false;
   // scalars
   Scalars *dsc = nullptr;   // device

@@ -662,31 +662,36 @@ static SeqLayout seq_layout(int64_t n) {
   return L;
 }
 
+// The three launches of one sum over a work area laid out by seq_layout(n).
+static int seqsum_launch(const double *a, int64_t n, double *dout, char *w, const SeqLayout &L,
+                         hipStream_t s) {
+  double *tot = (double *)(w + L.tot);
+  double2 *blk = (double2 *)(w + L.blk);
+  SegRec *segs = (SegRec *)(w + L.segs);
+  int32_t *cnt = (int32_t *)(w + L.cnt);
+  CompRec *cps = (CompRec *)(w + L.cps);
+  if (n == 0) {
+    hipLaunchKernelGGL(seq_zero, dim3(1), dim3(64), 0, s, dout);
+    HYD_HIP(hipGetLastError());
+    return HYD_OK;
+  }
+  hipLaunchKernelGGL(seq_chunks<0>, dim3((unsigned)L.nb), dim3(64), 0, s, a, n, L.nch, tot, blk,
+                     segs, cnt, cps, L.cap);
+  hipLaunchKernelGGL(seq_chunks<1>, dim3((unsigned)L.nb), dim3(64), 0, s, a, n, L.nch, tot, blk,
+                     segs, cnt, cps, L.cap);
+  hipLaunchKernelGGL(seq_chain, dim3(1), dim3(256), 0, s, a, segs, L.cap, cps, L.nw, dout);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
 int seqsum(Ctx &c, const double *a, int64_t n, double *dout) {
   const SeqLayout L = seq_layout(n);
   if (L.nch > c.seq_cap || seq_layout((int64_t)c.seq_cap * kChunk).bytes < L.bytes) {
     set_error("seqsum: array longer than the context's work space");
     return HYD_ECAPACITY;
   }
-  char *w = (char *)c.seq_work;
-  double *tot = (double *)(w + L.tot);
-  double2 *blk = (double2 *)(w + L.blk);
-  SegRec *segs = (SegRec *)(w + L.segs);
-  int32_t *cnt = (int32_t *)(w + L.cnt);
-  CompRec *cps = (CompRec *)(w + L.cps);
   KTimer t(c, "seqsum", 8 * n);
-  if (n == 0) {
-    hipLaunchKernelGGL(seq_zero, dim3(1), dim3(64), 0, c.s, dout);
-    HYD_HIP(hipGetLastError());
-    return HYD_OK;
-  }
-  hipLaunchKernelGGL(seq_chunks<0>, dim3((unsigned)L.nb), dim3(64), 0, c.s, a, n, L.nch, tot, blk,
-                     segs, cnt, cps, L.cap);
-  hipLaunchKernelGGL(seq_chunks<1>, dim3((unsigned)L.nb), dim3(64), 0, c.s, a, n, L.nch, tot, blk,
-                     segs, cnt, cps, L.cap);
-  hipLaunchKernelGGL(seq_chain, dim3(1), dim3(256), 0, c.s, a, segs, L.cap, cps, L.nw, dout);
-  HYD_HIP(hipGetLastError());
-  return HYD_OK;
+  return seqsum_launch(a, n, dout, (char *)c.seq_work, L, c.s);
 }
 
 int64_t seqsum_work_chunks(int64_t n) { return (n + kChunk - 1) / kChunk; }

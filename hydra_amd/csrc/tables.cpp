@@ -79,7 +79,7 @@ int build_consts(const hyd_params &p, Consts &c) {
   c.glyu = p.elly / (double)c.nyu;
   c.glyui = (double)c.nyu / p.elly;
   c.amu = 0.2;
-  c.ell = 6.25 * c.glx;
+  c.ell = (p.ell_cells > 0.0 ? p.ell_cells : 6.25) * c.glx;
   c.dm = c.amu * c.amu * c.ell / 4.0;
   c.dm2 = c.dm * c.dm;
   c.d4small = c.small * pow4_folded(c.glx);

@@ -18,6 +18,7 @@ type, bind(c) :: hyd_params
   real(c_double) :: tgsave, tcsave, dt_fixed
   integer(c_int64_t) :: node_capacity
   integer(c_int32_t) :: precision, device
+  real(c_double) :: ell_cells
 end type hyd_params
 
 interface

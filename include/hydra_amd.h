@@ -58,6 +58,12 @@ typedef struct hyd_params {
                             FP64 spectral state, nodes and RK accumulation
                             (no reference counterpart; not bit-exact)     */
   int32_t device;        /* HIP device ordinal                             */
+  double ell_cells;      /* surgery/renode length scale ell in grid lengths
+                            glx; 0 means the reference's 6.25
+                            (contours.f90:38: ell=6.25*glx, dm=amu**2*ell/4
+                            with amu=0.2, so dm = 0.0625*glx); 1.0 gives
+                            the dm = 0.01*glx that the ng=1024 workload
+                            names (about four times the nodes)            */
 } hyd_params;
 
 const char *hyd_last_error(void);
@@ -188,8 +194,17 @@ int hyd_advance(hyd_ctx *ctx);
 int hyd_advect_step(hyd_ctx *ctx);
 /* caps.f90:83-91 driven by a step count: recontour, init, then advect
    steps, recontouring whenever the regularisation count reaches 20, until
-   nsteps advances have been taken. */
+   nsteps advances have been taken.  A call resumes where the previous one
+   stopped: inside the advect loop it continues the loop (run(a) then
+   run(b) is run(a+b)); after set_field, set_contours or prepare it starts
+   with the recontouring, after recontour with advect's start. */
 int hyd_run(hyd_ctx *ctx, int32_t nsteps);
+/* Regularisation costs of the runs so far (no reference counterpart; the
+   reference's phases are timed from outside): out[0] surgery calls, out[1]
+   their wall seconds, out[2] recontourings, out[3] their wall seconds (each
+   call timed to its stream's drain).  reset != 0 zeroes the counters after
+   reading; out may be NULL. */
+int hyd_phase_times(hyd_ctx *ctx, double *out, int32_t reset);
 
 /* An ensemble of n independent flows (distinct contexts, any sizes and
  * precisions), each advanced by hyd_run's loop for nsteps steps.  The flows'

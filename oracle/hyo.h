@@ -43,6 +43,7 @@ typedef struct hyo_params {
   double tgsave, tcsave;
   double dt_fixed;     /* >0: fixed time step (the only patch vs adapt)  */
   int64_t npm;         /* node capacity (0 -> default)                   */
+  double ell_cells;    /* ell in grid lengths (0 -> 6.25, contours.f90:38) */
 } hyo_params;
 
 typedef struct hyo_model {

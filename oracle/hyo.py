@@ -28,7 +28,7 @@ class HyoParams(C.Structure):
         ("ellx", C.c_double), ("elly", C.c_double), ("kd", C.c_double), ("beta", C.c_double),
         ("rtherm", C.c_double), ("rekman", C.c_double), ("cdamp", C.c_double), ("nnu", C.c_double),
         ("tgsave", C.c_double), ("tcsave", C.c_double), ("dt_fixed", C.c_double),
-        ("npm", C.c_int64),
+        ("npm", C.c_int64), ("ell_cells", C.c_double),
     ]
 
 
@@ -93,10 +93,11 @@ def lib():
     return _lib
 
 
-def default_params(ng, kd=0.0, ncontq=80, dt_fixed=0.0, npm=0, tsave=1000.0, cdamp=2.0):
+def default_params(ng, kd=0.0, ncontq=80, dt_fixed=0.0, npm=0, tsave=1000.0, cdamp=2.0,
+                   ell_cells=0.0):
     return HyoParams(nx=ng, ny=ng, ncontq=ncontq, ellx=2 * np.pi, elly=2 * np.pi, kd=kd,
                      beta=0.0, rtherm=0.0, rekman=0.0, cdamp=cdamp, nnu=3.0, tgsave=tsave,
-                     tcsave=tsave, dt_fixed=dt_fixed, npm=npm)
+                     tcsave=tsave, dt_fixed=dt_fixed, npm=npm, ell_cells=ell_cells)
 
 
 class Oracle:

@@ -217,7 +217,7 @@ hyo_model *hyo_create(const hyo_params *p) {
   m->glyu = p->elly / (double)m->nyu;
   m->glyui = (double)m->nyu / p->elly;
   m->amu = 0.2;
-  m->ell = 6.25 * m->glx;
+  m->ell = (p->ell_cells > 0.0 ? p->ell_cells : 6.25) * m->glx;
   m->dm = m->amu * m->amu * m->ell / 4.0;
   m->dm2 = m->dm * m->dm;
   m->d4small = small * pow4_folded(m->glx);

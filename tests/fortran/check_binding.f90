@@ -15,7 +15,7 @@ pi = 4.d0 * atan(1.d0)
 p%ng = ng; p%ncontq = 20; p%ellx = 2.d0 * pi; p%elly = p%ellx
 p%kd = 0.d0; p%beta = 0.d0; p%rtherm = 0.d0; p%rekman = 0.d0
 p%cdamp = 2.d0; p%nnu = 3.d0; p%tgsave = 1000.d0; p%tcsave = 1000.d0
-p%dt_fixed = 0.d0; p%node_capacity = 0; p%precision = 0; p%device = 0
+p%dt_fixed = 0.d0; p%node_capacity = 0; p%precision = 0; p%device = 0; p%ell_cells = 0.d0
 rc = hyd_init(p, ctx)
 if (rc /= HYD_OK) then
   write(*, '(a,i0,a,a)') 'init rc=', rc, ' msg=', hyd_error_text()

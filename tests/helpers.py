@@ -60,3 +60,18 @@ def tangle_contours(dm, rng, ncirc, levels, rmin, rmax):
     return dict(x=cat(xs), y=cat(ys), nextq=cat(nxt).astype(np.int32),
                 i1q=np.array(i1, np.int32), i2q=np.array(i2, np.int32),
                 npq=np.array(npq, np.int32), indq=np.array(ind, np.int32), qjump=1.0)
+
+
+def state_digest(c, qs, t):
+    """SHA-256 of a flow's state: contour store (xq, yq as float64 bits,
+    nextq and the contour table as int32) and the spectral PV qs, plus t's
+    bits and the counts.  Equal digests mean bit-identical states."""
+    import hashlib
+    hc = hashlib.sha256()
+    for k in ("x", "y"):
+        hc.update(np.ascontiguousarray(c[k], dtype="<f8").tobytes())
+    for k in ("nextq", "i1q", "i2q", "npq", "indq"):
+        hc.update(np.ascontiguousarray(c[k], dtype="<i4").tobytes())
+    hq = hashlib.sha256(np.ascontiguousarray(qs, dtype="<f8").tobytes())
+    return {"t": float(t).hex(), "nptq": int(len(c["x"])), "nq": int(len(c["i1q"])),
+            "contours": hc.hexdigest()[:32], "qs": hq.hexdigest()[:32]}
