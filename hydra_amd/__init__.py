@@ -36,7 +36,9 @@ FIELDS = {"qs": 0, "qd": 1, "qc": 2, "qq": 3, "pp": 4, "uu": 5, "vv": 6, "qr": 7
 SCALARS = {"t": 0, "dt": 1, "twist": 2, "zzrms": 3, "zzmax": 4, "uumax": 5, "qjump": 6,
            "dtmax": 7}
 TABLES = {"green": 1, "flo": 2, "fhi": 3, "qdiss": 4, "rksq": 5, "hrkx": 6, "rkx": 7, "hrky": 8,
-          "rky": 9, "xtrig": 10, "ytrig": 11, "spmf": 12, "alk": 13, "shells": 14, "bety": 15}
+          "rky": 9, "xtrig": 10, "ytrig": 11, "spmf": 12, "alk": 13, "shells": 14, "bety": 15,
+          "rk": 16, "kmag": 17, "xgf": 18, "ygf": 19, "xgu": 20, "ygu": 21, "wcongen": 22,
+          "ixp": 23, "iyp": 24, "ixw": 25, "iyw": 26, "ibx": 27, "iby": 28}
 
 
 class HydraError(RuntimeError):
@@ -68,6 +70,9 @@ def lib():
         L.hyd_last_error.restype = C.c_char_p
         L.hyd_init.argtypes = [C.POINTER(HydParams), C.POINTER(C.c_void_p)]
         L.hyd_destroy.argtypes = [C.c_void_p]
+        L.hyd_initfft.argtypes = [C.c_int32, _I, _D]
+        for fn in ("hyd_forfft", "hyd_revfft"):
+            getattr(L, fn).argtypes = [C.c_void_p, C.c_int32, C.c_int32, _D, _D, _I]
         for fn in ("hyd_ptospc", "hyd_spctop", "hyd_xderiv", "hyd_yderiv"):
             getattr(L, fn).argtypes = [C.c_void_p, _D, _D]
         L.hyd_main_invert.argtypes = [C.c_void_p, _D, _D, _D, _D]
@@ -150,6 +155,14 @@ def _contour_args(c):
             ii["i2q"], ii["npq"], ii["indq"])
 
 
+def initfft(n):
+    """stafft.f90:63-125 initfft: (factors[5], trig[2n]) on the host."""
+    f = np.zeros(5, np.int32)
+    t = np.zeros(2 * n)
+    _check(lib().hyd_initfft(int(n), f, t))
+    return f, t
+
+
 def check_contours(c, ng, **kw):
     """The contour-table checks of set_contours, on the host without a
     device (contours numbered consecutively from node 1, ascending levels,
@@ -193,6 +206,25 @@ class Hydra:
         _check(fn(self.h, a, *outs))
         return outs[0] if nout == 1 else tuple(outs)
 
+    def forfft(self, m, n, x, trig=None, factors=None):
+        """stafft.f90:187-287: m vectors of length n, vector index fastest
+        (x has m*n values), transformed forward; returns the new array."""
+        if trig is None:
+            factors, trig = initfft(n)
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel().copy()
+        _check(self.L.hyd_forfft(self.h, int(m), int(n), x, _f64(trig),
+                                 np.ascontiguousarray(factors, dtype=np.int32)))
+        return x
+
+    def revfft(self, m, n, x, trig=None, factors=None):
+        """stafft.f90:290-406, the inverse of forfft."""
+        if trig is None:
+            factors, trig = initfft(n)
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel().copy()
+        _check(self.L.hyd_revfft(self.h, int(m), int(n), x, _f64(trig),
+                                 np.ascontiguousarray(factors, dtype=np.int32)))
+        return x
+
     def ptospc(self, rvar):
         return self._grid_op(self.L.hyd_ptospc, rvar)
 
@@ -235,8 +267,11 @@ class Hydra:
 
     def table(self, name):
         n = self.ng
+        u = 16 * n
         size = {1: n * n, 2: n * n, 3: n * n, 4: n * n, 5: n * n, 10: 2 * n, 11: 2 * n,
-                12: n + 1, 13: n + 1, 14: 3}.get(TABLES[name], n)
+                12: n + 1, 13: n + 1, 14: 3, 16: n * n, 17: n * n, 18: 4 * n, 19: 4 * n,
+                20: u, 21: u, 22: 1024, 25: 3 * u, 26: 3 * u, 27: u + 1,
+                28: u + 1}.get(TABLES[name], n)
         out = np.empty(size)
         _check(self.L.hyd_get_table(self.h, TABLES[name], out))
         return out

@@ -393,6 +393,61 @@ int hyd_spctop(hyd_ctx *h, const double *svar, double *rvar) {
   return d2h(c, rvar, c.wb, c.N);
 }
 
+int hyd_initfft(int32_t n, int32_t *factors, double *trig) {
+  if (!factors || !trig) {
+    set_error("initfft: null array");
+    return HYD_EARG;
+  }
+  if (n < 8 || n > 4096 || (n & (n - 1))) {
+    set_error("initfft: n must be a power of two in [8, 4096] (radix 2 and 4 only)");
+    return HYD_EFACTOR;
+  }
+  int f[5];
+  std::vector<double> t;
+  HYD_CHECK(initfft_table(n, f, t));
+  for (int k = 0; k < 5; k++) factors[k] = f[k];
+  memcpy(trig, t.data(), sizeof(double) * 2 * (size_t)n);
+  return HYD_OK;
+}
+
+static int fft_1d_host(Ctx &c, bool fwd, int32_t m, int32_t n, double *x, const double *trig,
+                       const int32_t *factors) {
+  if (!x || !trig || !factors || m < 1) {
+    set_error("forfft/revfft: m >= 1 and non-null arrays");
+    return HYD_EARG;
+  }
+  if (n < 8 || n > 4096 || (n & (n - 1))) {
+    set_error("forfft/revfft: n must be a power of two in [8, 4096]");
+    return HYD_EFACTOR;
+  }
+  int f[5];
+  std::vector<double> t;
+  HYD_CHECK(initfft_table(n, f, t));
+  for (int k = 0; k < 5; k++)
+    if (factors[k] != f[k]) {
+      set_error("forfft/revfft: factors are not initfft's for n");
+      return HYD_EFACTOR;
+    }
+  const int64_t len = (int64_t)m * n;
+  double *dx, *dt;
+  HYD_CHECK(dbuf(c, "api_fft_x", len, &dx));
+  HYD_CHECK(dbuf(c, "api_fft_trig", 2 * (int64_t)n, &dt));
+  HYD_HIP(hipMemcpyAsync(dx, x, sizeof(double) * len, hipMemcpyHostToDevice, c.s));
+  HYD_HIP(hipMemcpyAsync(dt, trig, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c.s));
+  HYD_CHECK(fft_1d(c, fwd, m, n, dx, dt));
+  return d2h(c, x, dx, len);
+}
+
+int hyd_forfft(hyd_ctx *h, int32_t m, int32_t n, double *x, const double *trig,
+               const int32_t *factors) {
+  return fft_1d_host(*C(h), true, m, n, x, trig, factors);
+}
+
+int hyd_revfft(hyd_ctx *h, int32_t m, int32_t n, double *x, const double *trig,
+               const int32_t *factors) {
+  return fft_1d_host(*C(h), false, m, n, x, trig, factors);
+}
+
 int hyd_xderiv(hyd_ctx *h, const double *var, double *der) {
   Ctx &c = *C(h);
   HYD_CHECK(h2d(c, c.wa, var, c.N));
@@ -461,6 +516,72 @@ int hyd_gradient(hyd_ctx *h, const double *ff, double *ffx, double *ffy) {
   return d2h(c, ffy, c.wc, c.N);
 }
 
+// The tables of init_contours (contours.f90:48-132), indices as doubles:
+// 18 xgf, 19 ygf (fine grid lines), 20 xgu, 21 ygu (ultra-fine), 22 the
+// congen weights w00, w10, w01, w11 (each mgu x mgu, iyf fastest), 23 ixp,
+// 24 iyp, 25 ixfw, ix0w, ix1w, 26 iyfw, iy0w, iy1w (nxu / nyu each),
+// 27 ibx (nxu+1), 28 iby (nyu+1).
+static int contour_table(const Consts &k, int id, double *out) {
+  const int mgu = k.nxu / k.nx;
+  switch (id) {
+    case 18:
+      for (int i = 1; i <= k.nxf; i++) out[i - 1] = k.xmin + k.glxf * (double)(i - 1);
+      return HYD_OK;
+    case 19:
+      for (int i = 1; i <= k.nyf; i++) out[i - 1] = k.ymin + k.glyf * (double)(i - 1);
+      return HYD_OK;
+    case 20:
+      for (int i = 1; i <= k.nxu; i++) out[i - 1] = k.xmin + k.glxu * (double)(i - 1);
+      return HYD_OK;
+    case 21:
+      for (int i = 1; i <= k.nyu; i++) out[i - 1] = k.ymin + k.glyu * (double)(i - 1);
+      return HYD_OK;
+    case 22: {
+      const double fac = 1.0 / (double)mgu;
+      const int w = mgu * mgu;
+      for (int ixf = 1; ixf <= mgu; ixf++) {
+        const double pxf = fac * (double)(ixf - 1), pxc = 1.0 - pxf;
+        for (int iyf = 1; iyf <= mgu; iyf++) {
+          const double pyf = fac * (double)(iyf - 1), pyc = 1.0 - pyf;
+          const int j = (ixf - 1) * mgu + (iyf - 1);
+          out[j] = pyc * pxc;
+          out[w + j] = pyf * pxc;
+          out[2 * w + j] = pyc * pxf;
+          out[3 * w + j] = pyf * pxf;
+        }
+      }
+      return HYD_OK;
+    }
+    case 23:
+    case 24: {
+      const int n = id == 23 ? k.nx : k.ny;
+      for (int i = 1; i <= n; i++) out[i - 1] = (double)(i < n ? i + 1 : 1);
+      return HYD_OK;
+    }
+    case 25:
+    case 26: {
+      const int nu = id == 25 ? k.nxu : k.nyu, n = id == 25 ? k.nx : k.ny;
+      for (int i = 1; i <= nu; i++) {
+        const int q = (i - 1) / mgu;
+        const int i0 = 1 + q;
+        out[i - 1] = (double)(i - mgu * q);          // ixfw
+        out[nu + i - 1] = (double)i0;                 // ix0w
+        out[2 * nu + i - 1] = (double)(2 + q - n * (i0 / n));  // ix1w
+      }
+      return HYD_OK;
+    }
+    case 27:
+    case 28: {
+      const int nu = id == 27 ? k.nxu : k.nyu;
+      out[0] = (double)(nu - 1);
+      for (int i = 2; i <= nu + 1; i++) out[i - 1] = (double)(i - 2);
+      return HYD_OK;
+    }
+  }
+  set_error("unknown contour table id");
+  return HYD_EARG;
+}
+
 int hyd_get_table(hyd_ctx *h, int32_t id, double *out) {
   Ctx &c = *C(h);
   const HostTables &t = c.ht;
@@ -485,7 +606,14 @@ int hyd_get_table(hyd_ctx *h, int32_t id, double *out) {
       out[2] = c.c.dsumi;
       return HYD_OK;
     case 15: v = &t.bety; break;
-    default: set_error("unknown table id"); return HYD_EARG;
+    case 16: v = &t.rk; break;
+    case 17:
+      for (size_t k = 0; k < t.kmag.size(); k++) out[k] = (double)t.kmag[k];
+      return HYD_OK;
+    default:
+      if (id >= 18 && id <= 28) return contour_table(c.c, id, out);
+      set_error("unknown table id");
+      return HYD_EARG;
   }
   memcpy(out, v->data(), sizeof(double) * v->size());
   return HYD_OK;
@@ -633,7 +761,7 @@ int hyd_set_field(hyd_ctx *h, const double *qq, double t0) {
     set_error("beta != 0 is not supported by the device step");
     return HYD_EARG;
   }
-  // contint, generic.f90:216-229
+  // contint, generic.f90:66-79
   double mx = qq[0], mn = qq[0];
   for (int64_t k = 0; k < c.N; k++) {
     mx = qq[k] > mx ? qq[k] : mx;

@@ -662,6 +662,11 @@ int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, i
   hipLaunchKernelGGL(c2g_avg9<double>, dim3((nh2 + 255) / 256, nh2), dim3(256), 0, c.s, c.mq,
                      c.c2g_w, nh2);
   HYD_HIP(hipGetLastError());
+  if (c.after_sweep) {
+    auto fn = std::move(c.after_sweep);
+    c.after_sweep = nullptr;
+    HYD_CHECK(fn());
+  }
   // sequential mean, contours.f90:330-336
   HYD_CHECK(seqsum(c, c.c2g_w, N, c.dsum));
   if (defer_mean) return HYD_OK;

@@ -540,19 +540,20 @@ static int fft_launch_t(Ctx &c, bool fwd, int F, const TI *const *in, TO *const 
   // batch per workgroup: the autotuned value of this layout class when it
   // divides m, else min(4096/n, kMaxV)
   const int cls = fwd ? (out_rows ? 0 : 1) : (in_rows ? 3 : 2);
-  int V = c.fft_v[cls];
+  int V = n == c.c.nx ? c.fft_v[cls] : 0;  // tuned for the context's length
   if (V < 1 || V > kMaxV || V > m || m % V) {
-    V = std::min(4096 / n, kMaxV);
-    if (V < 1) V = 1;
-    if (V > m) V = m;
-    while (m % V) V /= 2;
+    // the largest power of two <= 4096/n and kMaxV that divides m
+    V = std::max(1, std::min(4096 / n, kMaxV));
+    while (V > 1 && (V > m || m % V)) V /= 2;
   }
   g.V = V;
   g.lgV = __builtin_ctz(V);
   g.in_rows = in_rows;
   g.out_rows = out_rows;
-  g.n2 = c.plan.n2;
-  g.n4 = c.plan.n4;
+  // factor counts of this length (stafft.f90:146-178: 4s, then one 2)
+  const int lg = __builtin_ctz((unsigned)n);
+  g.n2 = lg & 1;
+  g.n4 = lg >> 1;
   const size_t shm = fft_lds_bytes(n, V, sizeof(T));
   const dim3 grid(m / V, F);
   // one launch reads and writes F x m vectors of n values
@@ -672,6 +673,14 @@ int fft_spctop(Ctx &c, const double *svar, double *rvar, int pre, double post_si
   HYD_CHECK(fft_launch(c, false, tmp, rvar, n, n, 1, 0, 0, nullptr, c.plan.rnorm * post_sign,
                        c.xtrig));
   return HYD_OK;
+}
+
+// forfft(m, n, x) / revfft(m, n, x) on a strided (m, n) array in place
+// (stafft.f90:187-406): scaled by 1/sqrt(n) forward, 2/sqrt(n) backward,
+// revfft's halving of the real-only entries inside the pass.
+int fft_1d(Ctx &c, bool fwd, int m, int n, double *x, const double *trig) {
+  const double scale = fwd ? 1.0 / sqrt((double)n) : 2.0 / sqrt((double)n);
+  return fft_launch(c, fwd, x, x, n, m, 0, 0, 0, nullptr, scale, trig);
 }
 
 // forfft(m, ng, x) / revfft(m, ng, x) on a strided (m, ng) array in place.

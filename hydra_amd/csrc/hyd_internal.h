@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <functional>
 
 #include <map>
 #include <string>
@@ -77,6 +78,9 @@ struct HostTables {
 
 int build_consts(const hyd_params &p, Consts &c);
 int build_tables(const hyd_params &p, const Consts &c, HostTables &t);
+// initfft's factors and trig table for a power-of-two length n
+// (stafft.f90:63-125); HYD_EFACTOR for other lengths.
+int initfft_table(int n, int f[5], std::vector<double> &trig);
 
 // FFT plan for one length n = 2^p (radix-2 stage then radix-4 stages, the
 // factor order of stafft.f90:146-178 and 212-271).
@@ -178,6 +182,11 @@ struct Ctx {
   double *diag_out = nullptr;    // 8 scratch results
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // work that the plain step starts on its second stream once the contour
+  // grid's node and sweep kernels are queued (so that those run alone on
+  // the GPU and the second stream's work overlaps the single-workgroup
+  // sequential mean instead); con2grid runs and clears it
+  std::function<int()> after_sweep;
   void *seq_work = nullptr; // chunk table of seqsum
   int64_t seq_cap = 0;      // chunks it holds
   // work buffers of recontour / renode, allocated on first use
@@ -200,6 +209,10 @@ int fft_ptospc_mean(Ctx &c, const double *rvar, const double *sum, double ndiv, 
 int fft_spctop(Ctx &c, const double *svar, double *rvar, int pre, double post_sign,
                double *tmp);
 int fft_forward_1d(Ctx &c, double *x, int m);   // forfft(m, ng, x)
+// One batched 1D pass in stafft's layout (m vectors of length n, the
+// vector index fastest) in place on device memory: forfft or revfft
+// (stafft.f90:187-406) with the caller's trig table (device copy).
+int fft_1d(Ctx &c, bool fwd, int m, int n, double *x, const double *trig);
 int fft_ptospc32(Ctx &c, const float *rvar, double *svar);
 int fft_spctop_n(Ctx &c, int F, const double *const *svar, double *const *rvar, const int *pre,
                  const double *post_sign, double *const *tmp);

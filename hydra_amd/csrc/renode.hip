@@ -6,14 +6,19 @@
 // expression order.  The two sequential pieces -- the sum of the fractional
 // node counts (contours.f90:799-802, whose nint decides how many nodes the
 // contour keeps) and the accumulator walk that places the new nodes
-// (813-930) -- run exactly as written as wavefront-uniform loops: the
-// wavefront loads 64 consecutive values at a time and the loop reads them
-// with readlane, so the only serial chain is the additions themselves.
-// The walk only records (old segment, accumulator) per new node; the
-// positions are then evaluated lane-parallel.
+// (813-930) -- run as wavefront-uniform chains: 64 consecutive values are
+// staged in LDS and every lane reads each one by broadcast, so the serial
+// chain is the floating-point additions alone.  The walk is written per old
+// node: acc + e(i) is the only rounded operation (acc - 1 is exact for acc
+// >= 1), so acc = s - floor(s) after s = acc + e(i) is the reference's
+// value after its floor(s) subtractions, and the nodes placed from old node
+// i carry acc values s - 1, s - 2, ...; each lane recovers the s of its old
+// node from LDS and emits its nodes in parallel.  Positions are then
+// evaluated lane-parallel.
 //
-// Two launches: RN_COUNT computes npr per contour (0 = delete); after an
-// exclusive scan of npr, RN_PLACE writes the nodes.
+// Two launches: RN_COUNT computes the geometry, the sum and npr per contour
+// (0 = delete) and keeps them; after an exclusive scan of npr, RN_PLACE
+// scales, walks and writes the nodes.
 #include "hyd_internal.h"
 
 namespace hyd {
@@ -28,6 +33,9 @@ struct RenodeArgs {
   double *sdx, *sdy, *sa, *sb, *sc, *sd, *se, *su, *sv, *stm;
   int32_t *wi;     // walk: old node index per new node (indexed like output)
   double *wacc;    // walk: accumulator per new node
+  // count pass results per contour, read by the place pass
+  double *csum;
+  int32_t *cnc, *cfirst;
   // output
   int32_t *npr;    // per contour
   const int64_t *off_out;
@@ -122,39 +130,108 @@ __device__ __forceinline__ double rl_d(double x, int l) {
   return __hiloint2double(hi, lo);
 }
 
-// Wavefront-uniform reader of se[o + i], 64 values per load.
-struct ERead {
-  const double *se;
-  int64_t o;
-  int n, base;
-  double blk;
-  __device__ void init(const double *s_, int64_t o_, int n_) {
-    se = s_;
-    o = o_;
-    n = n_;
-    base = -64;
-    blk = 0.0;
-  }
-  __device__ __forceinline__ double get(int i) {
-    if (i < base || i >= base + 64) {
-      base = i & ~63;
-      const int j = base + (int)(threadIdx.x & 63);
-      blk = j < n ? se[o + j] : 0.0;
-    }
-    return rl_d(blk, i - base);
-  }
-};
+// Orders the calling wavefront's LDS accesses (LDS serves one wavefront's
+// accesses in issue order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 
-// Sequential sum of e over [b, e) in node order (every lane, same value).
-__device__ __forceinline__ double seq_e(const double *se, int64_t o, int b, int e) {
+// e of the q-th node of a walk that starts at node ibeg and wraps (q < n).
+__device__ __forceinline__ double e_at(const double *se, int64_t o, int n, int ibeg, int q) {
+  int i = ibeg + q;
+  if (i >= n) i -= n;
+  return se[o + i];
+}
+
+// Sequential sum of e over len nodes from ibeg (wrapping: next() order),
+// contours.f90:799-802 and 880-884.  Wavefront-uniform (every lane returns
+// the same value); lds holds 64 doubles of the calling wavefront.
+__device__ double seq_sum_e(const double *se, int64_t o, int n, int ibeg, int len,
+                            double *lds) {
   const int l = threadIdx.x & 63;
   double s = 0.0;
-  for (int b0 = b; b0 < e; b0 += 64) {
-    const double v = b0 + l < e ? se[o + b0 + l] : 0.0;
-    const int m = e - b0 < 64 ? e - b0 : 64;
-    for (int j = 0; j < m; j++) s = s + rl_d(v, j);
+  double v = l < len ? e_at(se, o, n, ibeg, l) : 0.0;
+  for (int q0 = 0; q0 < len; q0 += 64) {
+    const int qn = q0 + 64 + l;
+    const double vn = qn < len ? e_at(se, o, n, ibeg, qn) : 0.0;  // next block in flight
+    lds[l] = v;
+    wave_sync();
+    const int m = len - q0 < 64 ? len - q0 : 64;
+    if (m == 64) {
+#pragma unroll
+      for (int j = 0; j < 64; j++) s = s + lds[j];
+    } else {
+      for (int j = 0; j < m; j++) s = s + lds[j];
+    }
+    wave_sync();
+    v = vn;
   }
   return s;
+}
+
+// The accumulator walk (contours.f90:826-838, 858-868, 907-920) over the
+// nodes ibeg, next(ibeg), ... for `want` new nodes, written to out slots
+// oo+1 .. oo+want (old node index, acc after the subtraction).  acc starts
+// at 0; at most maxlen old nodes are visited.  Wavefront 0 only.
+__device__ void walk_nodes(const RenodeArgs &g, int64_t o, int n, int ibeg, int maxlen,
+                           int want, int64_t oo, double *lds_e, double *lds_s) {
+  const int l = threadIdx.x & 63;
+  double acc = 0.0;
+  int done = 0;
+  double v = l < maxlen ? e_at(g.se, o, n, ibeg, l % n) : 0.0;
+  for (int q0 = 0; q0 < maxlen && done < want; q0 += 64) {
+    const int qn = q0 + 64 + l;
+    const double vn = qn < maxlen ? e_at(g.se, o, n, ibeg, qn % n) : 0.0;
+    lds_e[l] = v;
+    wave_sync();
+    const int m = maxlen - q0 < 64 ? maxlen - q0 : 64;
+    if (m == 64) {
+#pragma unroll 16
+      for (int j = 0; j < 64; j++) {
+        const double sj = acc + lds_e[j];
+        lds_s[j] = sj;
+        acc = sj - floor(sj);
+      }
+    } else {
+      for (int j = 0; j < m; j++) {
+        const double sj = acc + lds_e[j];
+        lds_s[j] = sj;
+        acc = sj - floor(sj);
+      }
+    }
+    wave_sync();
+    // old node q0+l places floor(s) new nodes, capped at the total wanted
+    const double sl = l < m ? lds_s[l] : 0.0;
+    const int k = l < m ? (int)floor(sl) : 0;
+    int incl = k;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(incl, d, 64);
+      if (l >= d) incl += t;
+    }
+    const int excl = incl - k;
+    int i = ibeg + ((q0 + l) % n);
+    if (i >= n) i -= n;
+    for (int r = 1; r <= k; r++) {
+      const int slot = done + excl + r;  // 1-based new node after the first
+      if (slot > want) break;
+      g.wi[oo + slot] = i;
+      g.wacc[oo + slot] = sl - (double)r;
+    }
+    done += __shfl(incl, 63, 64);
+    wave_sync();
+    v = vn;
+  }
+  // (sum(e) = the wanted count, so the walk always ends with done = want;
+  // should rounding ever leave slots, they get the last node, never an
+  // index outside the contour)
+  for (int slot = done + 1 + l; slot <= want; slot += 64) {
+    int i = ibeg + (maxlen - 1) % n;
+    if (i >= n) i -= n;
+    g.wi[oo + slot] = i;
+    g.wacc[oo + slot] = 0.0;
+  }
 }
 
 __device__ __forceinline__ int nint_d(double x) { return (int)llround(x); }
@@ -174,6 +251,8 @@ __device__ __forceinline__ void place(const RenodeArgs &g, int64_t o, int i, dou
 
 // One workgroup of 256 threads per contour: the lane-parallel phases use
 // all of them, the sequential sums and walks run on wavefront 0.
+// RN_COUNT (PLACE = false): geometry, corners, sums, npr; RN_PLACE: the
+// scaling, the walk and the node positions from the count pass's geometry.
 template <bool PLACE>
 __global__ void __launch_bounds__(256) renode_kernel(RenodeArgs g) {
   const int c = blockIdx.x;
@@ -182,33 +261,44 @@ __global__ void __launch_bounds__(256) renode_kernel(RenodeArgs g) {
   const int64_t o = g.off_in[c];
   const int t = threadIdx.x, l = t & 63, nt = blockDim.x;
   const bool w0 = t < 64;
+  __shared__ double lds_e[64], lds_s[64];
+  __shared__ int s_ncorn, s_first, s_tot;
+  __shared__ double s_sum;
   if (n <= 0) {
     if (!PLACE && t == 0) g.npr[c] = 0;
     return;
   }
-  renode_geometry(g, o, n);
-  // corners (d holds the flag): count, first; the total of e
-  // (contours.f90:799-809)
-  __shared__ int s_ncorn, s_first;
-  __shared__ double s_sum;
-  if (w0) {
-    int ncorn = 0, first = -1;
-    for (int b0 = 0; b0 < n; b0 += 64) {
-      const uint64_t m = __ballot(b0 + l < n && g.sd[o + b0 + l] != 0.0);
-      if (m && first < 0) first = b0 + __ffsll((unsigned long long)m) - 1;
-      ncorn += __popcll(m);
+  if (!PLACE) {
+    renode_geometry(g, o, n);
+    // corners (d holds the flag): count, first; the total of e
+    // (contours.f90:799-809)
+    if (w0) {
+      int ncorn = 0, first = -1;
+      for (int b0 = 0; b0 < n; b0 += 64) {
+        const uint64_t m = __ballot(b0 + l < n && g.sd[o + b0 + l] != 0.0);
+        if (m && first < 0) first = b0 + __ffsll((unsigned long long)m) - 1;
+        ncorn += __popcll(m);
+      }
+      const double sum = seq_sum_e(g.se, o, n, 0, n, lds_e);
+      if (l == 0) {
+        s_ncorn = ncorn;
+        s_first = first;
+        s_sum = sum;
+        g.csum[c] = sum;
+        g.cnc[c] = ncorn;
+        g.cfirst[c] = first;
+      }
     }
-    const double sum = seq_e(g.se, o, 0, n);
-    if (l == 0) {
-      s_ncorn = ncorn;
-      s_first = first;
-      s_sum = sum;
-    }
+    __syncthreads();
+  } else if (t == 0) {
+    s_sum = g.csum[c];
+    s_ncorn = g.cnc[c];
+    s_first = g.cfirst[c];
   }
-  __syncthreads();
-  const int ncorn = s_ncorn, s_first_ = s_first;
+  if (PLACE) __syncthreads();
+  const int ncorn = s_ncorn, first = s_first;
   const double sum = s_sum;
-  int npr = nint_d(sum) + 1;
+  const int npr = nint_d(sum) + 1;
   if (npr < 3) {
     if (!PLACE && t == 0) g.npr[c] = 0;
     return;
@@ -222,30 +312,15 @@ __global__ void __launch_bounds__(256) renode_kernel(RenodeArgs g) {
     for (int i = t; i < n; i += nt) g.se[o + i] = fac * g.se[o + i];
     __syncthreads();
     const int64_t oo = g.off_out[c];
-    const int start = ncorn == 0 ? 0 : s_first_;
+    const int start = ncorn == 0 ? 0 : first;
     if (w0) {
       if (l == 0) {
         g.xr[oo] = g.xd[o + start];
         g.yr[oo] = g.yd[o + start];
       }
-      // accumulator walk; ncorn=0 walks i = 1,2,.. ; ncorn=1 starts after
-      // the corner with next() wrapping (contours.f90:826-868)
-      ERead er;
-      er.init(g.se, o, n);
-      double acc = 0.0;
-      int i = ncorn == 0 ? -1 : start - 1;
-      for (int im = 1; im < npr; im++) {
-        while (acc < 1.0) {
-          i = (ncorn == 0) ? i + 1 : (i + 1 < n ? i + 1 : 0);
-          if (ncorn == 0 && i >= n) { i = n - 1; break; }  // unreachable: sum(e)=npr
-          acc = acc + er.get(i);
-        }
-        acc = acc - 1.0;
-        if (l == 0) {
-          g.wi[oo + im] = i;
-          g.wacc[oo + im] = acc;
-        }
-      }
+      // ncorn = 0 walks i = 1, 2, ...; ncorn = 1 starts at the corner and
+      // wraps (contours.f90:826-868)
+      walk_nodes(g, o, n, start, n, npr - 1, oo, lds_e, lds_s);
     }
     __syncthreads();
     for (int im = 1 + t; im < npr; im += nt) {
@@ -258,21 +333,17 @@ __global__ void __launch_bounds__(256) renode_kernel(RenodeArgs g) {
   }
   // several corners: renode each stretch between consecutive corners
   // (contours.f90:873-929); wavefront 0 does the sums, scalings and walks.
-  __shared__ int s_tot;
   if (w0) {
     const int64_t oo = PLACE ? g.off_out[c] : 0;
     int tot = 0;
-    int ibeg = s_first_;
-    int corner_idx = s_first_;
-    ERead er;
-    er.init(g.se, o, n);
+    int ibeg = first;
     for (int kk = 0; kk < ncorn; kk++) {
-      // next corner after corner_idx (wrapping to the first one at the end)
+      // next corner after ibeg (wrapping to the first one at the end)
       int iend;
       if (kk == ncorn - 1) {
-        iend = s_first_;
+        iend = first;
       } else {
-        int j = corner_idx + 1;
+        int j = ibeg + 1;
         for (;;) {
           const int jb = j & ~63;
           const uint64_t m = __ballot(jb + l >= j && jb + l < n && g.sd[o + jb + l] != 0.0);
@@ -285,49 +356,27 @@ __global__ void __launch_bounds__(256) renode_kernel(RenodeArgs g) {
         iend = j;
       }
       // sum over the stretch ibeg .. iend (exclusive), with next() wrapping
-      double sm = 0.0;
-      int i = ibeg;
-      do {
-        sm = sm + er.get(i);
-        i = i + 1 < n ? i + 1 : 0;
-      } while (i != iend);
+      const int len = iend > ibeg ? iend - ibeg : iend + n - ibeg;
+      const double sm = seq_sum_e(g.se, o, n, ibeg, len, lds_e);
       if (sm > g.small) {
         const int npseg = nint_d(sm) + 1;
         if (PLACE) {
           const double fac = (double)npseg / sm;
-          // scale the stretch (lane-parallel over its wrapped index range)
-          const int len = iend > ibeg ? iend - ibeg : iend + n - ibeg;
           for (int q0 = l; q0 < len; q0 += 64) {
             const int q = ibeg + q0 < n ? ibeg + q0 : ibeg + q0 - n;
             g.se[o + q] = fac * g.se[o + q];
           }
-          __threadfence();  // other lanes' stores visible to the reload below
-          er.base = -64;    // the scaled values replace the cached block
+          __threadfence();  // the walk reads the scaled values (other lanes' stores)
           if (l == 0) {
             g.xr[oo + tot] = g.xd[o + ibeg];
             g.yr[oo + tot] = g.yd[o + ibeg];
             g.wi[oo + tot] = -1;
           }
-          if (npseg > 1) {
-            double acc = 0.0;
-            i = ibeg - 1;
-            for (int im = tot + 1; im < tot + npseg; im++) {
-              while (acc < 1.0) {
-                i = i + 1 < n ? i + 1 : 0;
-                acc = acc + er.get(i);
-              }
-              acc = acc - 1.0;
-              if (l == 0) {
-                g.wi[oo + im] = i;
-                g.wacc[oo + im] = acc;
-              }
-            }
-          }
+          if (npseg > 1) walk_nodes(g, o, n, ibeg, n, npseg - 1, oo + tot, lds_e, lds_s);
         }
         tot += npseg;
       }
       ibeg = iend;
-      corner_idx = iend;
     }
     if (l == 0) s_tot = tot;
   }
@@ -389,6 +438,9 @@ int renode_batch(Ctx &c, const double *xd, const double *yd, const int64_t *off_
   g.densf = k.densf;
   g.wi = nullptr;
   g.wacc = nullptr;
+  HYD_CHECK(dbuf(c, "rn_csum", ncont, &g.csum));
+  HYD_CHECK(dbuf(c, "rn_cnc", ncont, &g.cnc));
+  HYD_CHECK(dbuf(c, "rn_cfirst", ncont, &g.cfirst));
   hipLaunchKernelGGL(renode_kernel<false>, dim3(ncont), dim3(256), 0, c.s, g);
   HYD_HIP(hipGetLastError());
   std::vector<int32_t> hn(ncont);

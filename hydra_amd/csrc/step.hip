@@ -218,6 +218,13 @@ static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); 
 
 // The contour part of inversion: qc = ptospc(con2grid) (evolution.f90:507-508).
 static int contour_grid(Ctx &c) {
+  // with per-launch timing (one stream) the deferred work runs first, so
+  // that the contour grid's records hold its own kernels only
+  if ((c.nptq == 0 || c.timing) && c.after_sweep) {
+    auto fn = std::move(c.after_sweep);
+    c.after_sweep = nullptr;
+    HYD_CHECK(fn());
+  }
   if (c.nptq == 0) {
     HYD_CHECK(con2grid(c, c.xq, c.yq, c.nextq, c.nptq, c.c2g_q, nullptr));
     return fft_ptospc(c, c.c2g_q, c.qc);
@@ -485,12 +492,26 @@ static int fork_s2(Ctx &c) {
 
 int advance(Ctx &c) {
   const int64_t N = c.N;
+  c.after_sweep = nullptr;
   auto fields = [&](int stage) {
     return on_s2(c, [&]() {
       HYD_CHECK(rk_fields(c, stage));
       HYD_HIP(hipEventRecord(c.ev_join, c.s));
       return HYD_OK;
     });
+  };
+  // stage k's source and field update, started once the next contour
+  // grid's sweep is queued (con2grid runs c.after_sweep)
+  auto source_fields_later = [&](int stage) {
+    c.after_sweep = [&c, stage]() {
+      HYD_CHECK(fork_s2(c));
+      return on_s2(c, [&]() {
+        HYD_CHECK(source(c, 1));
+        HYD_CHECK(rk_fields(c, stage));
+        HYD_HIP(hipEventRecord(c.ev_join, c.s));
+        return HYD_OK;
+      });
+    };
   };
   // stage 1
   HYD_CHECK(inversion(c));
@@ -515,11 +536,17 @@ int advance(Ctx &c) {
     }
     HYD_HIP(hipStreamWaitEvent(c.s, c.ev_join, 0));
     HYD_CHECK(inversion_fields(c));
-    HYD_CHECK(fork_s2(c));
-    HYD_CHECK(on_s2(c, [&]() { return source(c, stage == 4 ? 2 : 1); }));
-    HYD_CHECK(rk_nodes(c, stage));
-    HYD_CHECK(fields(stage));
+    if (stage < 4) {
+      source_fields_later(stage);
+      HYD_CHECK(rk_nodes(c, stage));
+    } else {
+      HYD_CHECK(fork_s2(c));
+      HYD_CHECK(on_s2(c, [&]() { return source(c, 2); }));
+      HYD_CHECK(rk_nodes(c, stage));
+      HYD_CHECK(fields(stage));
+    }
   }
+  c.after_sweep = nullptr;
   HYD_HIP(hipStreamWaitEvent(c.s, c.ev_join, 0));
   HYD_HIP(hipGetLastError());
   return HYD_OK;

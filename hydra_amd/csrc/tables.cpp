@@ -2,7 +2,7 @@
 //
 // These are computed once, on the host, with the same libm calls and in the
 // same operation order as the reference, then uploaded:
-//   constants.f90:104-133, contours.f90:19-42    (build_consts)
+//   constants.f90:44-75, contours.f90:19-42      (build_consts)
 //   stafft.f90:63-184 (initfft/factorisen),
 //   sta2dfft.f90:40-97 (init2dfft), spectral.f90:42-158 (build_tables)
 // The FFT trig table, the Hou-Li filtered wavenumbers and the Butterworth
@@ -125,6 +125,8 @@ static int trig_table(int n, int f[5], std::vector<double> &trig) {
   }
   return HYD_OK;
 }
+
+int initfft_table(int n, int f[5], std::vector<double> &trig) { return trig_table(n, f, trig); }
 
 int build_tables(const hyd_params &p, const Consts &c, HostTables &t) {
   const int nx = c.nx, ny = c.ny, nwx = nx / 2, nwy = ny / 2;

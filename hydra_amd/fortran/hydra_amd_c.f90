@@ -38,6 +38,41 @@ interface
     type(c_ptr), value :: ctx
   end function
 
+  ! stafft.f90:63-125 (no context)
+  integer(c_int) function hyd_initfft(n, factors, trig) bind(c, name="hyd_initfft")
+    import :: c_int, c_int32_t, c_double
+    integer(c_int32_t), value :: n
+    integer(c_int32_t), intent(out) :: factors(5)
+    real(c_double), intent(out) :: trig(*)
+  end function
+
+  ! stafft.f90:187-406
+  integer(c_int) function hyd_forfft(ctx, m, n, x, trig, factors) bind(c, name="hyd_forfft")
+    import :: c_int, c_int32_t, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: m, n
+    real(c_double), intent(inout) :: x(*)
+    real(c_double), intent(in) :: trig(*)
+    integer(c_int32_t), intent(in) :: factors(5)
+  end function
+
+  integer(c_int) function hyd_revfft(ctx, m, n, x, trig, factors) bind(c, name="hyd_revfft")
+    import :: c_int, c_int32_t, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: m, n
+    real(c_double), intent(inout) :: x(*)
+    real(c_double), intent(in) :: trig(*)
+    integer(c_int32_t), intent(in) :: factors(5)
+  end function
+
+  ! init_spectral / init_contours tables (spectral.f90:42-158, contours.f90:48-132)
+  integer(c_int) function hyd_get_table(ctx, id, out) bind(c, name="hyd_get_table")
+    import :: c_int, c_int32_t, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: id
+    real(c_double), intent(out) :: out(*)
+  end function
+
   ! sta2dfft.f90:103-165
   integer(c_int) function hyd_ptospc(ctx, rvar, svar) bind(c, name="hyd_ptospc")
     import :: c_int, c_ptr, c_double

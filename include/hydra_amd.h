@@ -42,7 +42,7 @@ extern "C" {
 typedef struct hyd_ctx hyd_ctx;
 
 /* Run-time form of the compile-time parameters of parameters.f90:14-36 and
-   the capacities of constants.f90:95-101. */
+   the capacities of constants.f90:34-41. */
 typedef struct hyd_params {
   int32_t ng;            /* nx = ny, a power of two >= 8                   */
   int32_t ncontq;        /* PV contour intervals (parameters.f90:17)       */
@@ -73,6 +73,23 @@ const char *hyd_last_error(void);
    the same libm calls as the reference and uploaded once. */
 int hyd_init(const hyd_params *p, hyd_ctx **ctx);
 int hyd_destroy(hyd_ctx *ctx);
+
+/* ---- lib/stafft/stafft.f90 (1D real FFT) ---------------------------------- */
+/* initfft(n,factors,trig) stafft.f90:63-125, on the host without a context:
+   factors[5] = the counts of the radices 6, 4, 2, 3, 5 (the reference's
+   order), trig[2n] = cos, then -sin, of its angle table.  n must be a power
+   of two in [8, 4096] (only radix 4 and 2 occur); HYD_EFACTOR otherwise, as
+   the reference stops (stafft.f90:85-93). */
+int hyd_initfft(int32_t n, int32_t *factors, double *trig);
+/* forfft(m,n,x,trig,factors) stafft.f90:187-287 and revfft stafft.f90:
+   290-406: the m vectors of length n in x(0:m*n-1) (vector index fastest)
+   transformed in place -- forward to the half-complex layout scaled by
+   1/sqrt(n), backward scaled by 2/sqrt(n) -- on the context's device, with
+   the caller's trig table and factors (those initfft gives for n). */
+int hyd_forfft(hyd_ctx *ctx, int32_t m, int32_t n, double *x, const double *trig,
+               const int32_t *factors);
+int hyd_revfft(hyd_ctx *ctx, int32_t m, int32_t n, double *x, const double *trig,
+               const int32_t *factors);
 
 /* ---- lib/stafft/sta2dfft.f90 (doubly periodic part) ---------------------- */
 /* ptospc(nx,ny,rvar,svar,...) sta2dfft.f90:103-132.  rvar(ny,nx) -> svar(nx,ny).
@@ -111,7 +128,12 @@ int hyd_energy(hyd_ctx *ctx, double *ekin, double *epot);
    3 fhi, 4 qdiss, 5 rksq (ng*ng each); 6 hrkx, 7 rkx, 8 hrky, 9 rky (ng);
    10 xtrig, 11 ytrig (2*ng); spectrum shells (spectral.f90:110-133): 12 spmf,
    13 alk (ng+1 each, alk(0) unused), 14 kmax, kmaxred, dsumi (3 values);
-   15 bety (ng). */
+   15 bety (ng); 16 rk, 17 kmag (ng*ng; kmag as whole numbers).
+   Tables of init_contours (contours.f90:48-132), integers as whole
+   numbers: 18 xgf (4ng), 19 ygf (4ng), 20 xgu (16ng), 21 ygu (16ng), 22
+   the congen weights w00, w10, w01, w11 one after another (16x16 each, iyf
+   fastest), 23 ixp (ng), 24 iyp (ng), 25 ixfw, ix0w, ix1w (16ng each), 26
+   iyfw, iy0w, iy1w (16ng each), 27 ibx (16ng+1), 28 iby (16ng+1). */
 int hyd_get_table(hyd_ctx *ctx, int32_t id, double *out);
 
 /* ---- ca/plane/qg/caps/contours.f90 ---------------------------------------- */
@@ -150,7 +172,7 @@ int hyd_sum_sequential(hyd_ctx *ctx, const double *a, int64_t n, double *sum);
 
 /* ---- device-resident state and the RK4 step ------------------------------ */
 /* caps.f90:143-183 (loopinit=0): read the full PV qq(ny,nx) at time t0,
-   set qjump (contint, generic.f90:216-229), qs = ptospc(qq), no contours. */
+   set qjump (contint, generic.f90:66-79), qs = ptospc(qq), no contours. */
 int hyd_set_field(hyd_ctx *ctx, const double *qq, double t0);
 /* Replace the device contour store (contours.f90:13-15). */
 int hyd_set_contours(hyd_ctx *ctx, int32_t nptq, int32_t nq, const double *xq, const double *yq,

@@ -66,6 +66,8 @@ def lib():
         for fn in ("hyo_forfft", "hyo_revfft"):
             getattr(L, fn).argtypes = [C.c_int, C.c_int, _D, _D, _I]
         L.hyo_initfft.argtypes = [C.c_int, _I, _D]
+        for fn in ("hyo_xderiv", "hyo_yderiv"):
+            getattr(L, fn).argtypes = [C.c_int, C.c_int, _D, _D, _D]
         L.hyo_ptospc.argtypes = [C.c_void_p, _D, _D]
         L.hyo_spctop.argtypes = [C.c_void_p, _D, _D]
         L.hyo_main_invert.argtypes = [C.c_void_p, _D, _D, _D, _D]

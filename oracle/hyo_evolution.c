@@ -47,7 +47,7 @@ int hyo_init_from_field(hyo_model *m, const double *qq_init, double t0) {
       for (int iy = 0; iy < m->ny; iy++) m->qr[(long)ix * m->ny + iy] -= m->bety[iy];
     m->qjump = m->p.beta * m->p.elly / (double)m->p.ncontq;
   } else {
-    /* generic.f90:216-229 contint */
+    /* generic.f90:66-79 contint */
     double mx = m->qr[0], mn = m->qr[0];
     for (long k = 0; k < N; k++) {
       mx = fmax_f(mx, m->qr[k]);

@@ -182,7 +182,7 @@ hyo_model *hyo_create(const hyo_params *p) {
   m->nyf = m->mgf * ny;
   m->nxu = m->mgu * nx;
   m->nyu = m->mgu * ny;
-  /* constants.f90:104-133 */
+  /* constants.f90:44-75 */
   const double small = 1.0e-12;
   m->hlx = 0.5 * p->ellx;
   m->hlxi = 1.0 / m->hlx;
@@ -252,7 +252,7 @@ hyo_model *hyo_create(const hyo_params *p) {
   m->bety = (double *)xcalloc(ny, sizeof(double));
   m->xtrig = (double *)xcalloc(2 * (size_t)nx, sizeof(double));
   m->ytrig = (double *)xcalloc(2 * (size_t)ny, sizeof(double));
-  /* capacities: constants.f90:95-101 scaled to a run-time default */
+  /* capacities: constants.f90:34-41 scaled to a run-time default */
   /* constants.f90:35-37 */
   m->npm = p->npm > 0 ? p->npm : (int64_t)500 * nx * ny;
   m->nm = m->npm / 20 + m->npm / 200;
@@ -286,7 +286,7 @@ void hyo_destroy(hyo_model *m) {
 int hyo_error(const hyo_model *m) { return m ? m->err : HYO_ERR_ARG; }
 
 /* Sequential sum in storage order, as gfortran (and amdflang -O3) evaluate
-   SUM() of a contiguous array (generic.f90:173,190,208; evolution.f90:553). */
+   SUM() of a contiguous array (generic.f90:23,40,58; evolution.f90:553,629,640). */
 double hyo_sum_seq(const double *a, int64_t n) {
   double s = 0.0;
   for (int64_t i = 0; i < n; i++) s = s + a[i];
