@@ -31,6 +31,8 @@
 //     same way (heads added in order, rejected segments summed in order).
 // Every accepted translation is exact by construction and every rejected
 // piece is summed in order, so the result is the sequential sum bit for bit.
+#include <mutex>
+
 #include "hyd_internal.h"
 
 namespace hyd {
@@ -694,8 +696,64 @@ int seqsum(Ctx &c, const double *a, int64_t n, double *dout) {
   return seqsum_launch(a, n, dout, (char *)c.seq_work, L, c.s);
 }
 
+// ---- the sequential sum without allocation ---------------------------------
+// Work area in a file-scope device array: each GPU the library runs on holds
+// its own copy (15.05 MiB used of 16 MiB for the largest sum, 2^22 values)
+// until the process ends.  Host code never passes `scratch` itself to a
+// kernel (in host code the name is a host-side stand-in): each device's
+// address comes from hipGetSymbolAddress, taken once per device, with that
+// device current, on the first call there.
+static constexpr int64_t kNoAllocMaxN = (int64_t)1 << 22;
+static constexpr int64_t kNoAllocBytes = (int64_t)16 << 20;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+static int scratch_address(void **out) {
+  int dev = 0;
+  HYD_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("sum_sequential_noalloc: device ordinal beyond the library's table");
+    return HYD_EARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HYD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error("sum_sequential_noalloc: no device address for the work area");
+      return HYD_EHIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return HYD_OK;
+}
+
 int64_t seqsum_work_chunks(int64_t n) { return (n + kChunk - 1) / kChunk; }
 
 int64_t seqsum_work_bytes(int64_t n) { return seq_layout(n).bytes; }
 
 }  // namespace hyd
+
+extern "C" int hyd_sum_sequential_noalloc(const double *a, int64_t n, double *sum, void *stream) {
+  using namespace hyd;
+  if (n < 0 || n > kNoAllocMaxN) {
+    set_error("sum_sequential_noalloc: length outside [0, 2^22]");
+    return n < 0 ? HYD_EARG : HYD_ECAPACITY;
+  }
+  if (!sum || (n > 0 && !a)) {
+    set_error("sum_sequential_noalloc: null device pointer");
+    return HYD_EARG;
+  }
+  const SeqLayout L = seq_layout(n);
+  if (L.bytes > kNoAllocBytes) {
+    set_error("sum_sequential_noalloc: work area larger than the static scratch");
+    return HYD_ECAPACITY;
+  }
+  void *w = nullptr;
+  HYD_CHECK(scratch_address(&w));
+  return seqsum_launch(a, n, sum, (char *)w, L, (hipStream_t)stream);
+}

@@ -169,6 +169,16 @@ int hyd_recontour(hyd_ctx *ctx);
    produce: ((a1 + a2) + a3) + ... in storage order, bit for bit.  Used for
    qavg0 (contours.f90:331-336) and zzrms (evolution.f90:553). */
 int hyd_sum_sequential(hyd_ctx *ctx, const double *a, int64_t n, double *sum);
+/* The same sequential sum without any allocation and without a context:
+   a (n doubles) and sum (one double) are device pointers, the work runs on
+   stream (a hipStream_t; NULL = the null stream) and the call returns
+   without waiting.  No counterpart in the reference: it is for callers that
+   cannot allocate -- code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc.  The work area is a static device array
+   (16 MiB per GPU, held until the process ends); n is limited to 2^22
+   (HYD_ECAPACITY beyond).  All calls on one GPU share that array, so calls
+   that overlap on one GPU (different streams, or threads) race. */
+int hyd_sum_sequential_noalloc(const double *a, int64_t n, double *sum, void *stream);
 
 /* ---- device-resident state and the RK4 step ------------------------------ */
 /* caps.f90:143-183 (loopinit=0): read the full PV qq(ny,nx) at time t0,
