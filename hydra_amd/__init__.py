@@ -102,6 +102,15 @@ def lib():
                                  C.POINTER(C.c_int32)]
         L.hyd_run.argtypes = [C.c_void_p, C.c_int32]
         L.hyd_phase_times.argtypes = [C.c_void_p, _D, C.c_int32]
+        L.hyd_take_records.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _D]
+        L.hyd_set_saves.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_int32]
+        L.hyd_pending_saves.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.hyd_take_grid_save.argtypes = [C.c_void_p, _D, _D]
+        L.hyd_contour_save_size.argtypes = [C.c_void_p, C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_int32)]
+        L.hyd_take_contour_save.argtypes = [C.c_void_p, _D, _D, _D, _D, _I, _I, _I]
+        L.hyd_save_now.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.hyd_save_counters.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.hyd_run_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int32]
         L.hyd_get_tuning.argtypes = [C.c_void_p, _I]
         L.hyd_copy_calibration.argtypes = [C.c_void_p, C.c_int64]
@@ -356,6 +365,59 @@ class Hydra:
 
     def evolution_init(self):
         _check(self.L.hyd_evolution_init(self.h))
+
+    def take_records(self, kind):
+        """Rows since the last call: "monitor" (t, cfl, zzrms^2/2, zzrms,
+        zzmax, uumax, twist) or "complexity" (t, nq, nptq)."""
+        k, w = {"monitor": (0, 7), "complexity": (1, 3)}[kind]
+        rows = []
+        while True:
+            buf = np.zeros(256 * w)
+            n = self.L.hyd_take_records(self.h, k, 256, buf)
+            if n < 0:
+                _check(n)
+            rows.append(buf[:n * w].reshape(n, w))
+            if n < 256:
+                return np.concatenate(rows)
+
+    def set_saves(self, on=True, tinit=0.0, igrec=0, icrec=1):
+        """Enable the reference's save schedule (evolution.f90:587-599)."""
+        _check(self.L.hyd_set_saves(self.h, int(bool(on)), float(tinit), int(igrec), int(icrec)))
+
+    def save_counters(self):
+        """(igrec, icrec): the next grid and contour record numbers."""
+        g, c = C.c_int32(), C.c_int32()
+        _check(self.L.hyd_save_counters(self.h, C.byref(g), C.byref(c)))
+        return g.value, c.value
+
+    def pending_saves(self):
+        g, c = C.c_int32(), C.c_int32()
+        _check(self.L.hyd_pending_saves(self.h, C.byref(g), C.byref(c)))
+        return g.value, c.value
+
+    def take_grid_save(self):
+        """dict(tgrid, ekin, epot, igrec, qqs) of the oldest grid save."""
+        info = np.zeros(4)
+        qqs = np.empty((self.ng, self.ng))
+        _check(self.L.hyd_take_grid_save(self.h, info, qqs))
+        return dict(tgrid=info[0], ekin=info[1], epot=info[2], igrec=int(info[3]), qqs=qqs)
+
+    def take_contour_save(self):
+        """dict(t, qjump, icrec, contex, residual, x, y, npq, i1q, indq)."""
+        nq, npt = C.c_int32(), C.c_int32()
+        _check(self.L.hyd_contour_save_size(self.h, C.byref(nq), C.byref(npt)))
+        nq, npt = nq.value, npt.value
+        info = np.zeros(4)
+        res = np.empty((self.ng, self.ng))
+        x, y = np.empty(max(npt, 1)), np.empty(max(npt, 1))
+        a = [np.empty(max(nq, 1), np.int32) for _ in range(3)]
+        _check(self.L.hyd_take_contour_save(self.h, info, res, x, y, *a))
+        return dict(t=info[0], qjump=info[1], icrec=int(info[2]), contex=bool(info[3]),
+                    residual=res, x=x[:npt], y=y[:npt], npq=a[0][:nq], i1q=a[1][:nq],
+                    indq=a[2][:nq])
+
+    def save_now(self, grid=False, cont=False):
+        _check(self.L.hyd_save_now(self.h, int(bool(grid)), int(bool(cont))))
 
     def phase_times(self, reset=False):
         """Surgery and recontouring calls and wall seconds so far:

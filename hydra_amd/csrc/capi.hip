@@ -340,6 +340,12 @@ static int timed_surgery(Ctx &c) {
   return HYD_OK;
 }
 
+// complexity.asc row (evolution.f90:72, 111): t, nq, nptq
+static void complexity_record(Ctx &c) {
+  const double row[3] = {c.hsc->t, (double)c.nq, (double)c.nptq};
+  c.rec_complexity.insert(c.rec_complexity.end(), row, row + 3);
+}
+
 static int timed_recontour(Ctx &c) {
   const double t0 = wall_now();
   HYD_CHECK(recontour(c, c.qr));
@@ -356,6 +362,144 @@ using namespace hyd;
 extern "C" {
 
 const char *hyd_last_error(void) { return g_err.c_str(); }
+
+int hyd_set_saves(hyd_ctx *h, int32_t on, double tinit, int32_t igrec, int32_t icrec) {
+  Ctx &c = *C(h);
+  if (on && c.p.precision != 0) {
+    set_error("set_saves: the save schedule needs the plain precision (precision 0)");
+    return HYD_ESTATE;
+  }
+  if (igrec < 0 || icrec < 1) {
+    set_error("set_saves: igrec >= 0 and icrec >= 1");
+    return HYD_EARG;
+  }
+  c.saves_on = on != 0;
+  c.tinit = tinit;
+  c.igrec = igrec;
+  c.icrec = icrec;
+  c.gsave = c.csave = 0;
+  return HYD_OK;
+}
+
+int hyd_save_counters(hyd_ctx *h, int32_t *igrec, int32_t *icrec) {
+  Ctx &c = *C(h);
+  if (!igrec || !icrec) {
+    set_error("save_counters: null pointer");
+    return HYD_EARG;
+  }
+  *igrec = c.igrec;
+  *icrec = c.icrec;
+  return HYD_OK;
+}
+
+int hyd_pending_saves(hyd_ctx *h, int32_t *grid, int32_t *cont) {
+  Ctx &c = *C(h);
+  if (!grid || !cont) {
+    set_error("pending_saves: null pointer");
+    return HYD_EARG;
+  }
+  *grid = (int32_t)c.grid_saves.size();
+  *cont = (int32_t)c.cont_saves.size();
+  return HYD_OK;
+}
+
+int hyd_take_grid_save(hyd_ctx *h, double *info, double *qqs) {
+  Ctx &c = *C(h);
+  if (c.grid_saves.empty()) {
+    set_error("take_grid_save: no grid save waits");
+    return HYD_ESTATE;
+  }
+  if (!info || !qqs) {
+    set_error("take_grid_save: null pointer");
+    return HYD_EARG;
+  }
+  const Ctx::GridSave &g = c.grid_saves.front();
+  info[0] = g.tgrid;
+  info[1] = g.ekin;
+  info[2] = g.epot;
+  info[3] = g.igrec;
+  std::copy(g.qqs.begin(), g.qqs.end(), qqs);
+  c.grid_saves.erase(c.grid_saves.begin());
+  return HYD_OK;
+}
+
+int hyd_contour_save_size(hyd_ctx *h, int32_t *nq, int32_t *nptq) {
+  Ctx &c = *C(h);
+  if (c.cont_saves.empty()) {
+    set_error("contour_save_size: no contour save waits");
+    return HYD_ESTATE;
+  }
+  if (!nq || !nptq) {
+    set_error("contour_save_size: null pointer");
+    return HYD_EARG;
+  }
+  *nq = (int32_t)c.cont_saves.front().npq.size();
+  *nptq = (int32_t)c.cont_saves.front().xq.size();
+  return HYD_OK;
+}
+
+int hyd_take_contour_save(hyd_ctx *h, double *info, double *residual, double *xq, double *yq,
+                          int32_t *npq, int32_t *i1q, int32_t *indq) {
+  Ctx &c = *C(h);
+  if (c.cont_saves.empty()) {
+    set_error("take_contour_save: no contour save waits");
+    return HYD_ESTATE;
+  }
+  const Ctx::ContSave &s = c.cont_saves.front();
+  if (!info || !residual || (!s.xq.empty() && (!xq || !yq)) ||
+      (!s.npq.empty() && (!npq || !i1q || !indq))) {
+    set_error("take_contour_save: null pointer");
+    return HYD_EARG;
+  }
+  info[0] = s.t;
+  info[1] = s.qjump;
+  info[2] = s.icrec;
+  info[3] = s.contex;
+  std::copy(s.res.begin(), s.res.end(), residual);
+  std::copy(s.xq.begin(), s.xq.end(), xq);
+  std::copy(s.yq.begin(), s.yq.end(), yq);
+  std::copy(s.npq.begin(), s.npq.end(), npq);
+  std::copy(s.i1q.begin(), s.i1q.end(), i1q);
+  std::copy(s.indq.begin(), s.indq.end(), indq);
+  c.cont_saves.erase(c.cont_saves.begin());
+  return HYD_OK;
+}
+
+// The end of advect (evolution.f90:85-89): saves at tsim itself, after an
+// inversion of the current state.
+int hyd_save_now(hyd_ctx *h, int32_t grid, int32_t cont) {
+  Ctx &c = *C(h);
+  if (!c.saves_on || !c.have_field) {
+    set_error("save_now: needs a field and the save schedule (hyd_set_saves)");
+    return HYD_ESTATE;
+  }
+  if (grid && !c.qspre) {
+    set_error("save_now: no step has kept qs for a grid save yet");
+    return HYD_ESTATE;
+  }
+  if (!grid && !cont) return HYD_OK;
+  HYD_CHECK(inversion(c));
+  if (grid) c.tgrid = c.tinit + c.p.tgsave * (double)c.igrec;
+  HYD_CHECK(save_slots(c, grid != 0, cont != 0));
+  return HYD_OK;
+}
+
+int hyd_take_records(hyd_ctx *h, int32_t kind, int32_t max, double *out) {
+  Ctx &c = *C(h);
+  std::vector<double> *v = kind == 0 ? &c.rec_monitor : (kind == 1 ? &c.rec_complexity : nullptr);
+  if (!v || max < 0 || (max > 0 && !out)) {
+    set_error("take_records: kind 0 (monitor) or 1 (complexity), max >= 0");
+    return HYD_EARG;
+  }
+  const int w = kind == 0 ? 7 : 3;
+  const int64_t have = (int64_t)v->size() / w;
+  const int64_t n = std::min<int64_t>(have, max);
+  if (n > 0) {
+    std::copy(v->begin(), v->begin() + n * w, out);
+    v->erase(v->begin(), v->begin() + n * w);
+  }
+  return (int)n;
+}
 
 int hyd_phase_times(hyd_ctx *h, double *out, int32_t reset) {
   Ctx &c = *C(h);
@@ -919,6 +1063,12 @@ static int advect_start(Ctx &c) {
   HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
   c.ireg = 0;
   c.phase = 2;
+  c.gsave = c.csave = 0;  // init drops pending saves (evolution.f90:115-116)
+  if (c.saves_on) {       // and keeps qspre = qs (evolution.f90:126)
+    HYD_CHECK(dbuf(c, "qspre", c.N, &c.qspre));
+    HYD_HIP(hipMemcpyAsync(c.qspre, c.qs, sizeof(double) * c.N, hipMemcpyDeviceToDevice, c.s));
+  }
+  complexity_record(c);
   return HYD_OK;
 }
 
@@ -1029,12 +1179,19 @@ int hyd_advance(hyd_ctx *h) {
     set_error("advance before set_field");
     return HYD_ESTATE;
   }
+  const double t0 = c.hsc->t;
   if (c.p.precision == 1)
     HYD_CHECK(advance_mixed_graph(c));
   else
     HYD_CHECK(advance(c));
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
-  return c2g_status(c);
+  HYD_CHECK(c2g_status(c));
+  if (c.p.precision == 1) {  // adapt ran on the device: its row from the scalars
+    Scalars s = *c.hsc;
+    s.t = t0;
+    monitor_record(c, s);
+  }
+  return HYD_OK;
 }
 
 int hyd_surgery(hyd_ctx *h) {
@@ -1060,6 +1217,7 @@ static int advect_step(Ctx &c) {
     HYD_CHECK(timed_surgery(c));
     HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
     HYD_HIP(hipStreamSynchronize(c.s));
+    complexity_record(c);
     c.hsc->twist = c.hsc->twist - twistmax;
     HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
   }
@@ -1136,6 +1294,7 @@ int hyd_run_batch(hyd_ctx **hs, int32_t n, int32_t nsteps) {
   const double twistmax = 2.5;
   const int nregmax = 20;
   std::vector<int> done(n, 0), stepped(n, 0);
+  std::vector<double> tstart(n, 0.0);
   for (;;) {
     bool any = false;
     for (int i = 0; i < n; i++) {
@@ -1155,9 +1314,11 @@ int hyd_run_batch(hyd_ctx **hs, int32_t n, int32_t nsteps) {
         HYD_CHECK(timed_surgery(c));
         HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
         HYD_HIP(hipStreamSynchronize(c.s));
+        complexity_record(c);
         c.hsc->twist = c.hsc->twist - twistmax;
         HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
       }
+      tstart[i] = c.hsc->t;
       if (c.p.precision == 1)
         HYD_CHECK(advance_mixed_graph(c));
       else
@@ -1168,7 +1329,13 @@ int hyd_run_batch(hyd_ctx **hs, int32_t n, int32_t nsteps) {
     if (!any) break;
     for (int i = 0; i < n; i++) {
       if (!stepped[i]) continue;
-      HYD_CHECK(c2g_status(*C(hs[i])));  // waits for the flow's stream
+      Ctx &c = *C(hs[i]);
+      HYD_CHECK(c2g_status(c));  // waits for the flow's stream
+      if (c.p.precision == 1) {
+        Scalars sc = *c.hsc;
+        sc.t = tstart[i];
+        monitor_record(c, sc);
+      }
       done[i]++;
     }
   }

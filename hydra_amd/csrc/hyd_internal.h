@@ -160,6 +160,35 @@ struct Ctx {
   int nptq = 0, nq = 0;
   bool contex = false;
   int ireg = 0;  // regularisations since the last recontouring
+  // diagnostics records (monitor.asc rows, evolution.f90:581-584: t, cfl,
+  // zzrms^2/2, zzrms, zzmax, uumax, twist; complexity.asc rows,
+  // evolution.f90:72 and 111: t, nq, nptq) and the save schedule of adapt
+  // (evolution.f90:587-599): when a grid save falls in the step, qs and the
+  // energies at adapt are kept for the interpolation (evolution.f90:660-668)
+  std::vector<double> rec_monitor, rec_complexity;
+  bool saves_on = false;
+  double tinit = 0.0;
+  int32_t igrec = 0, icrec = 1;
+  int32_t gsave = 0, csave = 0;
+  double tgrid = 0.0, save_dt = 0.0, ekinpre = 0.0, epotpre = 0.0;
+  double *qspre = nullptr;  // named work buffer "qspre" once a grid save occurs
+  // saves made at the start of a step (evolution.f90:202-204), waiting for
+  // the caller: grid saves (tgrid, ekin, epot, igrec; qqs spectral) and
+  // contour saves (t, qjump, icrec, contex; the residual spctop(qq - qc),
+  // npq, i1q, indq and the nodes)
+  struct GridSave {
+    double tgrid, ekin, epot;
+    int32_t igrec;
+    std::vector<double> qqs;
+  };
+  struct ContSave {
+    double t, qjump;
+    int32_t icrec, contex;
+    std::vector<double> res, xq, yq;
+    std::vector<int32_t> npq, i1q, indq;
+  };
+  std::vector<GridSave> grid_saves;
+  std::vector<ContSave> cont_saves;
   // where a step-count run resumes: 0 = contours to rebuild from qr
   // (caps.f90 recont), 1 = contours built, advect's start to run,
   // 2 = inside advect's loop
@@ -236,6 +265,8 @@ int launch_yderiv(Ctx &c, const double *var, double *der);
 int seqsum(Ctx &c, const double *a, int64_t n, double *dout);
 // diagnostics (diag.hip)
 int diag_energy(Ctx &c, double *ekin, double *epot);
+void monitor_record(Ctx &c, const Scalars &s);
+int save_slots(Ctx &c, bool grid, bool cont);
 int diag_norms(Ctx &c, const double *qq, double *ql1, double *ql2);
 int diag_spec1d(Ctx &c, double *ss, double *spec, int iopt);
 void diag_free(Ctx &c);

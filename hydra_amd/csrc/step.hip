@@ -363,6 +363,106 @@ static void host_exp_table(const std::vector<double> &qu, double dfac, double *o
   ExpPool::get().run(qu.data(), qu.size(), dfac, out);
 }
 
+PW_KERNEL(k_interp, (double *__restrict__ out, const double *__restrict__ a,
+                     const double *__restrict__ b, double pa, double pb, int64_t n),
+          { out[k] = pa * a[k] + pb * b[k]; })
+
+PW_KERNEL(k_sub, (double *__restrict__ out, const double *__restrict__ a,
+                  const double *__restrict__ b, int64_t n),
+          { out[k] = a[k] - b[k]; })
+
+// savegrid's and savecont's inputs (evolution.f90:648-668, 735-778) at the
+// start of a step, after its inversion: the PV and energies interpolated to
+// tgrid from the kept qspre/ekinpre/epotpre and this inversion; the
+// contours and the residual spctop(qq - qc).  Kept on the host for the
+// caller, who writes the files; the record counters advance as the
+// reference's do.  Main stream, before the second stream forks.
+int save_slots(Ctx &c, bool grid, bool cont) {
+  const int64_t N = c.N;
+  const double t = c.hsc->t;
+  if (grid) {
+    Ctx::GridSave g;
+    const double pt = (t - c.tgrid) / c.save_dt, ptc = 1.0 - pt;
+    double *qqs;
+    HYD_CHECK(dbuf(c, "qqs_slot", N, &qqs));
+    hipLaunchKernelGGL(k_interp, grid1(N), dim3(256), 0, c.s, qqs, c.qspre, c.qq, pt, ptc, N);
+    HYD_HIP(hipGetLastError());
+    double ekinpost = 0.0, epotpost = 0.0;
+    HYD_CHECK(diag_energy(c, &ekinpost, &epotpost));
+    g.tgrid = c.tgrid;
+    g.ekin = pt * c.ekinpre + ptc * ekinpost;
+    g.epot = pt * c.epotpre + ptc * epotpost;
+    g.igrec = c.igrec;
+    g.qqs.resize(N);
+    HYD_HIP(hipMemcpyAsync(g.qqs.data(), qqs, sizeof(double) * N, hipMemcpyDeviceToHost, c.s));
+    HYD_HIP(hipStreamSynchronize(c.s));
+    c.grid_saves.push_back(std::move(g));
+    c.igrec += 1;
+    c.gsave = 0;
+  }
+  if (cont) {
+    Ctx::ContSave s;
+    s.t = t;
+    s.qjump = c.hsc->qjump;
+    s.icrec = c.icrec;
+    s.contex = c.contex ? 1 : 0;
+    double *ss, *qa;
+    HYD_CHECK(dbuf(c, "res_ss", N, &ss));
+    HYD_CHECK(dbuf(c, "res_qa", N, &qa));
+    hipLaunchKernelGGL(k_sub, grid1(N), dim3(256), 0, c.s, ss, c.qq, c.qc, N);
+    HYD_HIP(hipGetLastError());
+    HYD_CHECK(fft_spctop(c, ss, qa, PRE_NONE, 1.0, c.wc));
+    s.res.resize(N);
+    HYD_HIP(hipMemcpyAsync(s.res.data(), qa, sizeof(double) * N, hipMemcpyDeviceToHost, c.s));
+    if (s.contex) {
+      s.xq.resize(c.nptq);
+      s.yq.resize(c.nptq);
+      HYD_HIP(hipMemcpyAsync(s.xq.data(), c.xq, sizeof(double) * c.nptq, hipMemcpyDeviceToHost,
+                             c.s));
+      HYD_HIP(hipMemcpyAsync(s.yq.data(), c.yq, sizeof(double) * c.nptq, hipMemcpyDeviceToHost,
+                             c.s));
+      s.npq = c.npq;
+      s.i1q = c.i1q;
+      s.indq = c.indq;
+    }
+    HYD_HIP(hipStreamSynchronize(c.s));
+    c.cont_saves.push_back(std::move(s));
+    c.icrec += 1;
+    c.csave = 0;
+  }
+  return HYD_OK;
+}
+
+// monitor.asc row of adapt (evolution.f90:581-584), from the scalars it set
+void monitor_record(Ctx &c, const Scalars &s) {
+  const double cfl = s.uumax * s.dt / c.c.glx;
+  const double row[7] = {s.t, cfl, 0.5 * (s.zzrms * s.zzrms), s.zzrms, s.zzmax, s.uumax, s.twist};
+  c.rec_monitor.insert(c.rec_monitor.end(), row, row + 7);
+}
+
+// evolution.f90:587-599: a grid save falls in this step when t + dt >= tgrid;
+// then qs (reset at the step's start) and the energy of this inversion are
+// kept for the interpolation to tgrid, which the caller makes after the
+// step; a contour save falls in it when t + dt >= tcont.  Runs on the main
+// stream before the second stream forks (the energy's sums use the shared
+// sequential-sum work area).
+static int adapt_saves(Ctx &c) {
+  HYD_HIP(hipEventSynchronize(c.ev_sc));
+  const double t = c.hsc->t, dt = c.hsc->dt;
+  c.save_dt = dt;
+  const double tgrid = c.tinit + c.p.tgsave * (double)c.igrec;
+  c.gsave = t + dt >= tgrid ? 1 : 0;
+  if (c.gsave) {
+    c.tgrid = tgrid;
+    HYD_CHECK(dbuf(c, "qspre", c.N, &c.qspre));
+    HYD_HIP(hipMemcpyAsync(c.qspre, c.qs, sizeof(double) * c.N, hipMemcpyDeviceToDevice, c.s));
+    HYD_CHECK(diag_energy(c, &c.ekinpre, &c.epotpre));
+  }
+  const double tcont = c.tinit + c.p.tcsave * (double)c.icrec;
+  c.csave = t + dt >= tcont ? 1 : 0;
+  return HYD_OK;
+}
+
 int adapt(Ctx &c) {
   const int64_t N = c.N;
   double *ss = c.wa, *zz = c.wb;
@@ -396,6 +496,7 @@ int adapt(Ctx &c) {
   // the sources of stage 1 (which do not use them)
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
   HYD_HIP(hipEventRecord(c.ev_sc, c.s));
+  if (c.saves_on) HYD_CHECK(adapt_saves(c));
   return HYD_OK;
 }
 
@@ -405,6 +506,7 @@ int adapt(Ctx &c) {
 int adapt_finish(Ctx &c) {
   const int64_t N = c.N;
   HYD_HIP(hipEventSynchronize(c.ev_sc));
+  monitor_record(c, *c.hsc);
   const double dfac = c.hsc->dt2 * c.hsc->zzrms;
   host_exp_table(c.ht.qdiss_u, dfac, c.exp_host);
   HYD_HIP(hipMemcpyAsync(c.qdiss_exp, c.exp_host, sizeof(double) * c.ht.qdiss_u.size(),
@@ -516,6 +618,8 @@ int advance(Ctx &c) {
   // stage 1
   HYD_CHECK(inversion(c));
   hipLaunchKernelGGL(k_reset, grid1(N), dim3(256), 0, c.s, c.qs, c.qd, c.qc, c.flo, c.fhi, N);
+  // saves that fall at this step's start (set by the previous adapt)
+  if (c.saves_on && (c.gsave || c.csave)) HYD_CHECK(save_slots(c, c.gsave != 0, c.csave != 0));
   HYD_CHECK(adapt(c));
   HYD_CHECK(fork_s2(c));
   HYD_CHECK(on_s2(c, [&]() { return source(c, 0); }));

@@ -139,6 +139,18 @@ def fmt_spectra_line(alk, lspec):
     return f" {_f(alk, 12, 8)} {_f(lspec, 12, 8)}"
 
 
+def fmt_complexity(t, nq, nptq):
+    """complexity.asc, evolution.f90:72 and 111: (1x,f12.5,1x,i9,1x,i10)."""
+    return f" {_f(t, 12, 5)} {_i(nq, 9)} {_i(nptq, 10)}"
+
+
+def fmt_monitor(t, cfl, ens, zzrms, zzmax, uumax, twist):
+    """monitor.asc, evolution.f90:583-584:
+    (1x,f12.5,1x,f6.4,4(1x,f12.6),1x,f5.3)."""
+    return (f" {_f(t, 12, 5)} {_f(cfl, 6, 4)} {_f(ens, 12, 6)} {_f(zzrms, 12, 6)} "
+            f"{_f(zzmax, 12, 6)} {_f(uumax, 12, 6)} {_f(twist, 5, 3)}")
+
+
 # ---- contours ---------------------------------------------------------------------
 
 def savecont(outdir, icrec, t, contours, residual):
@@ -246,3 +258,46 @@ def savegrid(h, outdir, igrec):
             for k in range(1, kmaxred + 1):
                 f.write(fmt_spectra_line(alk[k], np.log10(qspec[k])) + "\n")
     return igrec + 1
+
+
+def savegrid_slot(h, outdir, g):
+    """evolution.f90:648-729 from a grid save of the step loop
+    (Hydra.take_grid_save: the PV qqs and the energies already interpolated
+    to tgrid): appends to ene.asc, norms.asc and spectra.asc and writes
+    record igrec+1 of qq.r4.  h supplies the tables and transforms."""
+    os.makedirs(outdir, exist_ok=True)
+    tgrid, qqs = g["tgrid"], g["qqs"]
+    with open(os.path.join(outdir, "ene.asc"), "a") as f:
+        f.write(fmt_ene(tgrid, g["ekin"], g["epot"]) + "\n")
+    kmax, kmaxred, dsumi = h.table("shells")
+    kmax, kmaxred = int(kmax), int(kmaxred)
+    spmf, alk = h.table("spmf"), h.table("alk")
+    qspec, _ = h.spec1d(qqs, 0)
+    sumqspec = 0.0
+    for k in range(1, kmax + 1):
+        sumqspec = sumqspec + qspec[k]
+        qspec[k] = spmf[k] * qspec[k]
+    sumqspec = 8.0 * sumqspec * dsumi
+    wka = h.spctop(qqs)
+    bety = h.table("bety")
+    write_r4_record(os.path.join(outdir, "qq.r4"), g["igrec"] + 1, tgrid,
+                    (wka + bety[None, :]).astype(np.float32))
+    _, qql2 = h.norms(wka)
+    with open(os.path.join(outdir, "norms.asc"), "a") as f:
+        f.write(fmt_norms(tgrid, qql2) + "\n")
+    with open(os.path.join(outdir, "spectra.asc"), "a") as f:
+        f.write(fmt_spectra_head(tgrid, sumqspec, qql2, kmaxred) + "\n")
+        with np.errstate(divide="ignore"):
+            for k in range(1, kmaxred + 1):
+                f.write(fmt_spectra_line(alk[k], np.log10(qspec[k])) + "\n")
+
+
+def savecont_slot(outdir, s):
+    """evolution.f90:735-778 from a contour save of the step loop
+    (Hydra.take_contour_save)."""
+    c = dict(x=s["x"], y=s["y"], npq=s["npq"], i1q=s["i1q"], indq=s["indq"], qjump=s["qjump"])
+    if not s["contex"]:
+        c.update(x=np.zeros(0), y=np.zeros(0), npq=np.zeros(0, np.int32),
+                 i1q=np.zeros(0, np.int32), indq=np.zeros(0, np.int32))
+    savecont(outdir, s["icrec"], s["t"], c, s["residual"])
+

@@ -238,6 +238,46 @@ int hyd_run(hyd_ctx *ctx, int32_t nsteps);
    reading; out may be NULL. */
 int hyd_phase_times(hyd_ctx *ctx, double *out, int32_t reset);
 
+/* ---- diagnostics records and saves (evolution.f90:72, 111, 581-599) ------ */
+/* The rows the reference writes each step and at each regularisation,
+   oldest first, removed as they are taken: kind 0 monitor.asc (adapt,
+   evolution.f90:581-584): t, cfl = uumax*dt/glx, zzrms**2/2, zzrms, zzmax,
+   uumax, twist (7 values a row); kind 1 complexity.asc (evolution.f90:72,
+   111): t, nq, nptq (3 values).  Returns the number of rows copied (at
+   most max). */
+int hyd_take_records(hyd_ctx *ctx, int32_t kind, int32_t max, double *out);
+/* The save schedule of adapt (evolution.f90:587-599, caps.f90:206-238
+   counters): on != 0 enables it from time tinit with grid record igrec and
+   contour record icrec (plain precision only).  Each step's adapt then
+   decides gsave (t+dt >= tinit+tgsave*igrec) and csave (t+dt >=
+   tinit+tcsave*icrec); with gsave it keeps qs and the energies of that
+   inversion.  The saves fall at the start of the next step, after its
+   inversion (evolution.f90:193-204): the library computes their contents
+   there -- qq and the energies interpolated to tgrid (evolution.f90:
+   660-668); the contours and the residual spctop(qq - qc) (evolution.f90:
+   750-766) -- and keeps them, in order, for the caller, who writes the
+   files; the record counters advance as the reference's. */
+int hyd_set_saves(hyd_ctx *ctx, int32_t on, double tinit, int32_t igrec, int32_t icrec);
+/* The record counters: the next grid record igrec (0-based, qq.r4 writes
+   igrec+1) and the next contour record icrec (1-based). */
+int hyd_save_counters(hyd_ctx *ctx, int32_t *igrec, int32_t *icrec);
+/* Numbers of grid and contour saves waiting. */
+int hyd_pending_saves(hyd_ctx *ctx, int32_t *grid, int32_t *cont);
+/* The oldest grid save: info[4] = tgrid, ekin, epot, igrec (0-based record
+   before the increment); qqs = the PV interpolated to tgrid (spectral,
+   ng*ng).  HYD_ESTATE when none waits. */
+int hyd_take_grid_save(hyd_ctx *ctx, double *info, double *qqs);
+/* The oldest contour save's sizes, then its contents: info[4] = t, qjump,
+   icrec, contex; residual (ng*ng physical, spctop(qq - qc)); the nodes and
+   npq, i1q, indq as savecont writes them. */
+int hyd_contour_save_size(hyd_ctx *ctx, int32_t *nq, int32_t *nptq);
+int hyd_take_contour_save(hyd_ctx *ctx, double *info, double *residual, double *xq, double *yq,
+                          int32_t *npq, int32_t *i1q, int32_t *indq);
+/* The end of advect (evolution.f90:85-89): grid and/or contour saves of the
+   current state, after an inversion (the grid save interpolates to the next
+   save time from the last kept qs). */
+int hyd_save_now(hyd_ctx *ctx, int32_t grid, int32_t cont);
+
 /* An ensemble of n independent flows (distinct contexts, any sizes and
  * precisions), each advanced by hyd_run's loop for nsteps steps.  The flows'
  * RK4 steps of one round run concurrently on their own streams; each flow's

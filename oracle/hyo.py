@@ -96,10 +96,12 @@ def lib():
 
 
 def default_params(ng, kd=0.0, ncontq=80, dt_fixed=0.0, npm=0, tsave=1000.0, cdamp=2.0,
-                   ell_cells=0.0):
+                   ell_cells=0.0, tgsave=None, tcsave=None):
     return HyoParams(nx=ng, ny=ng, ncontq=ncontq, ellx=2 * np.pi, elly=2 * np.pi, kd=kd,
-                     beta=0.0, rtherm=0.0, rekman=0.0, cdamp=cdamp, nnu=3.0, tgsave=tsave,
-                     tcsave=tsave, dt_fixed=dt_fixed, npm=npm, ell_cells=ell_cells)
+                     beta=0.0, rtherm=0.0, rekman=0.0, cdamp=cdamp, nnu=3.0,
+                     tgsave=tsave if tgsave is None else tgsave,
+                     tcsave=tsave if tcsave is None else tcsave, dt_fixed=dt_fixed, npm=npm,
+                     ell_cells=ell_cells)
 
 
 class Oracle:
@@ -484,7 +486,7 @@ class Ref:
     def format(self, kind, v=(), iv=()):
         """The reference's output formats (see ref_shim.f90 ref_format)."""
         self.L.ref_format.argtypes = [C.c_int, _D, _I, C.c_char_p]
-        va = np.zeros(4)
+        va = np.zeros(8)
         va[:len(v)] = v
         ia = np.zeros(2, np.int32)
         ia[:len(iv)] = iv

@@ -229,10 +229,12 @@ end subroutine
 
 ! The formats of evolution.f90:676-728 and 750, written to line (len 128):
 ! kind 1 ene.asc, 2 norms.asc, 3 spectra.asc header, 4 spectra.asc line,
-! 5 qqsynopsis.asc.  v holds the reals, iv the integers in statement order.
+! 5 qqsynopsis.asc, 6 complexity.asc (evolution.f90:72, 111), 7 monitor.asc
+! (evolution.f90:583-584).  v holds the reals, iv the integers in statement
+! order.
 subroutine ref_format(kind, v, iv, line) bind(c, name='ref_format')
   integer(c_int), value:: kind
-  real(c_double):: v(4)
+  real(c_double):: v(8)
   integer(c_int):: iv(2)
   character(kind=c_char):: line(128)
   character(len=128):: buf
@@ -244,6 +246,8 @@ subroutine ref_format(kind, v, iv, line) bind(c, name='ref_format')
   case (3); write(buf,'(f9.2,2(1x,f16.9),1x,i5)') v(1),v(2),v(3),iv(1)
   case (4); write(buf,'(2(1x,f12.8))') v(1),v(2)
   case (5); write(buf,'(i9,1x,i10,1x,f12.5,1x,f16.12)') iv(1),iv(2),v(1),v(2)
+  case (6); write(buf,'(1x,f12.5,1x,i9,1x,i10)') v(1),iv(1),iv(2)
+  case (7); write(buf,'(1x,f12.5,1x,f6.4,4(1x,f12.6),1x,f5.3)') v(1),v(2),v(3),v(4),v(5),v(6),v(7)
   end select
   do i = 1, 128
     line(i) = buf(i:i)

@@ -28,6 +28,11 @@ FMT_CASES = [
     (4, (1.0, float("-inf")), ()),
     (4, (0.5, float("nan")), ()),
     (2, (12345678.0, 1.0), ()),
+    (6, (0.0,), (264, 7805)),
+    (6, (1.23456789,), (1321, 473547)),
+    (7, (0.0, 0.0, 78.95683520871, 12.56637061436, 40.5, 3.25, 0.25), ()),
+    (7, (0.21698, 0.6999999, 123.4567891, 15.7, 61.234567, 3.0000001, 2.5009), ()),
+    (7, (1.5, 12.5, 1.0, 1.0, 1.0, 1.0, 12.25), ()),
 ]
 
 
@@ -40,6 +45,10 @@ def python_line(kind, v, iv):
         return io.fmt_spectra_head(v[0], v[1], v[2], iv[0])
     if kind == 4:
         return io.fmt_spectra_line(v[0], v[1])
+    if kind == 6:
+        return io.fmt_complexity(v[0], iv[0], iv[1])
+    if kind == 7:
+        return io.fmt_monitor(*v[:7])
     return io.fmt_synopsis(iv[0], iv[1], v[0], v[1])
 
 
@@ -102,6 +111,9 @@ def test_formats_against_reference_build():
         assert io.fmt_norms(t, abs(v[1])) == r.format(2, (t, abs(v[1])))
         assert io.fmt_spectra_line(v[2], v[3]) == r.format(4, (v[2], v[3]))
         assert io.fmt_synopsis(7, 1234, t, abs(v[3])) == r.format(5, (t, abs(v[3])), (7, 1234))
+        assert io.fmt_complexity(t, 17, 123456) == r.format(6, (t,), (17, 123456))
+        m = tuple(abs(x) for x in rng.standard_normal(7) * 10.0 ** rng.integers(-3, 2, size=7))
+        assert io.fmt_monitor(*m) == r.format(7, m), m
 
 
 def test_qq_init_read_by_reference(tmp_path):
