@@ -133,6 +133,10 @@ def lib():
         L.hyd_cnn_set_weights.argtypes = [C.c_void_p, _F]
         _U16 = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
         L.hyd_cnn_mfma_probe.argtypes = [_U16, _U16, _F]
+        L.hyd_set_correction.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                         C.c_double]
+        L.hyd_contour_velocity.argtypes = [C.c_void_p, C.c_int32, _D, _D, _I, C.c_double,
+                                           C.c_int32, _D, _D, _D, _D, C.c_void_p]
         _lib = L
     return _lib
 
@@ -302,6 +306,29 @@ class Hydra:
         _check(self.L.hyd_con2grid(self.h, xq, yq, nextq, len(xq), float(qjump), qc,
                                    cnt.ctypes.data_as(C.c_void_p) if with_counts else None))
         return (qc, cnt) if with_counts else qc
+
+    def contour_velocity(self, xq, yq, nextq, qjump, xs, ys, fields=False):
+        """Velocity of the contours' PV on this context's grid (con2grid,
+        ptospc, main_invert) at the points (xs, ys); with fields=True also
+        the FP32 (u, v, q) grid [ng*ng, 3].  The context's state is not
+        touched."""
+        xq, yq, xs, ys = _f64(xq), _f64(yq), _f64(xs), _f64(ys)
+        nextq = np.ascontiguousarray(nextq, dtype=np.int32)
+        m = len(xs)
+        us, vs = np.empty(m), np.empty(m)
+        f = np.empty((self.ng * self.ng, 3), np.float32) if fields else None
+        _check(self.L.hyd_contour_velocity(self.h, len(xq), xq, yq, nextq, float(qjump), m, xs,
+                                           ys, us, vs,
+                                           f.ctypes.data_as(C.c_void_p) if fields else None))
+        return (us, vs, f) if fields else (us, vs)
+
+    def set_correction(self, net, su=1.0, sq=1.0, sd=1.0):
+        """Attach a trained VelocityCNN (None detaches): each RK4 stage adds
+        its (du, dv) at the nodes to velint's velocity; inputs are scaled by
+        1/su (velocity) and 1/sq (PV), the output by sd."""
+        _check(self.L.hyd_set_correction(self.h, net.h if net is not None else None,
+                                         float(su), float(sq), float(sd)))
+        self._net = net
 
     def renode(self, xd, yd, cap=None):
         """contours.f90:687-933: returns (xr, yr); empty when the contour is dropped."""
@@ -555,8 +582,8 @@ class VelocityCNN:
 
 
 def mfma_probe(a_bf16, b_bf16):
-    """D = A B of one 16x16x16 BF16 matrix-core block (A row-major,
-    b_bf16[n][k]); returns D (16x16 FP32)."""
+    """D = A B of one 16x16x32 BF16 matrix-core block (A row-major
+    [16][32], b_bf16[n][k] [16][32]); returns D (16x16 FP32)."""
     d = np.empty(256, np.float32)
     _check(lib().hyd_cnn_mfma_probe(np.ascontiguousarray(a_bf16, np.uint16).ravel(),
                                     np.ascontiguousarray(b_bf16, np.uint16).ravel(), d))

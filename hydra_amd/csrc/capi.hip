@@ -501,6 +501,27 @@ int hyd_take_records(hyd_ctx *h, int32_t kind, int32_t max, double *out) {
   return (int)n;
 }
 
+int hyd_set_correction(hyd_ctx *h, hyd_cnn *net, double su, double sq, double sd) {
+  Ctx &c = *C(h);
+  if (net && (cnn_grid(net) != c.c.nx || !(su > 0.0) || !(sq > 0.0) || !(sd > 0.0))) {
+    set_error("set_correction: the network's grid must be the context's, scales > 0");
+    return HYD_EARG;
+  }
+  if (net) {  // the stage's work buffers, allocated here and never inside a captured step
+    double *q, *tmp;
+    HYD_CHECK(dbuf(c, "cnn_q", c.N, &q));
+    HYD_CHECK(dbuf(c, "cnn_tmp", c.N, &tmp));
+    HYD_CHECK(dbuf(c, "cnn_du", c.cap, &c.cdu));
+    HYD_CHECK(dbuf(c, "cnn_dv", c.cap, &c.cdv));
+  }
+  c.net = net;
+  c.net_su = su;
+  c.net_sq = sq;
+  c.net_sd = sd;
+  c.graph_nptq = -1;  // the captured step changes
+  return HYD_OK;
+}
+
 int hyd_phase_times(hyd_ctx *h, double *out, int32_t reset) {
   Ctx &c = *C(h);
   if (out) {
@@ -879,6 +900,65 @@ int hyd_con2grid(hyd_ctx *h, const double *xq, const double *yq, const int32_t *
   c.hsc->qjump = saved;
   HYD_CHECK(rc);
   HYD_CHECK(d2h(c, qc, c.wa, c.N));
+  return c2g_status(c);
+}
+
+// The velocity that the contours' PV field has on this context's grid:
+// con2grid, ptospc, main_invert (contours.f90:173-346, spectral.f90:304-342)
+// on work buffers (the context's state is untouched), then velint at m
+// points.  fields (may be NULL) receives u, v and the PV grid q as FP32
+// [pixels][3], pixel = ix*ny + iy.
+int hyd_contour_velocity(hyd_ctx *h, int32_t nptq, const double *xq, const double *yq,
+                         const int32_t *nextq, double qjump, int32_t m, const double *xs,
+                         const double *ys, double *us, double *vs, float *fields) {
+  Ctx &c = *C(h);
+  if (nptq < 1 || nptq > c.cap || m < 0 || m > c.cap) {
+    set_error("contour_velocity: node or point count outside [1, capacity]");
+    return HYD_ECAPACITY;
+  }
+  HYD_CHECK(check_links(nextq, nptq));
+  HYD_CHECK(check_positions(c.c, xq, yq, nptq));
+  if (m > 0) HYD_CHECK(check_positions(c.c, xs, ys, m));
+  const int64_t N = c.N;
+  double *q, *qs, *u, *v, *pp, *tmp, *dxs, *dys, *dus, *dvs;
+  HYD_CHECK(dbuf(c, "cv_q", N, &q));
+  HYD_CHECK(dbuf(c, "cv_qs", N, &qs));
+  HYD_CHECK(dbuf(c, "cv_u", N, &u));
+  HYD_CHECK(dbuf(c, "cv_v", N, &v));
+  HYD_CHECK(dbuf(c, "cv_pp", N, &pp));
+  HYD_CHECK(dbuf(c, "cv_tmp", N, &tmp));
+  HYD_CHECK(h2d(c, c.xqi, xq, nptq));
+  HYD_CHECK(h2d(c, c.yqi, yq, nptq));
+  HYD_HIP(hipMemcpyAsync(c.nxt_tmp, nextq, sizeof(int32_t) * nptq, hipMemcpyHostToDevice, c.s));
+  const double saved = c.hsc->qjump;
+  c.hsc->qjump = qjump;
+  int rc = con2grid(c, c.xqi, c.yqi, c.nxt_tmp, nptq, q, nullptr);
+  c.hsc->qjump = saved;
+  HYD_CHECK(rc);
+  HYD_CHECK(fft_ptospc(c, q, qs));
+  HYD_CHECK(main_invert_dev(c, qs, u, v, pp, tmp));
+  if (m > 0) {
+    HYD_CHECK(dbuf(c, "cv_xs", m, &dxs));
+    HYD_CHECK(dbuf(c, "cv_ys", m, &dys));
+    HYD_CHECK(dbuf(c, "cv_us", m, &dus));
+    HYD_CHECK(dbuf(c, "cv_vs", m, &dvs));
+    HYD_CHECK(h2d(c, dxs, xs, m));
+    HYD_CHECK(h2d(c, dys, ys, m));
+    HYD_CHECK(velint(c, u, v, dxs, dys, m, dus, dvs));
+    HYD_CHECK(d2h(c, us, dus, m));
+    HYD_CHECK(d2h(c, vs, dvs, m));
+  }
+  if (fields) {
+    std::vector<double> hu(N), hv(N), hq(N);
+    HYD_CHECK(d2h(c, hu.data(), u, N));
+    HYD_CHECK(d2h(c, hv.data(), v, N));
+    HYD_CHECK(d2h(c, hq.data(), q, N));
+    for (int64_t k = 0; k < N; k++) {
+      fields[3 * k] = (float)hu[k];
+      fields[3 * k + 1] = (float)hv[k];
+      fields[3 * k + 2] = (float)hq[k];
+    }
+  }
   return c2g_status(c);
 }
 

@@ -4,12 +4,19 @@
 // contour nodes.  No reference counterpart (the reference has no learned
 // part); plain bilinear velint stays the fallback.
 //
-// Every convolution is an implicit GEMM on the matrix cores
-// (v_mfma_f32_16x16x16_bf16: BF16 operands, FP32 accumulation), one
-// wavefront per 16x16 output block:
+// Every convolution is an implicit GEMM on the CDNA4 matrix cores
+// (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation, K = 32 per
+// instruction, one 3x3 tap of 32 channels):
 //   forward      M = pixels, N = output channels, K = 9 taps x input channels
 //   input grads  M = pixels, N = input channels,  K = 9 taps x output channels
-//   weight grads M = output channels, N = 9 taps x input channels, K = pixels
+//   weight grads M = output channels, N = 9 taps x input channels (+ a
+//                column of ones for the bias), K = pixels
+// A workgroup of four wavefronts owns a tile of 4 columns x 64 rows of
+// pixels; the tile with its one-pixel halo (6 x 66 pixels x channels) is
+// staged in LDS once with 16-byte loads, and every operand of the tile's
+// matrix products is read from there.  The weights of a layer sit in
+// registers.  Weight gradients are reduced per workgroup in registers and
+// summed over the workgroups in a fixed order (deterministic).
 // The grid is doubly periodic (pixel p = ix * n + iy, neighbours wrap).
 // Activations and gradients between layers are BF16; pre-activations, the
 // network output, master weights and the Adam moments are FP32.
@@ -19,7 +26,7 @@
 
 namespace hyd {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float x) {  // round to nearest even
@@ -30,8 +37,19 @@ __device__ __forceinline__ uint16_t f2bf(float x) {  // round to nearest even
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+// D += A B for one 16x16 block with K = 32.  Lane l: A row l % 16 and B
+// column l % 16, k = 8 (l / 16) .. + 7; D rows 4 (l / 16) + i, column l % 16.
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) {
+  union {
+    uint4 u;
+    bf16x8 b;
+  } x;
+  x.u = v;
+  return x.b;
 }
 
 __device__ __forceinline__ float gelu(float z) {
@@ -43,162 +61,235 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return cdf + z * pdf;
 }
 
-// pixel p shifted by tap (0..8: dx = tap / 3 - 1 along ix, dy = tap % 3 - 1
-// along iy), times sgn (+1 forward, -1 for the transposed convolution)
-__device__ __forceinline__ int tap_pixel(int p, int tap, int sgn, int n) {
-  const int ix = p / n, iy = p - ix * n;
-  int jx = ix + sgn * (tap / 3 - 1), jy = iy + sgn * (tap % 3 - 1);
-  jx = jx < 0 ? jx + n : (jx >= n ? jx - n : jx);
-  jy = jy < 0 ? jy + n : (jy >= n ? jy - n : jy);
-  return jx * n + jy;
+// tiles: TW columns (ix) x TH rows (iy) of pixels per workgroup, one
+// wavefront per column
+constexpr int kTW = 4, kTH = 64, kHW = kTW + 2, kHH = kTH + 2;
+
+// Stage the tile (ix0.., iy0..) of a [pixels][C] BF16 tensor with its halo
+// into LDS as [hx][hy][C] (hx, hy = 0 .. kHW-1, kHH-1; wrapping).
+template <int C>
+__device__ __forceinline__ void stage_halo(const uint16_t *__restrict__ in, int n, int ix0,
+                                           int iy0, uint16_t *tile) {
+  constexpr int UPP = C * 2 / 8;  // 8-byte units per pixel (C = 4: 1, C = 32: 8)
+  constexpr int NU = kHW * kHH * UPP;
+  for (int u = threadIdx.x; u < NU; u += blockDim.x) {
+    const int q = u / UPP, part = u - q * UPP;
+    const int hx = q / kHH, hy = q - hx * kHH;
+    int gx = ix0 - 1 + hx, gy = iy0 - 1 + hy;
+    gx = gx < 0 ? gx + n : (gx >= n ? gx - n : gx);
+    gy %= n;
+    gy = gy < 0 ? gy + n : gy;
+    const uint2 v = *(const uint2 *)(in + ((int64_t)gx * n + gy) * C + part * 4);
+    *(uint2 *)(tile + (hx * kHH + hy) * C + part * 4) = v;
+  }
 }
 
-// One 16x16 block of C = A B with A (M x K) gathered by a functor and B
-// (K x N) stored N-major (b[n * ldb + k]).  Lane l: A row l % 16, B column
-// l % 16, k = k0 + 4 (l / 16) .. + 3; D rows 4 (l / 16) + i, column l % 16.
+// A operand of 16 pixels (rows r0 + l % 16 of column cx of the tile) for
+// K block kb of a tensor with C channels, shifted by sgn * off(tap).
+template <int C>
+__device__ __forceinline__ bf16x8 a_operand(const uint16_t *tile, int cx, int r, int kb, int g,
+                                            int sgn) {
+  if (C == 32) {
+    const int tap = kb, dx = sgn * (tap / 3 - 1), dy = sgn * (tap % 3 - 1);
+    const uint16_t *src = tile + ((cx + 1 + dx) * kHH + (r + 1 + dy)) * 32 + 8 * g;
+    return as_bf16x8(*(const uint4 *)src);
+  } else {  // C == 4: the lane's eight k are two taps of four channels
+    uint4 v = make_uint4(0, 0, 0, 0);
+    const int ta = (kb * 32 + 8 * g) / 4;
+    if (ta < 9) {
+      const int dx = sgn * (ta / 3 - 1), dy = sgn * (ta % 3 - 1);
+      const uint2 w = *(const uint2 *)(tile + ((cx + 1 + dx) * kHH + (r + 1 + dy)) * 4);
+      v.x = w.x;
+      v.y = w.y;
+    }
+    if (ta + 1 < 9) {
+      const int dx = sgn * ((ta + 1) / 3 - 1), dy = sgn * ((ta + 1) % 3 - 1);
+      const uint2 w = *(const uint2 *)(tile + ((cx + 1 + dx) * kHH + (r + 1 + dy)) * 4);
+      v.z = w.x;
+      v.w = w.y;
+    }
+    return as_bf16x8(v);
+  }
+}
 
-// Forward convolution: in [pixels][CI] (BF16), wf [co][9 CI] (BF16, co
-// padded to a multiple of 16), out z [pixels][CO] (FP32) and y = GELU(z)
-// (or z when last) as BF16 [pixels][CO].
-template <int CI, int CO, bool LAST>
-__global__ void __launch_bounds__(256) cnn_conv_fwd(const uint16_t *__restrict__ in,
-                                                    const uint16_t *__restrict__ wf,
-                                                    const float *__restrict__ bias, int n,
-                                                    float *__restrict__ z,
-                                                    uint16_t *__restrict__ y) {
-  constexpr int K = 9 * CI, NB = (CO + 15) / 16;
-  const int l = threadIdx.x & 63;
-  const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);  // 16-pixel block
-  const int P = n * n;
-  if (blk * 16 >= P) return;
-  const int p_a = blk * 16 + (l & 15);  // A row of this lane
-  const int g = l >> 4;
+// 3x3 convolution of a [pixels][CIN] BF16 tensor with packed weights w
+// [NB*16][KP] (KP = 9 CIN padded to 32; row = output column of the GEMM):
+//   MODE 0: forward, z = conv + bias (FP32 [pixels][cout]) and y = GELU(z)
+//   MODE 1: forward of the last layer, y = z
+//   MODE 2: input gradient (transposed convolution: taps shifted by -off),
+//           out = conv * GELU'(zprev) as BF16 [pixels][cout]
+template <int CIN, int NB, int MODE>
+__global__ void __launch_bounds__(256) cnn_conv(const uint16_t *__restrict__ in,
+                                                const uint16_t *__restrict__ w,
+                                                const float *__restrict__ bias, int n, int cout,
+                                                float *__restrict__ z, uint16_t *__restrict__ y,
+                                                const float *__restrict__ zprev) {
+  constexpr int KP = (9 * CIN + 31) / 32 * 32, KB = KP / 32;
+  __shared__ __align__(16) uint16_t tile[kHW * kHH * CIN];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 4;
+  const int iy0 = blockIdx.x * kTH, ix0 = blockIdx.y * kTW;
+  stage_halo<CIN>(in, n, ix0, iy0, tile);
+  bf16x8 wr[KB][NB];
+#pragma unroll
+  for (int kb = 0; kb < KB; kb++)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+      wr[kb][nb] = as_bf16x8(
+          *(const uint4 *)(w + (int64_t)(nb * 16 + (l & 15)) * KP + kb * 32 + 8 * g));
+  __syncthreads();
+  const int sgn = MODE == 2 ? -1 : 1;
+  f32x4 acc[4][NB];
+#pragma unroll
+  for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[mb][nb] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < KB; kb++)
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) {
+      const bf16x8 a = a_operand<CIN>(tile, wv, mb * 16 + (l & 15), kb, g, sgn);
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) acc[mb][nb] = mfma32(a, wr[kb][nb], acc[mb][nb]);
+    }
+  const int ix = ix0 + wv;
 #pragma unroll
   for (int nb = 0; nb < NB; nb++) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int co_b = nb * 16 + (l & 15);  // B column of this lane
-    for (int k0 = 0; k0 < K; k0 += 16) {
-      const int k = k0 + 4 * g;  // 4 consecutive k of one tap (CI % 4 == 0)
-      bf16x4 a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
-      if (k < K) {
-        const int tap = k / CI, ci = k - tap * CI;
-        const uint16_t *src = in + (int64_t)tap_pixel(p_a, tap, 1, n) * CI + ci;
-        const uint16_t *wsrc = wf + (int64_t)co_b * K + k;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          a[j] = (short)src[j];
-          b[j] = (short)wsrc[j];
-        }
-      }
-      acc = mfma16(a, b, acc);
-    }
     const int co = nb * 16 + (l & 15);
-    if (co < CO) {
+    if (co >= cout) continue;
+    const float b0 = MODE == 2 ? 0.f : bias[co];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int p = blk * 16 + 4 * g + i;
-        const float v = acc[i] + bias[co];
-        z[(int64_t)p * CO + co] = v;
-        y[(int64_t)p * CO + co] = f2bf(LAST ? v : gelu(v));
-      }
-    }
-  }
-}
-
-// Input gradients: dx[p][ci] = sum_{tap,co} W[co][ci][tap] dz[p - off(tap)][co]
-// with wb [ci][9 CO] (BF16); dz [pixels][CO] (BF16).  Writes dy_prev =
-// dx * GELU'(z_prev) as BF16 (the previous layer's pre-activation grads).
-template <int CI, int CO>
-__global__ void __launch_bounds__(256) cnn_conv_bwd_data(const uint16_t *__restrict__ dz,
-                                                         const uint16_t *__restrict__ wb, int n,
-                                                         const float *__restrict__ z_prev,
-                                                         uint16_t *__restrict__ dz_prev) {
-  constexpr int K = 9 * CO, NB = (CI + 15) / 16;
-  const int l = threadIdx.x & 63;
-  const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int P = n * n;
-  if (blk * 16 >= P) return;
-  const int p_a = blk * 16 + (l & 15);
-  const int g = l >> 4;
-#pragma unroll
-  for (int nb = 0; nb < NB; nb++) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int ci_b = nb * 16 + (l & 15);
-    for (int k0 = 0; k0 < K; k0 += 16) {
-      const int k = k0 + 4 * g;
-      bf16x4 a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
-      if (k < K) {
-        const int tap = k / CO, co = k - tap * CO;
-        const uint16_t *src = dz + (int64_t)tap_pixel(p_a, tap, -1, n) * CO + co;
-        const uint16_t *wsrc = wb + (int64_t)ci_b * K + k;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          a[j] = (short)src[j];
-          b[j] = (short)wsrc[j];
+        const int iy = iy0 + mb * 16 + 4 * g + i;
+        if (iy >= n) continue;
+        const int64_t o = ((int64_t)ix * n + iy) * cout + co;
+        const float v = acc[mb][nb][i];
+        if (MODE == 2) {
+          y[o] = f2bf(v * gelu_grad(zprev[o]));
+        } else {
+          const float zz = v + b0;
+          if (z) z[o] = zz;  // pre-activations are kept for training only
+          if (y) y[o] = f2bf(MODE == 1 ? zz : gelu(zz));
         }
       }
-      acc = mfma16(a, b, acc);
-    }
-    const int ci = nb * 16 + (l & 15);
-    if (ci < CI) {
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int p = blk * 16 + 4 * g + i;
-        const float zz = z_prev[(int64_t)p * CI + ci];
-        dz_prev[(int64_t)p * CI + ci] = f2bf(acc[i] * gelu_grad(zz));
-      }
-    }
   }
 }
 
-// Weight gradients: dW[co][k] += sum_p dz[p][co] x[p + off(tap(k))][ci(k)],
-// one wavefront per (16 co) x (16 k) block and pixel slice; FP32 atomics
-// into dw [co][9 CI]; block (0, *) also accumulates the bias gradient.  dz
-// rows hold CO channels of which the first co_n are real.
-template <int CI, int CO>
-__global__ void __launch_bounds__(64) cnn_conv_bwd_weight(const uint16_t *__restrict__ dz,
-                                                          const uint16_t *__restrict__ x, int n,
-                                                          int pix_per_wave, int co_n,
-                                                          float *__restrict__ dw,
-                                                          float *__restrict__ db) {
-  constexpr int K = 9 * CI;
-  const int l = threadIdx.x & 63, g = l >> 4;
-  const int kb = blockIdx.x, cb = blockIdx.y;
-  const int P = n * n;
-  const int p_begin = blockIdx.z * pix_per_wave, p_end = min(P, p_begin + pix_per_wave);
-  const int co_a = cb * 16 + (l & 15);  // A row (output channel)
-  const int k_b = kb * 16 + (l & 15);   // B column (tap, input channel)
-  const int tap = k_b / CI, ci = k_b - (k_b / CI) * CI;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  float bsum = 0.f;
-  for (int p0 = p_begin; p0 < p_end; p0 += 16) {
-    bf16x4 a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
+// Weight gradients: dW[co][k] = sum_p dz[p][co] x[p + off(tap(k))][ci(k)]
+// for k = tap CIN + ci < 9 CIN, and the bias gradient sum_p dz[p][co] in
+// column 9 CIN (a column of ones).  Each workgroup sums over its tiles
+// (grid-stride) with the wavefronts sharing the N blocks, and writes its
+// partial [MB*16][NK*16] block; cnn_wgrad_sum adds the partials in order.
+template <int CIN, int COT, int MB, int NK>
+__global__ void __launch_bounds__(256) cnn_wgrad(const uint16_t *__restrict__ dz,
+                                                 const uint16_t *__restrict__ x, int n,
+                                                 int ntiles, float *__restrict__ part) {
+  __shared__ __align__(16) uint16_t xt[kHW * kHH * CIN];
+  __shared__ __align__(16) uint16_t dzt[kTW * kTH * COT];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 4;
+  constexpr int NPW = (NK + 3) / 4;  // N blocks per wavefront
+  f32x4 acc[MB][NPW];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int p = p0 + 4 * g + j;
-      if (p < p_end) {
-        if (co_a < CO) a[j] = (short)dz[(int64_t)p * CO + co_a];
-        if (k_b < K) b[j] = (short)x[(int64_t)tap_pixel(p, tap, 1, n) * CI + ci];
+  for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+    for (int j = 0; j < NPW; j++) acc[mb][j] = {0.f, 0.f, 0.f, 0.f};
+  const int tiles_y = (n + kTH - 1) / kTH;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int iy0 = (tile % tiles_y) * kTH, ix0 = (tile / tiles_y) * kTW;
+    __syncthreads();  // the previous tile's operands are no longer read
+    stage_halo<CIN>(x, n, ix0, iy0, xt);
+    // dz of the tile itself (no halo): kTW columns x kTH rows
+    constexpr int UPP = COT * 2 / 8;
+    for (int u = threadIdx.x; u < kTW * kTH * UPP; u += blockDim.x) {
+      const int q = u / UPP, part8 = u - q * UPP;
+      const int cx = q / kTH, r = q - cx * kTH;
+      const int gy = (iy0 + r) % n;
+      const uint2 v = (iy0 + r < n)
+                          ? *(const uint2 *)(dz + ((int64_t)(ix0 + cx) * n + gy) * COT + part8 * 4)
+                          : make_uint2(0, 0);  // rows past n contribute nothing
+      *(uint2 *)(dzt + q * COT + part8 * 4) = v;
+    }
+    __syncthreads();
+    // K = the tile's 256 pixels, 32 at a time: pixel index q = cx kTH + r
+#pragma unroll 1
+    for (int q0 = 0; q0 < kTW * kTH; q0 += 32) {
+      bf16x8 a[MB];
+#pragma unroll
+      for (int mb = 0; mb < MB; mb++) {
+        const int co = mb * 16 + (l & 15);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const uint16_t v = co < COT ? dzt[(q0 + 8 * g + j) * COT + co] : (uint16_t)0;
+          a[mb][j] = __builtin_bit_cast(__bf16, v);
+        }
+      }
+#pragma unroll
+      for (int jn = 0; jn < NPW; jn++) {
+        const int nb = wv + 4 * jn;
+        if (nb >= NK) continue;
+        const int k = nb * 16 + (l & 15);
+        const int tap = k / CIN, ci = k - tap * CIN;
+        bf16x8 b;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const int q = q0 + 8 * g + j, cx = q / kTH, r = q - cx * kTH;
+          uint16_t v;
+          if (tap < 9) {
+            const int dx = tap / 3 - 1, dy = tap % 3 - 1;
+            v = xt[((cx + 1 + dx) * kHH + (r + 1 + dy)) * CIN + ci];
+          } else {
+            v = (k == 9 * CIN) ? (uint16_t)0x3f80 : (uint16_t)0;  // ones: the bias column
+          }
+          b[j] = __builtin_bit_cast(__bf16, v);
+        }
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) acc[mb][jn] = mfma32(a[mb], b, acc[mb][jn]);
       }
     }
-    if (kb == 0 && co_a < CO)
-#pragma unroll
-      for (int j = 0; j < 4; j++) bsum += bf2f((uint16_t)a[j]);
-    acc = mfma16(a, b, acc);
   }
-  const int k_out = kb * 16 + (l & 15);
-  if (k_out < K)
+  float *pw = part + (int64_t)blockIdx.x * (MB * 16) * (NK * 16);
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int co = cb * 16 + 4 * g + i;
-      if (co < co_n) atomicAdd(&dw[(int64_t)co * K + k_out], acc[i]);
-    }
-  if (kb == 0) {
-    // lanes l, l+16, l+32, l+48 hold the same channel (every lane takes
-    // part in the shuffles; padded channels hold zeros)
-    bsum += __shfl_down(bsum, 32);
-    bsum += __shfl_down(bsum, 16);
-    if (g == 0 && co_a < co_n) atomicAdd(&db[co_a], bsum);
+  for (int jn = 0; jn < NPW; jn++) {
+    const int nb = wv + 4 * jn;
+    if (nb >= NK) continue;
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        pw[(int64_t)(mb * 16 + 4 * g + i) * (NK * 16) + nb * 16 + (l & 15)] = acc[mb][jn][i];
+  }
+}
+
+// Sum the workgroups' partials in a fixed order (8 strided groups of
+// workgroups, then the 8 group sums in order); unpack into the master
+// layout W[co][ci][tap] and the biases of one layer.  32 weights per
+// workgroup of 256 threads: lane e % 32, group threadIdx.x / 32.
+constexpr int kSumLanes = 32, kSumGroups = 8;
+__global__ void __launch_bounds__(256) cnn_wgrad_sum(const float *__restrict__ part, int nparts,
+                                                     int rows, int cols, int ci_n, int co_n,
+                                                     float *__restrict__ gW,
+                                                     float *__restrict__ gb) {
+  __shared__ float red[kSumGroups][kSumLanes];
+  const int lane = threadIdx.x % kSumLanes, grp = threadIdx.x / kSumLanes;
+  const int e = blockIdx.x * kSumLanes + lane;
+  const int cnt = co_n * (9 * ci_n + 1);
+  const int co = e / (9 * ci_n + 1), k = e - co * (9 * ci_n + 1);
+  float s = 0.f;
+  if (e < cnt)
+    for (int pidx = grp; pidx < nparts; pidx += kSumGroups)
+      s += part[((int64_t)pidx * rows + co) * cols + k];
+  red[grp][lane] = s;
+  __syncthreads();
+  if (grp != 0 || e >= cnt) return;
+  s = red[0][lane];
+#pragma unroll
+  for (int j = 1; j < kSumGroups; j++) s += red[j][lane];
+  if (k == 9 * ci_n) {
+    gb[co] = s;
+  } else {
+    const int tap = k / ci_n, ci = k - tap * ci_n;
+    gW[((int64_t)co * ci_n + ci) * 9 + tap] = s;
   }
 }
 
@@ -306,36 +397,29 @@ __global__ void cnn_adam(float *__restrict__ w, float *__restrict__ m1, float *_
   w[k] = w[k] - lr * (a / c1) / (sqrtf(v / c2) + eps);
 }
 
-// master layout per layer: W[co][ci][tap] then bias[co]
+// master layout per layer: W[co][ci][tap] then bias[co].  Matrix-core
+// operand layouts (rows padded to 16, K padded to 32 with zeros):
+//   wf[co][k], k = tap CI + ci            (forward, B = W^T)
+//   wb[ci][k], k = tap COT + co, COT = co padded to 4   (input gradients)
 __global__ void cnn_pack_weights(const float *__restrict__ w, CnnLayers d, uint16_t *wf,
                                  uint16_t *wb) {
   const int layer = blockIdx.y;
   const LayerDims D = d.L[layer];
-  const int K = 9 * D.ci, total = D.cop * K;
   const float *W = w + D.off;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
-    const int co = e / K, k = e - co * K, tap = k / D.ci, ci = k - tap * D.ci;
-    const float v = co < D.co ? W[((int64_t)co * D.ci + ci) * 9 + tap] : 0.f;
+  const int Kf = 9 * D.ci, KPf = (Kf + 31) / 32 * 32, totf = D.cop * KPf;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < totf; e += gridDim.x * blockDim.x) {
+    const int co = e / KPf, k = e - co * KPf, tap = k / D.ci, ci = k - tap * D.ci;
+    const float v = (co < D.co && k < Kf) ? W[((int64_t)co * D.ci + ci) * 9 + tap] : 0.f;
     wf[d.wf_off[layer] + e] = f2bf(v);
   }
-  // wb[ci][tap * COP4 + co], COP4 = co padded to 4 (k chunks of 4 in one tap)
-  const int cop4 = (D.co + 3) / 4 * 4, Kb = 9 * cop4, totb = ((D.ci + 15) / 16 * 16) * Kb;
+  const int cot = (D.co + 3) / 4 * 4, Kb = 9 * cot, KPb = (Kb + 31) / 32 * 32;
+  const int totb = ((D.ci + 15) / 16 * 16) * KPb;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < totb; e += gridDim.x * blockDim.x) {
-    const int ci = e / Kb, k = e - ci * Kb, tap = k / cop4, co = k - tap * cop4;
-    const float v = (ci < D.ci && co < D.co) ? W[((int64_t)co * D.ci + ci) * 9 + tap] : 0.f;
+    const int ci = e / KPb, k = e - ci * KPb, tap = k / cot, co = k - tap * cot;
+    const float v =
+        (ci < D.ci && k < Kb && co < D.co) ? W[((int64_t)co * D.ci + ci) * 9 + tap] : 0.f;
     wb[d.wb_off[layer] + e] = f2bf(v);
   }
-}
-
-// Repack the weight gradient of one layer from the GEMM layout
-// dw[co][tap CI + ci] into the master layout W[co][ci][tap].
-__global__ void cnn_unpack_grad(const float *__restrict__ dwk, int ci_n, int co_n,
-                                float *__restrict__ g) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int K = 9 * ci_n;
-  if (e >= co_n * K) return;
-  const int co = e / K, k = e - co * K, tap = k / ci_n, ci = k - tap * ci_n;
-  g[((int64_t)co * ci_n + ci) * 9 + tap] = dwk[e];
 }
 
 // Widen a [pixels][2] gradient to [pixels][4] BF16 (the last layer's output
@@ -350,17 +434,17 @@ __global__ void cnn_pack_dz_last(const float *__restrict__ g2, int P, uint16_t *
 }
 
 // Probe of the matrix-core operand layout (used by the tests): D = A B for
-// one 16x16x16 block, A row-major [16][16], B N-major [16][16] (b[n][k]),
-// D row-major, all read and written as the kernels above read them.
+// one 16x16x32 block, A row-major [16][32], B N-major [16][32] (b[n][k]),
+// D row-major [16][16], read and written as the kernels above do.
 __global__ void cnn_mfma_probe(const uint16_t *a, const uint16_t *b, float *d) {
   const int l = threadIdx.x & 63, g = l >> 4;
-  bf16x4 av, bv;
-  for (int j = 0; j < 4; j++) {
-    av[j] = (short)a[(l & 15) * 16 + 4 * g + j];
-    bv[j] = (short)b[(l & 15) * 16 + 4 * g + j];
+  bf16x8 av, bv;
+  for (int j = 0; j < 8; j++) {
+    av[j] = __builtin_bit_cast(__bf16, a[(l & 15) * 32 + 8 * g + j]);
+    bv[j] = __builtin_bit_cast(__bf16, b[(l & 15) * 32 + 8 * g + j]);
   }
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  acc = mfma16(av, bv, acc);
+  acc = mfma32(av, bv, acc);
   for (int i = 0; i < 4; i++) d[(4 * g + i) * 16 + (l & 15)] = acc[i];
 }
 
@@ -372,7 +456,8 @@ struct Cnn {
   int64_t nw = 0;                     // master weights incl. biases
   float *w = nullptr, *m1 = nullptr, *m2 = nullptr, *gw = nullptr;
   uint16_t *wf = nullptr, *wb = nullptr;
-  float *dwk = nullptr;               // GEMM-layout weight gradient (largest layer)
+  float *part = nullptr;              // per-workgroup weight-gradient partials
+  int nparts = 0;                     // workgroups of the weight-gradient kernels
   uint16_t *x0 = nullptr;             // packed input [P][4]
   float *z[4] = {};                   // pre-activations
   uint16_t *y[4] = {};                // activations (BF16)
@@ -397,7 +482,7 @@ using namespace hyd;
 static const int kCi[4] = {4, 32, 32, 32}, kCo[4] = {32, 32, 32, 2};
 
 static void cnn_free(Cnn &c) {
-  void *ptrs[] = {c.w, c.m1, c.m2, c.gw, c.wf, c.wb, c.dwk, c.x0, c.gout, c.in3, c.xn, c.yn,
+  void *ptrs[] = {c.w, c.m1, c.m2, c.gw, c.wf, c.wb, c.part, c.x0, c.gout, c.in3, c.xn, c.yn,
                   c.tu, c.tv, c.pu, c.pv, c.loss};
   for (void *p : ptrs)
     if (p) hipFree(p);
@@ -453,7 +538,7 @@ extern "C" int hyd_cnn_create(int32_t n, double ell, uint64_t seed, int32_t devi
   c.geom.xmin = -0.5 * ell;
   c.geom.ymin = -0.5 * ell;
   c.geom.gli = n / ell;
-  int64_t off = 0, wfo = 0, wbo = 0, maxk = 0;
+  int64_t off = 0, wfo = 0, wbo = 0;
   for (int i = 0; i < 4; i++) {
     LayerDims &D = c.dims.L[i];
     D.ci = kCi[i];
@@ -462,12 +547,16 @@ extern "C" int hyd_cnn_create(int32_t n, double ell, uint64_t seed, int32_t devi
     D.off = (int)off;
     off += (int64_t)D.co * D.ci * 9 + D.co;  // weights, then biases
     c.dims.wf_off[i] = (int)wfo;
-    wfo += (int64_t)D.cop * 9 * D.ci;
+    wfo += (int64_t)D.cop * ((9 * D.ci + 31) / 32 * 32);
     c.dims.wb_off[i] = (int)wbo;
-    wbo += (int64_t)((D.ci + 15) / 16 * 16) * 9 * ((D.co + 3) / 4 * 4);
-    maxk = std::max<int64_t>(maxk, (int64_t)D.co * 9 * D.ci);
+    wbo += (int64_t)((D.ci + 15) / 16 * 16) * ((9 * ((D.co + 3) / 4 * 4) + 31) / 32 * 32);
   }
   c.nw = off;
+  // weight gradients: at most 256 workgroups, each a partial of at most
+  // 32 x 304 values (layers 2-3: 32 output channels, 9 x 32 + 1 columns
+  // padded to 16)
+  const int ntiles = ((n + kTH - 1) / kTH) * (n / kTW);
+  c.nparts = std::min(ntiles, 256);
   int rc = HYD_OK;
   auto T = [&](int r) {
     if (rc == HYD_OK) rc = r;
@@ -479,7 +568,7 @@ extern "C" int hyd_cnn_create(int32_t n, double ell, uint64_t seed, int32_t devi
   T(cmalloc(&c.gw, c.nw));
   T(cmalloc(&c.wf, wfo));
   T(cmalloc(&c.wb, wbo));
-  T(cmalloc(&c.dwk, maxk));
+  T(cmalloc(&c.part, (int64_t)c.nparts * 32 * 304));
   T(cmalloc(&c.x0, (int64_t)c.P * 4));
   T(cmalloc(&c.gout, (int64_t)c.P * 2));
   T(cmalloc(&c.in3, (int64_t)c.P * 3));
@@ -550,19 +639,20 @@ extern "C" int hyd_cnn_destroy(hyd_cnn *h) {
 }
 
 static int cnn_forward(Cnn &c) {
-  const int P = c.P, nblk = (P / 16 + 3) / 4;
+  const int P = c.P;
   hipLaunchKernelGGL(cnn_pack_input, dim3((P + 255) / 256), dim3(256), 0, c.s, c.in3, P, c.x0);
   const uint16_t *wf = c.wf;
   const float *w = c.w;
   auto bias = [&](int i) { return w + c.dims.L[i].off + (int64_t)c.dims.L[i].co * c.dims.L[i].ci * 9; };
-  hipLaunchKernelGGL((cnn_conv_fwd<4, 32, false>), dim3(nblk), dim3(256), 0, c.s, c.x0,
-                     wf + c.dims.wf_off[0], bias(0), c.n, c.z[0], c.y[0]);
-  hipLaunchKernelGGL((cnn_conv_fwd<32, 32, false>), dim3(nblk), dim3(256), 0, c.s, c.y[0],
-                     wf + c.dims.wf_off[1], bias(1), c.n, c.z[1], c.y[1]);
-  hipLaunchKernelGGL((cnn_conv_fwd<32, 32, false>), dim3(nblk), dim3(256), 0, c.s, c.y[1],
-                     wf + c.dims.wf_off[2], bias(2), c.n, c.z[2], c.y[2]);
-  hipLaunchKernelGGL((cnn_conv_fwd<32, 2, true>), dim3(nblk), dim3(256), 0, c.s, c.y[2],
-                     wf + c.dims.wf_off[3], bias(3), c.n, c.z[3], c.y[3]);
+  const dim3 grid((c.n + kTH - 1) / kTH, c.n / kTW);
+  hipLaunchKernelGGL((cnn_conv<4, 2, 0>), grid, dim3(256), 0, c.s, c.x0, wf + c.dims.wf_off[0],
+                     bias(0), c.n, 32, c.z[0], c.y[0], nullptr);
+  hipLaunchKernelGGL((cnn_conv<32, 2, 0>), grid, dim3(256), 0, c.s, c.y[0],
+                     wf + c.dims.wf_off[1], bias(1), c.n, 32, c.z[1], c.y[1], nullptr);
+  hipLaunchKernelGGL((cnn_conv<32, 2, 0>), grid, dim3(256), 0, c.s, c.y[1],
+                     wf + c.dims.wf_off[2], bias(2), c.n, 32, c.z[2], c.y[2], nullptr);
+  hipLaunchKernelGGL((cnn_conv<32, 1, 1>), grid, dim3(256), 0, c.s, c.y[2],
+                     wf + c.dims.wf_off[3], bias(3), c.n, 2, c.z[3], c.y[3], nullptr);
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }
@@ -596,35 +686,38 @@ static int cnn_backward(Cnn &c, int m) {
   hipLaunchKernelGGL(cnn_pack_dz_last, dim3((P + 255) / 256), dim3(256), 0, c.s, c.gout, P,
                      c.dz[3]);
   const uint16_t *wb = c.wb;
+  const dim3 grid((c.n + kTH - 1) / kTH, c.n / kTW);
   // input gradients back through layers 4..2 (layer 1's input needs none)
-  hipLaunchKernelGGL((cnn_conv_bwd_data<32, 4>), dim3(nblk), dim3(256), 0, c.s, c.dz[3],
-                     wb + c.dims.wb_off[3], c.n, c.z[2], c.dz[2]);
-  hipLaunchKernelGGL((cnn_conv_bwd_data<32, 32>), dim3(nblk), dim3(256), 0, c.s, c.dz[2],
-                     wb + c.dims.wb_off[2], c.n, c.z[1], c.dz[1]);
-  hipLaunchKernelGGL((cnn_conv_bwd_data<32, 32>), dim3(nblk), dim3(256), 0, c.s, c.dz[1],
-                     wb + c.dims.wb_off[1], c.n, c.z[0], c.dz[0]);
-  // weight gradients
-  const int slices = std::max(1, std::min(256, P / 256));
-  const int ppw = ((P + slices - 1) / slices + 15) / 16 * 16;
+  hipLaunchKernelGGL((cnn_conv<4, 2, 2>), grid, dim3(256), 0, c.s, c.dz[3],
+                     wb + c.dims.wb_off[3], nullptr, c.n, 32, nullptr, c.dz[2], c.z[2]);
+  hipLaunchKernelGGL((cnn_conv<32, 2, 2>), grid, dim3(256), 0, c.s, c.dz[2],
+                     wb + c.dims.wb_off[2], nullptr, c.n, 32, nullptr, c.dz[1], c.z[1]);
+  hipLaunchKernelGGL((cnn_conv<32, 2, 2>), grid, dim3(256), 0, c.s, c.dz[1],
+                     wb + c.dims.wb_off[1], nullptr, c.n, 32, nullptr, c.dz[0], c.z[0]);
+  // weight and bias gradients: partials per workgroup, summed in order
+  const int ntiles = ((c.n + kTH - 1) / kTH) * (c.n / kTW);
   const uint16_t *xin[4] = {c.x0, c.y[0], c.y[1], c.y[2]};
   for (int i = 0; i < 4; i++) {
     const LayerDims &D = c.dims.L[i];
-    const int K = 9 * D.ci;
-    hipLaunchKernelGGL(cnn_zero, dim3((unsigned)((D.co * K + 255) / 256)), dim3(256), 0, c.s,
-                       c.dwk, (int64_t)D.co * K);
-    float *db = c.gw + D.off + (int64_t)D.co * D.ci * 9;
-    const dim3 grid((K + 15) / 16, (D.co + 15) / 16, (P + ppw - 1) / ppw);
-    if (i == 0)
-      hipLaunchKernelGGL((cnn_conv_bwd_weight<4, 32>), grid, dim3(64), 0, c.s, c.dz[0], xin[0],
-                         c.n, ppw, D.co, c.dwk, db);
-    else if (i < 3)
-      hipLaunchKernelGGL((cnn_conv_bwd_weight<32, 32>), grid, dim3(64), 0, c.s, c.dz[i],
-                         xin[i], c.n, ppw, D.co, c.dwk, db);
-    else
-      hipLaunchKernelGGL((cnn_conv_bwd_weight<32, 4>), grid, dim3(64), 0, c.s, c.dz[3], xin[3],
-                         c.n, ppw, D.co, c.dwk, db);
-    hipLaunchKernelGGL(cnn_unpack_grad, dim3((D.co * K + 255) / 256), dim3(256), 0, c.s, c.dwk,
-                       D.ci, D.co, c.gw + D.off);
+    int rows = 0, cols = 0;
+    if (i == 0) {
+      hipLaunchKernelGGL((cnn_wgrad<4, 32, 2, 3>), dim3(c.nparts), dim3(256), 0, c.s, c.dz[0],
+                         xin[0], c.n, ntiles, c.part);
+      rows = 32, cols = 48;
+    } else if (i < 3) {
+      hipLaunchKernelGGL((cnn_wgrad<32, 32, 2, 19>), dim3(c.nparts), dim3(256), 0, c.s, c.dz[i],
+                         xin[i], c.n, ntiles, c.part);
+      rows = 32, cols = 304;
+    } else {
+      hipLaunchKernelGGL((cnn_wgrad<32, 4, 1, 19>), dim3(c.nparts), dim3(256), 0, c.s, c.dz[3],
+                         xin[3], c.n, ntiles, c.part);
+      rows = 16, cols = 304;
+    }
+    const int cnt = D.co * (9 * D.ci + 1);
+    hipLaunchKernelGGL(cnn_wgrad_sum, dim3((cnt + kSumLanes - 1) / kSumLanes),
+                       dim3(kSumLanes * kSumGroups), 0, c.s, c.part,
+                       c.nparts, rows, cols, D.ci, D.co, c.gw + D.off,
+                       c.gw + D.off + (int64_t)D.co * D.ci * 9);
   }
   HYD_HIP(hipGetLastError());
   return HYD_OK;
@@ -732,6 +825,81 @@ extern "C" int hyd_cnn_set_weights(hyd_cnn *h, const float *w) {
   return HYD_OK;
 }
 
+// ---- the correction inside the time step ------------------------------------
+namespace hyd {
+
+template <class T>
+__global__ void cnn_pack_fields(const T *__restrict__ uu, const T *__restrict__ vv,
+                                const double *__restrict__ q, int P, double sui, double sqi,
+                                uint16_t *__restrict__ x) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  x[(int64_t)p * 4] = f2bf((float)((double)uu[p] * sui));
+  x[(int64_t)p * 4 + 1] = f2bf((float)((double)vv[p] * sui));
+  x[(int64_t)p * 4 + 2] = f2bf((float)((double)q[p] * sqi));
+  x[(int64_t)p * 4 + 3] = 0;
+}
+
+__global__ void cnn_sample_scaled(const float *__restrict__ out, const double *__restrict__ xn,
+                                  const double *__restrict__ yn, int m, CnnGeom g, float scale,
+                                  float *__restrict__ du, float *__restrict__ dv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int idx[4];
+  float w[4];
+  bilinear(xn[i], yn[i], g, idx, w);
+  float u = 0.f, v = 0.f;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    u += w[j] * out[(int64_t)idx[j] * 2];
+    v += w[j] * out[(int64_t)idx[j] * 2 + 1];
+  }
+  du[i] = scale * u;
+  dv[i] = scale * v;
+}
+
+}  // namespace hyd
+
+// The correction (du, dv) at m nodes from the gridded u, v and PV q of a
+// time-step stage (physical layout f[ix*n + iy]; u, v FP64 or FP32, q
+// FP64), inputs
+// scaled by 1/su and 1/sq as in training, the output by sd; everything on
+// stream s (the caller's), using the network's own work buffers.
+int hyd::cnn_correction(hyd_cnn *h, hipStream_t s, const void *uu, const void *vv,
+                        const double *q, bool f32, double su, double sq, double sd, const double *xn,
+                        const double *yn, int m, float *du, float *dv) {
+  Cnn &c = h->c;
+  const int P = c.P;
+  const double sui = 1.0 / su, sqi = 1.0 / sq;
+  if (f32)
+    hipLaunchKernelGGL(cnn_pack_fields<float>, dim3((P + 255) / 256), dim3(256), 0, s,
+                       (const float *)uu, (const float *)vv, q, P, sui, sqi, c.x0);
+  else
+    hipLaunchKernelGGL(cnn_pack_fields<double>, dim3((P + 255) / 256), dim3(256), 0, s,
+                       (const double *)uu, (const double *)vv, q, P, sui, sqi, c.x0);
+  const uint16_t *wf = c.wf;
+  const float *w = c.w;
+  auto bias = [&](int i) { return w + c.dims.L[i].off + (int64_t)c.dims.L[i].co * c.dims.L[i].ci * 9; };
+  const dim3 grid((c.n + kTH - 1) / kTH, c.n / kTW);
+  // inference: the hidden pre-activations and the last layer's BF16 copy
+  // are not needed
+  hipLaunchKernelGGL((cnn_conv<4, 2, 0>), grid, dim3(256), 0, s, c.x0, wf + c.dims.wf_off[0],
+                     bias(0), c.n, 32, nullptr, c.y[0], nullptr);
+  hipLaunchKernelGGL((cnn_conv<32, 2, 0>), grid, dim3(256), 0, s, c.y[0], wf + c.dims.wf_off[1],
+                     bias(1), c.n, 32, nullptr, c.y[1], nullptr);
+  hipLaunchKernelGGL((cnn_conv<32, 2, 0>), grid, dim3(256), 0, s, c.y[1], wf + c.dims.wf_off[2],
+                     bias(2), c.n, 32, nullptr, c.y[2], nullptr);
+  hipLaunchKernelGGL((cnn_conv<32, 1, 1>), grid, dim3(256), 0, s, c.y[2], wf + c.dims.wf_off[3],
+                     bias(3), c.n, 2, c.z[3], nullptr, nullptr);
+  if (m > 0)
+    hipLaunchKernelGGL(cnn_sample_scaled, dim3((m + 255) / 256), dim3(256), 0, s, c.z[3], xn, yn,
+                       m, c.geom, (float)sd, du, dv);
+  HYD_HIP(hipGetLastError());
+  return HYD_OK;
+}
+
+int hyd::cnn_grid(hyd_cnn *h) { return h ? h->c.n : 0; }
+
 extern "C" int hyd_cnn_mfma_probe(const uint16_t *a, const uint16_t *b, float *d) {
   if (!a || !b || !d) {
     set_error("cnn_mfma_probe: bad arguments");
@@ -739,12 +907,12 @@ extern "C" int hyd_cnn_mfma_probe(const uint16_t *a, const uint16_t *b, float *d
   }
   uint16_t *da, *db;
   float *dd;
-  HYD_HIP(hipMalloc((void **)&da, 512));
-  HYD_HIP(hipMalloc((void **)&db, 512));
+  HYD_HIP(hipMalloc((void **)&da, 1024));
+  HYD_HIP(hipMalloc((void **)&db, 1024));
   HYD_HIP(hipMalloc((void **)&dd, 1024));
   int rc = HYD_OK;
-  if (hipMemcpy(da, a, 512, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(db, b, 512, hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpy(da, a, 1024, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(db, b, 1024, hipMemcpyHostToDevice) != hipSuccess)
     rc = HYD_EHIP;
   if (rc == HYD_OK) {
     hipLaunchKernelGGL(cnn_mfma_probe, dim3(1), dim3(64), 0, 0, da, db, dd);

@@ -189,6 +189,11 @@ struct Ctx {
   };
   std::vector<GridSave> grid_saves;
   std::vector<ContSave> cont_saves;
+  // learned velocity correction added to velint at the nodes in every
+  // stage (no reference counterpart; off unless a network is attached)
+  hyd_cnn *net = nullptr;
+  double net_su = 1.0, net_sq = 1.0, net_sd = 1.0;  // input and output scales
+  float *cdu = nullptr, *cdv = nullptr;  // named buffers "cnn_du", "cnn_dv"
   // where a step-count run resumes: 0 = contours to rebuild from qr
   // (caps.f90 recont), 1 = contours built, advect's start to run,
   // 2 = inside advect's loop
@@ -267,6 +272,12 @@ int seqsum(Ctx &c, const double *a, int64_t n, double *dout);
 int diag_energy(Ctx &c, double *ekin, double *epot);
 void monitor_record(Ctx &c, const Scalars &s);
 int save_slots(Ctx &c, bool grid, bool cont);
+// the learned correction at the nodes (cnn.hip)
+int cnn_correction(hyd_cnn *h, hipStream_t s, const void *uu, const void *vv, const double *q,
+                   bool f32, double su, double sq, double sd, const double *xn, const double *yn, int m,
+                   float *du, float *dv);
+int cnn_grid(hyd_cnn *h);
+int correction(Ctx &c);
 int diag_norms(Ctx &c, const double *qq, double *ql1, double *ql2);
 int diag_spec1d(Ctx &c, double *ss, double *spec, int iopt);
 void diag_free(Ctx &c);

@@ -67,12 +67,17 @@ __global__ void rk_nodes_kernel(const G *__restrict__ uu, const G *__restrict__ 
                                 double *__restrict__ xq, double *__restrict__ yq,
                                 double *__restrict__ xqi, double *__restrict__ yqi,
                                 double *__restrict__ xqf, double *__restrict__ yqf, int n,
-                                int stage, const Scalars *__restrict__ sc, NodeGeom g) {
+                                int stage, const Scalars *__restrict__ sc, NodeGeom g,
+                                const float *__restrict__ du, const float *__restrict__ dv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double dt = sc->dt, dt2 = sc->dt2, dt3 = sc->dt3, dt6 = sc->dt6;
   double uq, vq;
   interp(uu, vv, xq[i], yq[i], g, uq, vq);
+  if (du) {  // the learned correction, when a network is attached
+    uq = uq + (double)du[i];
+    vq = vq + (double)dv[i];
+  }
   if (stage == 1) {
     const double x0 = xq[i], y0 = yq[i];
     xqi[i] = x0;
@@ -123,17 +128,43 @@ int velint(Ctx &c, const double *uu, const double *vv, const double *xq, const d
   return HYD_OK;
 }
 
+// The learned correction of this stage at the current node positions:
+// physical PV q = spctop(qq) of the stage's inversion, then the network on
+// (u, v, q) sampled at the nodes (cnn.hip), into c.cdu, c.cdv.
+int correction(Ctx &c) {
+  const int64_t N = c.N;
+  double *q, *tmp;
+  HYD_CHECK(dbuf(c, "cnn_q", N, &q));
+  HYD_CHECK(dbuf(c, "cnn_tmp", N, &tmp));
+  // sized by the node capacity: allocated once, outside any captured step
+  HYD_CHECK(dbuf(c, "cnn_du", c.cap, &c.cdu));
+  HYD_CHECK(dbuf(c, "cnn_dv", c.cap, &c.cdv));
+  HYD_CHECK(fft_spctop(c, c.qq, q, PRE_NONE, 1.0, tmp));
+  const bool f32 = c.p.precision == 1;
+  return cnn_correction(c.net, c.s, f32 ? (const void *)c.uu32 : (const void *)c.uu,
+                        f32 ? (const void *)c.vv32 : (const void *)c.vv, q, f32, c.net_su,
+                        c.net_sq, c.net_sd, c.xq, c.yq, c.nptq, c.cdu, c.cdv);
+}
+
 int rk_nodes(Ctx &c, int stage) {
   const int n = c.nptq;
   if (n <= 0 || !c.contex) return HYD_OK;
   static const int64_t words[5] = {0, 8, 10, 10, 6};
+  const float *du = nullptr, *dv = nullptr;
+  if (c.net) {
+    HYD_CHECK(correction(c));
+    du = c.cdu;
+    dv = c.cdv;
+  }
   KTimer t(c, "rk_nodes", (int64_t)n * 8 * words[stage] + 16 * c.N);
   if (c.p.precision == 1)
     hipLaunchKernelGGL(rk_nodes_kernel<float>, dim3((n + 255) / 256), dim3(256), 0, c.s, c.uu32,
-                       c.vv32, c.xq, c.yq, c.xqi, c.yqi, c.xqf, c.yqf, n, stage, c.dsc, geom(c));
+                       c.vv32, c.xq, c.yq, c.xqi, c.yqi, c.xqf, c.yqf, n, stage, c.dsc, geom(c),
+                       du, dv);
   else
     hipLaunchKernelGGL(rk_nodes_kernel<double>, dim3((n + 255) / 256), dim3(256), 0, c.s, c.uu,
-                       c.vv, c.xq, c.yq, c.xqi, c.yqi, c.xqf, c.yqf, n, stage, c.dsc, geom(c));
+                       c.vv, c.xq, c.yq, c.xqi, c.yqi, c.xqf, c.yqf, n, stage, c.dsc, geom(c),
+                       du, dv);
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }

@@ -334,10 +334,27 @@ int hyd_cnn_train_step(hyd_cnn *cnn, const float *in3, const double *xn, const d
 int hyd_cnn_num_weights(hyd_cnn *cnn, int64_t *nw);
 int hyd_cnn_get_weights(hyd_cnn *cnn, float *w);
 int hyd_cnn_set_weights(hyd_cnn *cnn, const float *w);
-/* One 16x16x16 BF16 matrix-core product D = A B (A row-major, B column
- * by column, D row-major; 256 BF16 / 256 FP32 values): the operand layout
- * check of the network's kernels. */
+/* One 16x16x32 BF16 matrix-core product D = A B (A [16][32] row-major, B
+ * column by column [16][32], D [16][16] row-major; 512 BF16 each, 256
+ * FP32): the operand layout check of the network's kernels. */
 int hyd_cnn_mfma_probe(const uint16_t *a, const uint16_t *b, float *d);
+/* Attach a trained network to a flow (NULL detaches): every RK4 stage then
+ * adds the network's (du, dv) at the nodes to velint's velocity.  Inputs
+ * are the stage's u/su, v/su and PV q/sq on the context's grid (the
+ * network's n must be ng), the output is scaled by sd.  The step is then no
+ * longer the reference's; plain velint is the default.  One network per
+ * running flow (its work buffers are used on the flow's stream). */
+int hyd_set_correction(hyd_ctx *ctx, hyd_cnn *net, double su, double sq, double sd);
+/* The velocity that the PV of the contours (xq, yq, nextq, qjump) has on
+ * this context's grid -- con2grid, ptospc, main_invert (contours.f90:
+ * 173-346, spectral.f90:304-342) -- at m points (xs, ys) by velint
+ * (contours.f90:136-169); the context's own state is not touched.  On a
+ * grid 4x finer than a flow's, this is the judge of the flow's velint and
+ * the target of the learned correction.  fields (may be NULL) receives u,
+ * v and the PV grid as FP32 [ng*ng][3]. */
+int hyd_contour_velocity(hyd_ctx *ctx, int32_t nptq, const double *xq, const double *yq,
+                         const int32_t *nextq, double qjump, int32_t m, const double *xs,
+                         const double *ys, double *us, double *vs, float *fields);
 
 #ifdef __cplusplus
 }

@@ -22,8 +22,8 @@ def _bf16_bits(a):
 def test_mfma_operand_layout():
     from hydra_amd import mfma_probe
     rng = np.random.default_rng(0)
-    A = rng.normal(size=(16, 16)).astype(np.float32)
-    B = rng.normal(size=(16, 16)).astype(np.float32)  # B[k][n]
+    A = rng.normal(size=(16, 32)).astype(np.float32)
+    B = rng.normal(size=(32, 16)).astype(np.float32)  # B[k][n]
     Ab, Bb = _bf16_bits(A), _bf16_bits(B)
     D = mfma_probe(Ab, Bb.T.copy())  # the probe takes B column by column
     ref = K.bf16(A) @ K.bf16(B)
