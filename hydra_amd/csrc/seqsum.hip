@@ -441,7 +441,11 @@ __device__ double lane_seq(const double *__restrict__ a, int64_t b, int64_t e, d
   return S;
 }
 
-// The member chain over up to 64 compacted records (one per lane).
+// The member chain over up to 64 compacted records (one per lane).  Runs
+// of at most kWalk records are walked one record at a time (a handful of
+// dependent operations each); longer runs take the wavefront scans.
+static constexpr int kWalk = 4;
+
 __device__ void chain_group(const double *__restrict__ a, const SegRec &R, int nv, double &S) {
   const int l = lane_id();
   const bool valid = l < nv;
@@ -476,6 +480,27 @@ __device__ void chain_group(const double *__restrict__ a, const SegRec &R, int n
     // the run: lanes q .. h-1 without a head, replay or binade change
     const uint64_t stop = __ballot(valid && l > q && (head || rep || key != key_q));
     const int h = stop ? __ffsll((unsigned long long)stop) - 1 : nv;
+    if (h - q <= kWalk) {
+      // a short run (the records between binade crossings): each record is
+      // checked and translated on its own, S being exact at its start
+      for (int r = q; r < h; r++) {
+        const double g_r = readlane_d(R.g, r), d0_r = readlane_d(R.d0, r);
+        const double d1_r = readlane_d(R.d1, r), lo_r = readlane_d(R.lo, r);
+        const double hi_r = readlane_d(R.hi, r), ui_r = readlane_d(ui, r);
+        const int gin_r = __builtin_amdgcn_readlane((int)g_inside, r);
+        if (gin_r && segkey(S) == key_q && run_inside(S, binade(g_r), lo_r, hi_r)) {
+          S = S + (parity_of(S - g_r, ui_r) ? d1_r : d0_r);
+        } else {
+          const int64_t chunk_r = (int64_t)__builtin_amdgcn_readlane(R.chunk, r);
+          const int meta_r = __builtin_amdgcn_readlane(R.meta, r);
+          const int64_t b = chunk_r * kChunk + (meta_r & 511) + ((meta_r >> 18) & 1);
+          const int64_t e = chunk_r * kChunk + ((meta_r >> 9) & 511);
+          S = lane_seq(a, b, e, S);
+        }
+      }
+      q = h;
+      continue;
+    }
     const bool in = l >= q && l < h;
     const double g_q = readlane_d(R.g, q);
     const double ui_q = readlane_d(ui, q);
@@ -537,10 +562,179 @@ __device__ void member_level(const double *__restrict__ a, const SegRec *__restr
   chain_group(a, R, tot, S);
 }
 
+// ---- class tables of the windows that did not fold -------------------------
+// A window whose guessed run crosses binades (heads) is a translation only
+// per residue class of the start: for a start S = G_c + Delta, G_c = g + c u
+// (g the window's first guess, u the grid of g's binade, c < M) and Delta a
+// multiple of M u = 2 u_max (u_max the coarsest grid the run visits), every
+// rounding of the run from S happens on the grid it happens on from G_c,
+// displaced by Delta, as long as every partial sum stays in its binade:
+// the run then ends exactly at E_c + Delta.  seq_classes walks the window's
+// records from the 32 starts G_0..G_31 (one lane each, heads added, rejected
+// records summed in order), intersecting the admissible Delta interval over
+// every step (4-ulp margins, as run_inside).  The chain then resolves such a
+// window with one lookup: c from the low mantissa bits of S - g, Delta =
+// S - G_c, a range check, S = E_c + Delta; a start outside its interval, or
+// a window whose run climbs more than 4 binades (M > 32), takes the member
+// chain.
+static constexpr int kClasses = 32;
+
+struct ClassTab {
+  double e[kClasses], gs[kClasses], lo[kClasses], hi[kClasses];
+};
+
+// Delta keeps y + Delta in y's binade (margins of 4 ulps); a power of two or
+// a value outside the normal range admits Delta = 0 only.
+__device__ __forceinline__ void keep_binade(double y, double &dlo, double &dhi, int &emax) {
+  const uint64_t bits = (uint64_t)__double_as_longlong(y);
+  const int eb = (int)((bits >> 52) & 0x7ff);
+  if (eb < 64 || eb > 2045 || (bits & 0xfffffffffffffull) == 0) {
+    dlo = fmax(dlo, 0.0);
+    dhi = fmin(dhi, 0.0);
+    emax = max(emax, eb);
+    return;
+  }
+  const double bl = pow2_field(eb), bh = pow2_field(eb + 1), m = pow2_field(eb - 50);
+  const double L = y > 0.0 ? bl + m : -bh + m, U = y > 0.0 ? bh - m : -bl - m;
+  dlo = fmax(dlo, L - y);
+  dhi = fmin(dhi, U - y);
+  emax = max(emax, eb);
+}
+
+// The values a[b, e) added in order to S, with the binade constraints.
+__device__ __forceinline__ double ordered_steps(const double *__restrict__ a, int64_t b, int64_t e,
+                                               double S, double &dlo, double &dhi, int &emax) {
+  for (int64_t i = b; i < e; i++) {
+    S = S + a[i];
+    keep_binade(S, dlo, dhi, emax);
+  }
+  return S;
+}
+
+__global__ void __launch_bounds__(64) seq_classes(const double *__restrict__ a,
+                                                  const SegRec *__restrict__ sg, int cap,
+                                                  const CompRec *__restrict__ cp,
+                                                  int32_t *__restrict__ tab_m,
+                                                  ClassTab *__restrict__ tab) {
+  const int64_t w = blockIdx.x;
+  const CompRec C = cp[w];
+  if ((C.meta >> 5) & 1) return;  // folded: no table needed
+  const int l = threadIdx.x, nm = C.meta & 31;
+  const int64_t c0 = w * kWin;
+  // lane r holds record r of the window (as the member chain)
+  int ch = 0, k = l, tot = 0;
+  for (int i = 0; i < nm; i++) {
+    if (l >= tot) {
+      ch = i;
+      k = l - tot;
+    }
+    tot += (C.cnts >> (4 * i)) & 15;
+  }
+  SegRec R;
+  if (l < tot) {
+    R = sg[(c0 + ch) * cap + k];
+  } else {
+    R.meta = 0, R.g = 0.0, R.d0 = R.d1 = R.lo = R.hi = R.xh = 0.0, R.chunk = 0;
+  }
+  const double g = readlane_d(R.g, 0);
+  const int meta0 = __builtin_amdgcn_readlane(R.meta, 0);
+  const int e0 = binade(g);
+  bool ok = tot > 0 && !((meta0 >> 19) & 1) && e0 >= 64 && e0 <= 2045;
+  const double u = ok ? pow2_field(e0) * 0x1p-52 : 0.0;
+  const int c = l < kClasses ? l : 0;
+  double S = g + (double)c * u;  // exact: c < 2^5 ulps
+  ok = ok && segkey(S) == segkey(g);
+  double dlo = -0x1p1000, dhi = 0x1p1000;
+  int emax = e0;
+  // per-record quantities that do not depend on the start
+  const int ebr = binade(R.g);
+  const bool ebr_ok = ebr >= 2 && ebr <= 2045;
+  const double uir = ebr_ok ? pow2_field(2098 - ebr) : 0.0;
+  const bool gin = ebr_ok && run_inside(R.g, ebr, R.lo, R.hi);
+  for (int r = 0; r < tot && __ballot(ok); r++) {
+    const int meta = __builtin_amdgcn_readlane(R.meta, r);
+    const int64_t base = (int64_t)__builtin_amdgcn_readlane(R.chunk, r) * kChunk;
+    const int64_t b = base + (meta & 511), e = base + ((meta >> 9) & 511);
+    if ((meta >> 19) & 1) {
+      S = ordered_steps(a, b, e, S, dlo, dhi, emax);
+      continue;
+    }
+    const int head = (meta >> 18) & 1;
+    if (head) {
+      S = S + readlane_d(R.xh, r);
+      keep_binade(S, dlo, dhi, emax);
+    }
+    const double gr = readlane_d(R.g, r), lor = readlane_d(R.lo, r), hir = readlane_d(R.hi, r);
+    const int eb = binade(gr);
+    if (__builtin_amdgcn_readlane((int)gin, r) && segkey(S) == segkey(gr) &&
+        run_inside(S, eb, lor, hir)) {
+      // the record's run from S + Delta is its run from S displaced: the
+      // range of run_inside, relative to S
+      const double bl = pow2_field(eb), bh = pow2_field(eb + 1), m = 4.0 * bl * 0x1p-52;
+      const double rlo = S < 0.0 ? -hir : lor, rhi = S < 0.0 ? -lor : hir;
+      const double ax = fabs(S);
+      const double alo = (bl + m - rlo) - ax, ahi = (bh - m - rhi) - ax;  // on |S|
+      dlo = fmax(dlo, S < 0.0 ? -ahi : alo);
+      dhi = fmin(dhi, S < 0.0 ? -alo : ahi);
+      emax = max(emax, eb);
+      const int p = parity_of(S - gr, readlane_d(uir, r));
+      S = S + (p ? readlane_d(R.d1, r) : readlane_d(R.d0, r));
+    } else {
+      S = ordered_steps(a, b + head, e, S, dlo, dhi, emax);
+    }
+  }
+  // M u = 2 u_max over every lane's run
+  int em = emax;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) em = max(em, __shfl_xor(em, o));
+  const int lgm = 1 + em - e0;
+  const int M = (lgm >= 1 && lgm <= 5) ? (1 << lgm) : 0;
+  const bool all_ok = __ballot(l < M && !ok) == 0;
+  if (l < kClasses) {
+    tab[w].e[l] = S;
+    tab[w].gs[l] = g + (double)c * u;
+    tab[w].lo[l] = dlo;
+    tab[w].hi[l] = dhi;
+  }
+  if (l == 0) tab_m[w] = (M > 0 && all_ok && __ballot(ok) != 0) ? M : 0;
+}
+
+// The window's run from S by its class table; false when the table does
+// not cover S (the caller then takes the member chain).
+__device__ __forceinline__ bool class_lookup(const int32_t *__restrict__ tab_m,
+                                             const ClassTab *__restrict__ tab,
+                                             const SegRec *__restrict__ sg, int cap, int64_t win,
+                                             double &S) {
+  const int M = tab_m[win];
+  if (M == 0) return false;
+  const int l = lane_id();
+  const ClassTab &T = tab[win];
+  const double e = l < kClasses ? T.e[l] : 0.0, gs = l < kClasses ? T.gs[l] : 0.0;
+  const double lo = l < kClasses ? T.lo[l] : 0.0, hi = l < kClasses ? T.hi[l] : 0.0;
+  const double g = readlane_d(gs, 0);
+  if (segkey(S) != segkey(g)) return false;
+  // k = (S - g) / u: the difference of the mantissas, negated for g < 0
+  const int dm = (int)((uint32_t)__double2loint(S) - (uint32_t)__double2loint(g));
+  const int c = (g < 0.0 ? -dm : dm) & (M - 1);
+  const double delta = S - readlane_d(gs, c);  // exact: one binade
+  if (!(delta >= readlane_d(lo, c) && delta <= readlane_d(hi, c))) return false;
+  S = readlane_d(e, c) + delta;
+  return true;
+}
+
+// A window that did not fold: its class table, else the member chain.
+__device__ void window_level(const double *__restrict__ a, const SegRec *__restrict__ sg, int cap,
+                             const int32_t *__restrict__ tab_m, const ClassTab *__restrict__ tab,
+                             int64_t win, int nm, uint64_t cnts, double &S) {
+  if (class_lookup(tab_m, tab, sg, cap, win, S)) return;
+  member_level(a, sg, cap, win, nm, cnts, S);
+}
+
 // The chain over 64 windows: as chain_group, with each composite's parity
 // map, translations and admissible intervals; windows that did not fold,
 // and the first rejected one, go to the member chain.
 __device__ void chain_comp(const double *__restrict__ a, const SegRec *__restrict__ sg, int cap,
+                           const int32_t *__restrict__ tab_m, const ClassTab *__restrict__ tab,
                            const CompRec &C, int nv, double &S) {
   const int l = lane_id();
   const bool valid = l < nv;
@@ -551,7 +745,7 @@ __device__ void chain_comp(const double *__restrict__ a, const SegRec *__restric
     const int meta_q = __builtin_amdgcn_readlane(C.meta, q);
     const int win_q = __builtin_amdgcn_readlane(C.win, q);
     if (!((meta_q >> 5) & 1)) {
-      member_level(a, sg, cap, win_q, meta_q & 31, readlane_u64(C.cnts, q), S);
+      window_level(a, sg, cap, tab_m, tab, win_q, meta_q & 31, readlane_u64(C.cnts, q), S);
       q++;
       continue;
     }
@@ -588,7 +782,7 @@ __device__ void chain_comp(const double *__restrict__ a, const SegRec *__restric
     S = readlane_d(Sl, f);
     const int meta_f = __builtin_amdgcn_readlane(C.meta, f);
     const int win_f = __builtin_amdgcn_readlane(C.win, f);
-    member_level(a, sg, cap, win_f, meta_f & 31, readlane_u64(C.cnts, f), S);
+    window_level(a, sg, cap, tab_m, tab, win_f, meta_f & 31, readlane_u64(C.cnts, f), S);
     q = f + 1;
   }
 }
@@ -601,8 +795,10 @@ static constexpr int kBlockComps = 256;
 __global__ void __launch_bounds__(256) seq_chain(const double *__restrict__ a,
                                                  const SegRec *__restrict__ sg, int cap,
                                                  const CompRec *__restrict__ cp, int64_t M,
-                                                 double *out) {
+                                                 const int32_t *__restrict__ tab_m,
+                                                 const ClassTab *__restrict__ tab, double *out) {
   __shared__ CompRec buf[2][kBlockComps];
+  __shared__ int sink;
   const int t = threadIdx.x, w = t >> 6;
   const int64_t nblk = (M + kBlockComps - 1) / kBlockComps;
   double S = 0.0;  // the reference's accumulator starts at +0
@@ -612,8 +808,20 @@ __global__ void __launch_bounds__(256) seq_chain(const double *__restrict__ a,
   for (int64_t k = 0; k < nblk; k++) {
     if (w != 0) {
       const int64_t b1 = (k + 1) * kBlockComps;
-      for (int j = t - 64; j < kBlockComps; j += 192)
-        if (b1 + j < M) buf[(k + 1) & 1][j] = cp[b1 + j];
+      uint64_t touched = 0;
+      for (int j = t - 64; j < kBlockComps; j += 192) {
+        if (b1 + j >= M) break;
+        const CompRec C = cp[b1 + j];
+        buf[(k + 1) & 1][j] = C;
+        if (!((C.meta >> 5) & 1)) {
+          // the member chain will read this window's records: one load per
+          // 128-byte line brings them into this CU's caches meanwhile
+          const char *base = (const char *)(sg + (int64_t)C.win * kWin * cap);
+          const int bytes = (C.meta & 31) * cap * (int)sizeof(SegRec);
+          for (int o = 0; o < bytes; o += 128) touched ^= *(const uint64_t *)(base + o);
+        }
+      }
+      if (touched == 0x7ff4dead0badf00dull) sink = 1;  // keeps the loads
     } else {
       const int64_t b0 = k * kBlockComps;
       for (int g0 = 0; g0 < kBlockComps && b0 + g0 < M; g0 += 64) {
@@ -627,7 +835,7 @@ __global__ void __launch_bounds__(256) seq_chain(const double *__restrict__ a,
           C.meta = 0;
           C.cnts = 0;
         }
-        chain_comp(a, sg, cap, C, nv, S);
+        chain_comp(a, sg, cap, tab_m, tab, C, nv, S);
       }
     }
     __syncthreads();
@@ -644,7 +852,7 @@ __global__ void seq_zero(double *out) {
 static int seg_cap(int64_t nch) { return nch <= 65536 ? 64 / kWin : 2; }
 
 struct SeqLayout {
-  int64_t nch, nb, nw, tot, blk, segs, cnt, cps, bytes;
+  int64_t nch, nb, nw, tot, blk, segs, cnt, cps, tabm, tab, bytes;
   int cap;
 };
 
@@ -660,11 +868,13 @@ static SeqLayout seq_layout(int64_t n) {
   L.segs = al(L.blk + 16 * L.nb);
   L.cnt = al(L.segs + (int64_t)sizeof(SegRec) * L.nch * L.cap);
   L.cps = al(L.cnt + 4 * L.nch);
-  L.bytes = al(L.cps + (int64_t)sizeof(CompRec) * L.nw);
+  L.tabm = al(L.cps + (int64_t)sizeof(CompRec) * L.nw);
+  L.tab = al(L.tabm + 4 * L.nw);
+  L.bytes = al(L.tab + (int64_t)sizeof(ClassTab) * L.nw);
   return L;
 }
 
-// The three launches of one sum over a work area laid out by seq_layout(n).
+// The four launches of one sum over a work area laid out by seq_layout(n).
 static int seqsum_launch(const double *a, int64_t n, double *dout, char *w, const SeqLayout &L,
                          hipStream_t s) {
   double *tot = (double *)(w + L.tot);
@@ -672,6 +882,8 @@ static int seqsum_launch(const double *a, int64_t n, double *dout, char *w, cons
   SegRec *segs = (SegRec *)(w + L.segs);
   int32_t *cnt = (int32_t *)(w + L.cnt);
   CompRec *cps = (CompRec *)(w + L.cps);
+  int32_t *tabm = (int32_t *)(w + L.tabm);
+  ClassTab *tab = (ClassTab *)(w + L.tab);
   if (n == 0) {
     hipLaunchKernelGGL(seq_zero, dim3(1), dim3(64), 0, s, dout);
     HYD_HIP(hipGetLastError());
@@ -681,7 +893,10 @@ static int seqsum_launch(const double *a, int64_t n, double *dout, char *w, cons
                      segs, cnt, cps, L.cap);
   hipLaunchKernelGGL(seq_chunks<1>, dim3((unsigned)L.nb), dim3(64), 0, s, a, n, L.nch, tot, blk,
                      segs, cnt, cps, L.cap);
-  hipLaunchKernelGGL(seq_chain, dim3(1), dim3(256), 0, s, a, segs, L.cap, cps, L.nw, dout);
+  hipLaunchKernelGGL(seq_classes, dim3((unsigned)L.nw), dim3(64), 0, s, a, segs, L.cap, cps, tabm,
+                     tab);
+  hipLaunchKernelGGL(seq_chain, dim3(1), dim3(256), 0, s, a, segs, L.cap, cps, L.nw, tabm, tab,
+                     dout);
   HYD_HIP(hipGetLastError());
   return HYD_OK;
 }
@@ -698,13 +913,13 @@ int seqsum(Ctx &c, const double *a, int64_t n, double *dout) {
 
 // ---- the sequential sum without allocation ---------------------------------
 // Work area in a file-scope device array: each GPU the library runs on holds
-// its own copy (15.05 MiB used of 16 MiB for the largest sum, 2^22 values)
+// its own copy (19.06 MiB used of 24 MiB for the largest sum, 2^22 values)
 // until the process ends.  Host code never passes `scratch` itself to a
 // kernel (in host code the name is a host-side stand-in): each device's
 // address comes from hipGetSymbolAddress, taken once per device, with that
 // device current, on the first call there.
 static constexpr int64_t kNoAllocMaxN = (int64_t)1 << 22;
-static constexpr int64_t kNoAllocBytes = (int64_t)16 << 20;
+static constexpr int64_t kNoAllocBytes = (int64_t)24 << 20;
 static __device__ __align__(256) char scratch[kNoAllocBytes];
 
 static constexpr int kMaxDevices = 64;

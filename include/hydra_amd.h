@@ -175,7 +175,7 @@ int hyd_sum_sequential(hyd_ctx *ctx, const double *a, int64_t n, double *sum);
    without waiting.  No counterpart in the reference: it is for callers that
    cannot allocate -- code under HIP graph capture, tight serving loops,
    harnesses that forbid hipMalloc.  The work area is a static device array
-   (16 MiB per GPU, held until the process ends); n is limited to 2^22
+   (24 MiB per GPU, held until the process ends); n is limited to 2^22
    (HYD_ECAPACITY beyond).  All calls on one GPU share that array, so calls
    that overlap on one GPU (different streams, or threads) race. */
 int hyd_sum_sequential_noalloc(const double *a, int64_t n, double *sum, void *stream);

@@ -127,6 +127,37 @@ def test_sum_sequential_long_bitwise():
     h.close()
 
 
+def test_sum_sequential_binade_walks_bitwise():
+    """Running sums that wander across powers of two inside the windows of
+    1024 values (crossings up and down, several binades, sign changes, exact
+    ties), the case the per-window class tables resolve; bit for bit."""
+    from hydra_amd import Hydra
+    h = Hydra(256)
+    rng = np.random.default_rng(21)
+    n = 4 * 256 * 256
+    steps = rng.standard_normal(n)
+    cases = []
+    for level in (1024.0, 1.0, 2.0 ** -20, 3.0e5):
+        a = steps * level * 0.02
+        a[0] = level * 0.99  # start just below the power of two
+        cases.append(a)
+    walk = np.cumsum(rng.standard_normal(n) * 0.5)
+    cases.append(np.diff(np.r_[0.0, np.sin(walk) * 700.0 + 1024.0 * np.cos(walk / 50.0)]))
+    ties = np.round(rng.standard_normal(n) * 8.0) * 0.25
+    ties[0] = 511.875
+    cases.append(ties)
+    sign = rng.standard_normal(n) * 3.0
+    sign[::4096] += np.where(np.arange(len(sign[::4096])) % 2 == 0, 5e4, -5e4)
+    cases.append(sign)
+    for a in cases:
+        ref = 0.0
+        for v in a.tolist():
+            ref = ref + v
+        got = h.sum_sequential(a)
+        assert np.float64(got).view(np.int64) == np.float64(ref).view(np.int64), (got, ref)
+    h.close()
+
+
 @pytest.mark.parametrize("ng", [32, 64])
 def test_advance_bitwise(ng):
     """Several RK4 steps of the device-resident state, every field and node
