@@ -38,7 +38,6 @@
 namespace hyd {
 
 static constexpr int kChunk = 64;    // values per chunk
-static constexpr int kTile = 32;     // columns staged through LDS per pass
 
 // A segment of a chunk's guessed run: an optional head addition, then
 // values whose guessed partial sums all share the binade of g.
@@ -118,6 +117,16 @@ __device__ __forceinline__ double wave_incl_add(double v) {
   t = dpp_d<0x143>(v); if (l >= 32) v = t + v;
   return v;
 }
+// Inclusive scans inside each DPP row of 16 lanes.
+__device__ __forceinline__ double row_incl_add(double v) {
+  const int rl = lane_id() & 15;
+  double t;
+  t = dpp_d<0x111>(v); if (rl >= 1) v = t + v;
+  t = dpp_d<0x112>(v); if (rl >= 2) v = t + v;
+  t = dpp_d<0x114>(v); if (rl >= 4) v = t + v;
+  t = dpp_d<0x118>(v); if (rl >= 8) v = t + v;
+  return v;
+}
 // value of lane l-1 (0 for lane 0)
 __device__ __forceinline__ double wave_prev_d(double v) {
   const double t = dpp_d<0x138>(v);
@@ -139,6 +148,16 @@ __device__ __forceinline__ int wave_incl_map(int v) {
   t = dpp_i<0x118>(v); if (rl >= 8) v = map_then(t, v);
   t = dpp_i<0x142>(v); if ((l & 31) >= 16) v = map_then(t, v);
   t = dpp_i<0x143>(v); if (l >= 32) v = map_then(t, v);
+  return v;
+}
+
+__device__ __forceinline__ int row_incl_map(int v) {
+  const int rl = lane_id() & 15;
+  int t;
+  t = dpp_i<0x111>(v); if (rl >= 1) v = map_then(t, v);
+  t = dpp_i<0x112>(v); if (rl >= 2) v = map_then(t, v);
+  t = dpp_i<0x114>(v); if (rl >= 4) v = map_then(t, v);
+  t = dpp_i<0x118>(v); if (rl >= 8) v = map_then(t, v);
   return v;
 }
 
@@ -201,7 +220,7 @@ __device__ __forceinline__ int parity_of(double num, double uinv) {
 // following the GF(2) maps) and the window adds D(p) = off_(K+1)(p).  The
 // member chain would accept member i iff its start lies on the grid, inside
 // the binade with the 4-ulp margins around its run; those conditions are
-// intervals in delta, intersected per parity here with one extra ulp for
+// intervals in delta, intersected per parity (level 1) with one extra ulp for
 // the rounding of the margin tests.  A start inside its interval is
 // therefore accepted by every member and gives the member chain's value;
 // any other window is summed by the member chain.
@@ -216,67 +235,10 @@ struct CompRec {
   uint64_t cnts;              // record count of member chunk i in bits 4i..4i+3
 };
 
-// Folds members R[0..nm) (one record per chunk); gnext is the guess of the
-// record after the window (has_next: one exists).
-__device__ CompRec compose_window(const SegRec *R, int nm, bool has_next, double gnext,
-                                  int64_t win) {
-  const SegRec &A = R[0];
-  const int key = segkey(A.g), eb = binade(A.g);
-  int ok = !((A.meta >> 18) & 3) && eb >= 64 && eb <= 2045;
-  const double ui = ok ? pow2_field(2098 - eb) : 0.0;
-  const double sig = A.g < 0.0 ? -1.0 : 1.0;
-  const double G1 = ok ? (fabs(A.g) - pow2_field(eb)) * ui : 0.0;
-  double D[2] = {0.0, 0.0}, lo[2] = {-0x1p60, -0x1p60}, hi[2] = {0x1p60, 0x1p60};
-  int pn[2] = {0, 0};
-  for (int i = 0; i < nm && ok; i++) {
-    const SegRec &Ri = R[i];
-    if (i > 0 && (((Ri.meta >> 18) & 3) != 0 || segkey(Ri.g) != key)) {
-      ok = 0;
-      break;
-    }
-    if (!run_inside(Ri.g, eb, Ri.lo, Ri.hi)) {
-      ok = 0;
-      break;
-    }
-    const double rlo = sig > 0.0 ? Ri.lo : -Ri.hi;
-    const double rhi = sig > 0.0 ? Ri.hi : -Ri.lo;
-    const double Li = ceil(5.0 - rlo * ui) + 1.0;
-    const double Ui = floor(0x1p52 - 5.0 - rhi * ui) - 1.0;
-    const bool last = i + 1 == nm;
-    const double gn = last ? gnext : R[i + 1].g;
-    const int am = (!last || has_next) ? parity_of((Ri.g + Ri.d0) - gn, ui) : 0;
-    const int bm = parity_of(Ri.d1 - Ri.d0, ui);
-#pragma unroll
-    for (int p0 = 0; p0 < 2; p0++) {
-      // member i's start magnitude offset is G1 + delta + sig * off_i / u
-      const double O = sig * D[p0] * ui;
-      lo[p0] = fmax(lo[p0], Li - G1 - O);
-      hi[p0] = fmin(hi[p0], Ui - G1 - O);
-      const int p = i == 0 ? p0 : pn[p0];
-      D[p0] = D[p0] + (p ? Ri.d1 : Ri.d0);
-      pn[p0] = bm ? am : (p ^ am);
-      // partial translations stay well inside 2^53 u, so every sum is exact
-      if (fabs(D[p0]) * ui > 0x1p51) ok = 0;
-    }
-  }
-  CompRec C;
-  C.g = A.g;
-  C.d0 = D[0];
-  C.d1 = D[1];
-  C.lo0 = lo[0];
-  C.hi0 = hi[0];
-  C.lo1 = lo[1];
-  C.hi1 = hi[1];
-  C.win = (int32_t)win;
-  const int mc = pn[0], mk = pn[0] ^ pn[1];
-  C.meta = nm | (ok << 5) | (((mk << 1) | mc) << 6);
-  return C;
-}
-
 // ---- level 1 --------------------------------------------------------------
-// One wavefront owns 64 consecutive chunks (a thread per chunk); values are
-// staged through LDS in kTile-wide column tiles, the next tile's loads in
-// flight while the current one is summed.
+// One wavefront owns 64 consecutive chunks (a thread per chunk); the block's
+// 64 x 64 values are staged through LDS in one coalesced pass and each
+// thread walks its chunk's row, reading eight values ahead of its adds.
 // MODE 0: chunk totals from +0 (tot) and the block's double-double total
 //         (blk).
 // MODE 1: guesses (double-double prefix of the chunk totals: earlier blocks,
@@ -292,17 +254,31 @@ __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, i
                                                  int64_t nch, double *tot, double2 *blk,
                                                  SegRec *segs, int32_t *segcnt, CompRec *cps,
                                                  int cap) {
-  __shared__ double tile[64][kTile + 1];
+  __shared__ double tile[64][kChunk + 1];
   __shared__ SegRec first[64];
   __shared__ int32_t fcnt[64];
   __shared__ double gnx;
   const int t = threadIdx.x;
   const int64_t c0 = (int64_t)blockIdx.x * 64;
   const int64_t c = c0 + t;
+  // the block's 64 chunks, staged whole: thread t loads value t of every
+  // chunk (coalesced) before the prologue, so the loads overlap it
+  double v[kChunk];
+#pragma unroll
+  for (int j = 0; j < kChunk; j++) {
+    const int64_t gi = (c0 + j) * kChunk + t;
+    v[j] = gi < n ? a[gi] : 0.0;
+  }
   double s = 0.0;
   SegState z;
   int cnt = 0;
   SegRec *out = segs + c * cap;
+  double gch = 0.0;
+  // MODE 1 walks the two runs (from g and g + u) branch-free first; a chunk
+  // whose run from g changes binade is walked again, value by value, into
+  // its segments
+  double f0 = 0.0, f1 = 0.0, fmn = 0.0, fmx = 0.0;
+  bool chg = false, fbad = false;
   if (MODE == 1) {
     // offset: earlier blocks' totals (any order: the guess needs no exactness)
     double oh = 0.0, ol = 0.0;
@@ -323,57 +299,50 @@ __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, i
     if (t == 0) eh = el = 0.0;
     double gh, gl;
     dd_add(oh, ol, eh, el, gh, gl);
-    if (c < nch) seg_open(z, 0, 0, 0.0, gh + gl);
+    gch = gh + gl;
+    if (c < nch) seg_open(z, 0, 0, 0.0, gch);
+    f0 = z.s0;
+    f1 = z.s1;
+    fmn = fmx = z.g;
     if (t == 63) {
       dd_add(oh, ol, ih, il, gh, gl);
       gnx = gh + gl;  // guess of the next block's first chunk
     }
   }
-  constexpr int per = 64 * kTile / 64;  // values each thread stages per tile
-  double nxt[per];
-  auto fetch = [&](int col0) {
 #pragma unroll
-    for (int j = 0; j < per; j++) {
-      const int idx = t + 64 * j, row = idx / kTile, col = idx - row * kTile;
-      const int64_t gi = (c0 + row) * kChunk + col0 + col;
-      nxt[j] = gi < n ? a[gi] : 0.0;
-    }
-  };
-  fetch(0);
-  for (int col0 = 0; col0 < kChunk; col0 += kTile) {
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < per; j++) {
-      const int idx = t + 64 * j, row = idx / kTile, col = idx - row * kTile;
-      tile[row][col] = nxt[j];
-    }
-    __syncthreads();
-    if (col0 + kTile < kChunk) fetch(col0 + kTile);
-    if (c < nch) {
-      const int64_t base = c * kChunk + col0;
-#pragma unroll
-      for (int j = 0; j < kTile; j++) {
-        if (base + j < n) {
-          const double x = tile[t][j];
-          if (MODE == 0) {
-            s = s + x;
-          } else {
-            const double t0 = z.s0 + x;
-            if (segkey(t0) != z.key) {
-              seg_close(z, col0 + j, c, out, cnt, cap);
-              seg_open(z, col0 + j, 1, x, t0);
-            } else {
-              const double r0 = (z.s0 - z.g) + x;
-              const double r1 = (z.s1 - z.g) + x;
-              z.lo = fmin(z.lo, fmin(r0, r1));
-              z.hi = fmax(z.hi, fmax(r0, r1));
-              z.s0 = t0;
-              z.s1 = z.s1 + x;
-              if (segkey(z.s1) != z.key) z.bad = 1;
-            }
-          }
-        }
+  for (int j = 0; j < kChunk; j++) tile[j][t] = v[j];
+  __syncthreads();
+  if (c < nch) {
+    const double *row = tile[t];
+    auto step = [&](double x) {
+      if (MODE == 0) {
+        s = s + x;
+      } else {
+        // the rounded partial sums bound the exact ones within half an ulp,
+        // as the pre-rounding offsets would
+        const double t0 = f0 + x, t1 = f1 + x;
+        chg = chg || segkey(t0) != z.key;
+        fbad = fbad || segkey(t1) != z.key;
+        const double lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
+        fmn = lo < fmn ? lo : fmn;
+        fmx = hi > fmx ? hi : fmx;
+        f0 = t0;
+        f1 = t1;
       }
+    };
+    if ((c + 1) * kChunk <= n) {
+      // eight values read ahead of the chain that consumes them
+#pragma unroll
+      for (int j0 = 0; j0 < kChunk; j0 += 8) {
+        double x[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[j] = row[j0 + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) step(x[j]);
+      }
+    } else {
+      const int e = (int)(n - c * kChunk);
+      for (int j = 0; j < e; j++) step(row[j]);
     }
   }
   if (MODE == 0) {
@@ -386,11 +355,53 @@ __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, i
   }
   if (c < nch) {
     const int e = (int)min((int64_t)kChunk, n - c * kChunk);
+    if (!chg) {
+      // one segment: the fast walk's sums (offsets exact within one binade)
+      z.s0 = f0;
+      z.s1 = f1;
+      z.lo = fmin(0.0, fmn - z.g);
+      z.hi = fmax(0.0, fmx - z.g);
+      if (fbad) z.bad = 1;
+    } else {
+      // the same walk, cut into segments where the run from g changes
+      // binade (values read eight ahead)
+      const double *row = tile[t];
+      double mn = z.g, mx = z.g;
+      for (int j0 = 0; j0 < e; j0 += 8) {
+        double x8[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) x8[j] = j0 + j < e ? row[j0 + j] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          if (j0 + j >= e) break;
+          const double x = x8[j];
+          const double t0 = z.s0 + x;
+          if (segkey(t0) != z.key) {
+            z.lo = fmin(0.0, mn - z.g);
+            z.hi = fmax(0.0, mx - z.g);
+            seg_close(z, j0 + j, c, out, cnt, cap);
+            seg_open(z, j0 + j, 1, x, t0);
+            mn = mx = z.g;
+          } else {
+            const double t1 = z.s1 + x;
+            const double lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
+            mn = lo < mn ? lo : mn;
+            mx = hi > mx ? hi : mx;
+            z.s0 = t0;
+            z.s1 = t1;
+            if (segkey(t1) != z.key) z.bad = 1;
+          }
+        }
+      }
+      z.lo = fmin(0.0, mn - z.g);
+      z.hi = fmax(0.0, mx - z.g);
+    }
     seg_close(z, e, c, out, cnt, cap);
     if (cnt > cap) {
-      // too many binade changes: the whole chunk is summed in order
+      // too many binade changes: the whole chunk is summed in order (the
+      // record keeps the chunk's guess for the window's class table)
       SegRec r;
-      r.g = 0.0;
+      r.g = gch;
       r.d0 = r.d1 = r.lo = r.hi = r.xh = 0.0;
       r.chunk = (int32_t)c;
       r.meta = 0 | (e << 9) | (1 << 19);
@@ -402,28 +413,98 @@ __global__ void __launch_bounds__(64) seq_chunks(const double *__restrict__ a, i
     fcnt[t] = cnt;
   }
   __syncthreads();
-  // composites: thread w folds window w of the block
-  if (t < 64 / kWin) {
-    const int64_t w0 = c0 + (int64_t)t * kWin;
-    if (w0 < nch) {
-      const int nm = (int)min((int64_t)kWin, nch - w0);
-      bool single = true;
-      for (int i = 0; i < nm; i++) single = single && fcnt[t * kWin + i] == 1;
-      const bool has_next = w0 + nm < nch;
-      const double gnext = (t + 1) * kWin < 64 ? first[(t + 1) * kWin].g : gnx;
-      uint64_t cn = 0;
-      for (int i = 0; i < nm; i++) cn |= (uint64_t)fcnt[t * kWin + i] << (4 * i);
-      CompRec C;
-      if (single) {
-        C = compose_window(&first[t * kWin], nm, has_next, gnext, w0 / kWin);
-      } else {
-        C.g = C.d0 = C.d1 = C.lo0 = C.hi0 = C.lo1 = C.hi1 = 0.0;
-        C.win = (int32_t)(w0 / kWin);
-        C.meta = nm;
-      }
-      C.cnts = cn;
-      cps[w0 / kWin] = C;
+  // composites: the 16 lanes of DPP row w fold window w of the block, lane
+  // i taking member i: start parities by a row scan of the members' maps,
+  // translations by row scans (exact sums), intervals by row max/min
+  const int wl = t / kWin, i = t % kWin;
+  const int64_t w0 = c0 + (int64_t)wl * kWin;
+  const int nm = w0 < nch ? (int)min((int64_t)kWin, nch - w0) : 0;
+  const bool mem = i < nm;
+  const int fc = mem ? fcnt[t] : 1;
+  const bool single = __ballot(!(fc == 1)) >> (wl * kWin) & 0xffff ? false : true;
+  uint64_t cn = mem ? (uint64_t)fc << (4 * i) : 0;
+#pragma unroll
+  for (int m = 1; m < kWin; m <<= 1) cn |= __shfl_xor(cn, m);
+  const SegRec A = first[wl * kWin];
+  const SegRec Ri = first[t];
+  const int key = segkey(A.g), eb = binade(A.g);
+  const bool ok0 = !((A.meta >> 18) & 3) && eb >= 64 && eb <= 2045;
+  const double ui = ok0 ? pow2_field(2098 - eb) : 0.0;
+  const double sig = A.g < 0.0 ? -1.0 : 1.0;
+  const double G1 = ok0 ? (fabs(A.g) - pow2_field(eb)) * ui : 0.0;
+  bool okm = true;
+  int mapm = 2;  // identity
+  double d0m = 0.0, d1m = 0.0, Li = -0x1p60, Ui = 0x1p60;
+  if (mem && single && ok0) {
+    okm = (i == 0 || (((Ri.meta >> 18) & 3) == 0 && segkey(Ri.g) == key)) &&
+          run_inside(Ri.g, eb, Ri.lo, Ri.hi);
+    const double rlo = sig > 0.0 ? Ri.lo : -Ri.hi;
+    const double rhi = sig > 0.0 ? Ri.hi : -Ri.lo;
+    Li = ceil(5.0 - rlo * ui) + 1.0;
+    Ui = floor(0x1p52 - 5.0 - rhi * ui) - 1.0;
+    const bool last = i + 1 == nm;
+    const bool has_next = w0 + nm < nch;
+    const double gn = !last ? first[t + 1].g : ((wl + 1) * kWin < 64 ? first[(wl + 1) * kWin].g : gnx);
+    const int am = (!last || has_next) ? parity_of((Ri.g + Ri.d0) - gn, ui) : 0;
+    const int bm = parity_of(Ri.d1 - Ri.d0, ui);
+    mapm = ((bm ? 0 : 1) << 1) | am;  // p -> bm ? am : p ^ am
+    d0m = Ri.d0;
+    d1m = Ri.d1;
+  }
+  // start parity of member i for window start parity 0 and 1
+  const int rl = t & 15;
+  const int incl = row_incl_map(mapm);
+  int ex = dpp_i<0x111>(incl);
+  if (rl == 0) ex = 2;
+  const int pa = (ex >> 1) ? (0 ^ (ex & 1)) : (ex & 1);
+  const int pb = (ex >> 1) ? (1 ^ (ex & 1)) : (ex & 1);
+  const double da = mem ? (pa ? d1m : d0m) : 0.0, db = mem ? (pb ? d1m : d0m) : 0.0;
+  const double ia = row_incl_add(da), ib = row_incl_add(db);
+  double xa = dpp_d<0x111>(ia), xb = dpp_d<0x111>(ib);
+  if (rl == 0) xa = xb = 0.0;
+  // admissible start offsets: member i's start is G1 + delta + sig D / u
+  double lo0 = -0x1p60, hi0 = 0x1p60, lo1 = -0x1p60, hi1 = 0x1p60;
+  if (mem && single && ok0) {
+    const double Oa = sig * xa * ui, Ob = sig * xb * ui;
+    lo0 = Li - G1 - Oa;
+    hi0 = Ui - G1 - Oa;
+    lo1 = Li - G1 - Ob;
+    hi1 = Ui - G1 - Ob;
+    // partial translations stay well inside 2^53 u, so every sum is exact
+    if (fabs(ia) * ui > 0x1p51 || fabs(ib) * ui > 0x1p51) okm = false;
+  }
+#pragma unroll
+  for (int m = 1; m < kWin; m <<= 1) {
+    lo0 = fmax(lo0, __shfl_xor(lo0, m));
+    hi0 = fmin(hi0, __shfl_xor(hi0, m));
+    lo1 = fmax(lo1, __shfl_xor(lo1, m));
+    hi1 = fmin(hi1, __shfl_xor(hi1, m));
+  }
+  const bool ok = single && ok0 && ((__ballot(!okm) >> (wl * kWin)) & 0xffff) == 0;
+  // totals and the whole window's map sit in member nm - 1's lane
+  const int src = wl * kWin + (nm > 0 ? nm - 1 : 0);
+  const double Da = __shfl(ia, src), Db = __shfl(ib, src);
+  const int mw = __shfl(incl, src);
+  if (i == 0 && nm > 0) {
+    CompRec C;
+    if (single) {
+      const int pn0 = (mw >> 1) ? (0 ^ (mw & 1)) : (mw & 1);
+      const int pn1 = (mw >> 1) ? (1 ^ (mw & 1)) : (mw & 1);
+      C.g = A.g;
+      C.d0 = Da;
+      C.d1 = Db;
+      C.lo0 = lo0;
+      C.hi0 = hi0;
+      C.lo1 = lo1;
+      C.hi1 = hi1;
+      C.meta = nm | ((ok ? 1 : 0) << 5) | ((((pn0 ^ pn1) << 1) | pn0) << 6);
+    } else {
+      C.g = C.d0 = C.d1 = C.lo0 = C.hi0 = C.lo1 = C.hi1 = 0.0;
+      C.meta = nm;
     }
+    C.win = (int32_t)(w0 / kWin);
+    C.cnts = cn;
+    cps[w0 / kWin] = C;
   }
 }
 
