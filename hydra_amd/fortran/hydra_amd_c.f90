@@ -270,6 +270,70 @@ interface
     type(c_ptr), intent(in) :: ctxs(*)
     integer(c_int32_t), value :: n, nsteps
   end function
+
+  ! the save schedule decided inside adapt (evolution.f90:582-599)
+  integer(c_int) function hyd_set_saves(ctx, on, tinit, igrec, icrec) bind(c, name="hyd_set_saves")
+    import :: c_int, c_ptr, c_int32_t, c_double
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: on, igrec, icrec
+    real(c_double), value :: tinit
+  end function
+
+  integer(c_int) function hyd_pending_saves(ctx, ngrid, ncont) bind(c, name="hyd_pending_saves")
+    import :: c_int, c_ptr, c_int32_t
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), intent(out) :: ngrid, ncont
+  end function
+
+  ! savegrid's record (evolution.f90:646-676): info = (tgrid, ekin, epot,
+  ! igrec), qqs(ny,nx) interpolated to the save time
+  integer(c_int) function hyd_take_grid_save(ctx, info, qqs) bind(c, name="hyd_take_grid_save")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double), intent(out) :: info(4), qqs(*)
+  end function
+
+  integer(c_int) function hyd_contour_save_size(ctx, nq, nptq) bind(c, name="hyd_contour_save_size")
+    import :: c_int, c_ptr, c_int32_t
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), intent(out) :: nq, nptq
+  end function
+
+  ! savecont's record (evolution.f90:680-778): info = (t, qjump, icrec,
+  ! contex), the residual and the contours
+  integer(c_int) function hyd_take_contour_save(ctx, info, residual, xq, yq, npq, i1q, indq) &
+       bind(c, name="hyd_take_contour_save")
+    import :: c_int, c_ptr, c_double, c_int32_t
+    type(c_ptr), value :: ctx
+    real(c_double), intent(out) :: info(4), residual(*), xq(*), yq(*)
+    integer(c_int32_t), intent(out) :: npq(*), i1q(*), indq(*)
+  end function
+
+  ! monitor.asc (kind 0, 7 values a row) and complexity.asc (kind 1, 3 a row)
+  integer(c_int) function hyd_take_records(ctx, kind, max_rows, rows) bind(c, name="hyd_take_records")
+    import :: c_int, c_ptr, c_int32_t, c_double
+    type(c_ptr), value :: ctx
+    integer(c_int32_t), value :: kind, max_rows
+    real(c_double), intent(out) :: rows(*)
+  end function
+
+  ! the learned velocity correction (no reference counterpart)
+  integer(c_int) function hyd_set_correction(ctx, net, su, sq, sd) bind(c, name="hyd_set_correction")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx, net
+    real(c_double), value :: su, sq, sd
+  end function
+
+  integer(c_int) function hyd_contour_velocity(ctx, nptq, xq, yq, nextq, qjump, m, xs, ys, us, vs, &
+       fields) bind(c, name="hyd_contour_velocity")
+    import :: c_int, c_ptr, c_int32_t, c_double
+    type(c_ptr), value :: ctx, fields
+    integer(c_int32_t), value :: nptq, m
+    real(c_double), value :: qjump
+    real(c_double), intent(in) :: xq(*), yq(*), xs(*), ys(*)
+    integer(c_int32_t), intent(in) :: nextq(*)
+    real(c_double), intent(out) :: us(*), vs(*)
+  end function
 end interface
 
 contains

@@ -135,7 +135,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--ng", type=int, default=512)
     ap.add_argument("--factor", type=int, default=4)
-    ap.add_argument("--snapshots", type=int, default=6)
+    ap.add_argument("--snapshots", type=int, default=6, help="training snapshots in all")
+    ap.add_argument("--flows", type=int, default=1,
+                    help="seeded flows the training snapshots are drawn from")
     ap.add_argument("--every", type=int, default=8)
     ap.add_argument("--nodes", type=int, default=60000)
     ap.add_argument("--iters", type=int, default=360)
@@ -155,13 +157,21 @@ def main(argv=None):
     judge = judge_for(f, a.factor)
     sc = flow_scales(f, judge)
     t0 = time.perf_counter()
-    train_set = collect(f, judge, a.snapshots, a.every, sc, a.nodes, seed=1)
+    per = -(-a.snapshots // a.flows)
+    train_set = []
+    for k in range(a.flows):
+        fk = f if k == 0 else flow(a.seed + 1000 + k)
+        train_set += collect(fk, judge, min(per, a.snapshots - len(train_set)), a.every, sc,
+                             a.nodes, seed=1 + k)
+        if fk is not f:
+            fk.close()
     test_set = collect(flow(a.seed + 1), judge, 2, a.every, sc, a.nodes, seed=2)
     t1 = time.perf_counter()
     net = new_network(f)
     hist = train(net, train_set, a.iters)
     t2 = time.perf_counter()
-    out = dict(ng=a.ng, judge_ng=judge.ng, su=sc.su, sq=sc.sq, sd=sc.sd, snapshot_s=t1 - t0,
+    out = dict(ng=a.ng, judge_ng=judge.ng, su=sc.su, sq=sc.sq, sd=sc.sd,
+               snapshots=len(train_set), flows=a.flows, snapshot_s=t1 - t0,
                train_s=t2 - t1, iters=a.iters, loss_first=hist[0], loss_last=hist[-1],
                train_left=residual_ratio(net, train_set[:2]),
                held_out_left=residual_ratio(net, test_set),
