@@ -92,8 +92,8 @@ static void free_ctx(Ctx *c) {
   void *ptrs[] = {c->green, c->flo, c->fhi, c->hrkx, c->hrky, c->xtrig, c->ytrig, c->xgf,
                   c->qdiss_map, c->qdiss_exp, c->qs, c->qd, c->qc, c->qq, c->pp, c->uu, c->vv,
                   c->qr, c->emq, c->epq, c->qsi, c->qsf, c->sqs, c->qdi, c->qdf, c->sqd, c->wa,
-                  c->wb, c->wc, c->wd, c->cnt, c->fq, c->mq, c->qjx_cnt, c->qbot, c->ixc,
-                  c->nxc, c->ndx, c->ndy, c->flags, c->replay, c->replay_val, c->rp_tkey,
+                  c->wb, c->wc, c->wd, c->cnt, c->fq, c->mq, c->qjx_cnt, c->qbot,
+                  c->flags, c->replay, c->replay_val, c->rp_tkey,
                   c->rp_tid, c->rp_list, c->xq, c->yq, c->xqi, c->yqi, c->xqf, c->yqf, c->uq,
                   c->vq, c->nextq, c->nxt_tmp, c->dsc, c->dsum, c->seq_work, c->uu32,
                   c->vv32, c->fa32, c->fb32, c->fc32, c->tmp32, c->fq32, c->mq32, c->xtrig32,
@@ -223,10 +223,6 @@ static int create(const hyd_params *p, hyd_ctx **out) {
     return HYD_EHIP;
   }
   const int64_t P = c->cap;
-  TRY(dalloc(&c->ixc, P));
-  TRY(dalloc(&c->nxc, P));
-  TRY(dalloc(&c->ndx, P));
-  TRY(dalloc(&c->ndy, P));
   double **nodes[] = {&c->xq, &c->yq, &c->xqi, &c->yqi, &c->xqf, &c->yqf, &c->uq, &c->vq};
   for (double **f : nodes) TRY(dalloc(f, P));
   TRY(dalloc(&c->nextq, P));

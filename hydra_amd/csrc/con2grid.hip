@@ -58,8 +58,6 @@ struct C2GArgs {
   // used, [5] dropped edge crossings; [8] sticky error (kept across sweeps)
   int32_t *flags;
   Buckets bk;
-  int32_t *ixc, *nxc;
-  double *ndx, *ndy;
 };
 
 __device__ __forceinline__ bool is_flagged(int32_t w) {
@@ -89,28 +87,41 @@ __device__ __forceinline__ int64_t bk_find(const Buckets &b, int32_t key) {
   return -1;
 }
 
+// The segment from node i to its successor: fine cell of the node,
+// periodic increments, x-crossing count (contours.f90:206-231, congen.f90:
+// 354-383).  Both node passes evaluate it the same way, so the bucket pass
+// recomputes it rather than reading it back.
+struct SegGeom {
+  double xi, yi, dx, dy, yy;
+  int ixci, nxc;
+};
+__device__ __forceinline__ SegGeom seg_geom(const C2GArgs &a, int i) {
+  SegGeom s;
+  s.xi = a.xq[i];
+  s.yi = a.yq[i];
+  const int ia = a.nextq[i] - 1;
+  const double xa = a.xq[ia], ya = a.yq[ia];
+  // fine cell of each node: contours.f90:207 (1-based) / congen.f90:354 (0-based)
+  const int base = a.ultra ? 0 : 1;
+  s.ixci = base + (int)(a.glxfi * (s.xi - a.xmin));
+  const int ixca = base + (int)(a.glxfi * (xa - a.xmin));
+  const double xx = xa - s.xi;
+  s.dx = xx - a.ellx * (double)(int)(xx * a.hlxi);
+  s.yy = ya - s.yi;
+  s.dy = s.yy - a.elly * (double)(int)(s.yy * a.hlyi);
+  const int ixdif = ixca - s.ixci;
+  s.nxc = ixdif - a.nxf * ((2 * ixdif) / a.nxf);
+  return s;
+}
+
 // Node phase: contours.f90:206-231 and the crossing loop 249-269.
 static constexpr int kCrossBatch = 4;
 __global__ void c2g_nodes(C2GArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
-  const double xi = a.xq[i], yi = a.yq[i];
-  const int ia = a.nextq[i] - 1;
-  const double xa = a.xq[ia], ya = a.yq[ia];
-  // fine cell of each node: contours.f90:207 (1-based) / congen.f90:354 (0-based)
-  const int base = a.ultra ? 0 : 1;
-  const int ixci = base + (int)(a.glxfi * (xi - a.xmin));
-  const int ixca = base + (int)(a.glxfi * (xa - a.xmin));
-  const double xx = xa - xi;
-  const double dx = xx - a.ellx * (double)(int)(xx * a.hlxi);
-  const double yy = ya - yi;
-  const double dy = yy - a.elly * (double)(int)(yy * a.hlyi);
-  const int ixdif = ixca - ixci;
-  const int nxc = ixdif - a.nxf * ((2 * ixdif) / a.nxf);
-  a.ixc[i] = ixci;
-  a.nxc[i] = nxc;
-  a.ndx[i] = dx;
-  a.ndy[i] = dy;
+  const SegGeom sg = seg_geom(a, i);
+  const double xi = sg.xi, yi = sg.yi, dx = sg.dx, dy = sg.dy, yy = sg.yy;
+  const int ixci = sg.ixci, nxc = sg.nxc;
   if (fabs(yy) > a.hly) {
     const double cc = copysign(1.0, dy);
     const double p = -(yi - a.hly * cc) / (dy + a.small);
@@ -218,11 +229,9 @@ __global__ void c2g_buckets(C2GArgs a) {
   if (a.flags[0] == 0 && a.flags[1] == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
-  const double xi = a.xq[i], yi = a.yq[i];
-  const double dx = a.ndx[i], dy = a.ndy[i];
-  const int nxc = a.nxc[i], ixci = a.ixc[i];
-  const int ia = a.nextq[i] - 1;
-  const double yy = a.yq[ia] - yi;
+  const SegGeom sg = seg_geom(a, i);
+  const double xi = sg.xi, yi = sg.yi, dx = sg.dx, dy = sg.dy, yy = sg.yy;
+  const int nxc = sg.nxc, ixci = sg.ixci;
   const int64_t nkey = (int64_t)a.nxf * a.nyf;
   if (a.flags[1] != 0 && fabs(yy) > a.hly) {
     const double cc = copysign(1.0, dy);
@@ -590,10 +599,6 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
   a.bk.tmask = (uint32_t)(c.rp_tsize - 1);
   a.bk.bcap = (int32_t)c.rp_bcap;
   a.bk.scap = c.replay_cap;
-  a.ixc = c.ixc;
-  a.nxc = c.nxc;
-  a.ndx = c.ndx;
-  a.ndy = c.ndy;
   const int nb = (n + 255) / 256;
   if (n > 0) {
     hipLaunchKernelGGL(c2g_nodes, dim3(nb), dim3(256), 0, c.s, a);
@@ -890,10 +895,6 @@ int con2grid_mixed(Ctx &c, float *qc, bool defer_mean) {
     a.cnt = c.cnt;
     a.qjx = c.qjx_cnt;
     a.flags = c.flags;
-    a.ixc = c.ixc;
-    a.nxc = c.nxc;
-    a.ndx = c.ndx;
-    a.ndy = c.ndy;
     hipLaunchKernelGGL(c2g_nodes, dim3((n + 255) / 256), dim3(256), 0, c.s, a);
   }
   const double qjump = c.hsc->qjump;  // fixed between recontourings

@@ -132,8 +132,6 @@ struct Ctx {
   double *mq = nullptr;        // (2ng)^2 mid grid
   int32_t *qjx_cnt = nullptr;  // nxf packed bottom-row counts
   double *qbot = nullptr;      // nxf
-  int32_t *ixc = nullptr, *nxc = nullptr;
-  double *ndx = nullptr, *ndy = nullptr;
   int32_t *flags = nullptr;    // small device counters; [8] sticky overflow
   // order-dependent cells: key -> bucket table, bucket list, entries, sums
   int32_t *rp_tkey = nullptr, *rp_tid = nullptr, *rp_list = nullptr;
