@@ -780,13 +780,19 @@ __global__ void __launch_bounds__(64 * kAvgW)
   const int32_t *cc = cnt + (int64_t)col * n2;
   int32_t run = qb[col];
   const int tid = threadIdx.x, oc = tid >> 7, jr = tid & 127;
+  // counts of the next stage load while the current one is scanned
+  int32_t wv[TPS], wn[TPS];
+#pragma unroll
+  for (int u = 0; u < TPS; u++) {
+    const int r = 64 * u + l;
+    wn[u] = (u < ntiles && r >= 1) ? cc[r] : 0;
+  }
   for (int st = 0; st * TPS < ntiles; st++) {
     const int t0 = st * TPS, b = st & 1;
-    int32_t wv[TPS];
 #pragma unroll
     for (int u = 0; u < TPS; u++) {
-      const int r = 64 * (t0 + u) + l;
-      wv[u] = (t0 + u < ntiles && r >= 1) ? level_jump(cc[r]) : 0;
+      wv[u] = level_jump(wn[u]);
+      wn[u] = t0 + TPS + u < ntiles ? cc[64 * (t0 + TPS + u) + l] : 0;
     }
     if (l == 0) tile[b][w][0] = (float)((double)run * qjump);  // row 64 t0 - 1 (unused for st = 0)
 #pragma unroll

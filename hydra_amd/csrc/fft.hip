@@ -540,7 +540,8 @@ static int fft_launch_t(Ctx &c, bool fwd, int F, const TI *const *in, TO *const 
   // batch per workgroup: the autotuned value of this layout class when it
   // divides m, else min(4096/n, kMaxV)
   const int cls = fwd ? (out_rows ? 0 : 1) : (in_rows ? 3 : 2);
-  int V = n == c.c.nx ? c.fft_v[cls] : 0;  // tuned for the context's length
+  // tuned for the context's length, per arithmetic type
+  int V = n == c.c.nx ? (sizeof(T) == 4 ? c.fft_v32[cls] : c.fft_v[cls]) : 0;
   if (V < 1 || V > kMaxV || V > m || m % V) {
     // the largest power of two <= 4096/n and kMaxV that divides m
     V = std::max(1, std::min(4096 / n, kMaxV));
@@ -630,6 +631,36 @@ int fft_autotune(Ctx &c) {
       }
     }
     c.fft_v[cls] = bestV;
+  }
+  // the FP32 passes of the mixed step (half the LDS per vector)
+  for (int cls = 0; cls < 4 && rc == HYD_OK && c.p.precision == 1; cls++) {
+    float best = 1e30f;
+    int bestV = 0;
+    for (int V = 1; V <= kMaxV && V <= n; V *= 2) {
+      if (fft_lds_bytes(n, V, sizeof(float)) > 160 * 1024) break;
+      c.fft_v32[cls] = V;
+      const float *trig = fw[cls] ? c.xtrig32 : c.ytrig32;
+      auto run = [&]() {
+        return fft_launch1<float, float, float>(c, fw[cls], c.fa32, c.fb32, n, n, inr[cls],
+                                                outr[cls], 0, nullptr, 1.0, trig);
+      };
+      rc = run();
+      if (rc != HYD_OK) break;
+      if (hipEventRecord(e0, c.s) != hipSuccess) { rc = HYD_EHIP; break; }
+      for (int r = 0; r < 5 && rc == HYD_OK; r++) rc = run();
+      if (rc != HYD_OK) break;
+      float ms = 0.0f;
+      if (hipEventRecord(e1, c.s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = HYD_EHIP;
+        break;
+      }
+      if (ms < best) {
+        best = ms;
+        bestV = V;
+      }
+    }
+    c.fft_v32[cls] = bestV;
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);

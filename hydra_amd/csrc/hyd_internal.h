@@ -107,7 +107,8 @@ struct Ctx {
   Consts c;
   HostTables ht;
   FftPlan plan;
-  int fft_v[4] = {0, 0, 0, 0};  // autotuned FFT batch per layout class
+  int fft_v[4] = {0, 0, 0, 0};    // autotuned FFT batch per layout class (FP64 passes)
+  int fft_v32[4] = {0, 0, 0, 0};  // the same for the FP32 passes of the mixed step
   hipStream_t s = nullptr;
   int dev = 0;
   int64_t N = 0;        // ng*ng
