@@ -376,7 +376,68 @@ __global__ void sg_store(const double *xr, const double *yr, const int64_t *src,
   }
 }
 
+// ---- device-side bookkeeping of the rebuild (integer scans replace the
+// host loops over all nodes)
+// level of every node: levels are contiguous ascending node ranges
+__global__ void sg_level_of_node(const int32_t *ib1, int nlev, int n, int32_t *lon) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int node = j + 1;
+  int lo = 0, hi = nlev - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (ib1[mid] <= node) lo = mid;
+    else hi = mid - 1;
+  }
+  lon[j] = lo;
+}
+__global__ void sg_widen(const int32_t *a, int n, int64_t *b) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) b[j] = a[j];
+}
+__global__ void sg_start_flags(const int32_t *head, int n, int32_t *flag) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) flag[j] = head[j] == j + 1 ? 1 : 0;
+}
+// contour c (numbered by start node, which is the reference's order: by
+// level, then by start node) starts at node j: its length and level
+__global__ void sg_contours(const int32_t *head, const int32_t *dist, const int32_t *incl,
+                            const int32_t *lon, const int32_t *levt, int n, int32_t *cid,
+                            int32_t *clen, int32_t *clev) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || head[j] != j + 1) return;
+  const int c = incl[j] - 1;
+  cid[j] = c;
+  clen[c] = dist[j] + 1;
+  clev[c] = levt[lon[j]];
+}
+__global__ void sg_renode_flags(const int32_t *clen, const int32_t *ncont, int32_t *flag, int cap) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < cap) flag[c] = (c < *ncont && clen[c] > 3) ? 1 : 0;
+}
+// the contours renode gets (more than three nodes, contours.f90:645-647)
+__global__ void sg_renode_list(const int32_t *flag, const int32_t *incl, const int64_t *coff,
+                               const int32_t *clen, const int32_t *clev, int cap, int64_t *roff,
+                               int32_t *rlen, int32_t *rlev) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cap || !flag[c]) return;
+  const int r = incl[c] - 1;
+  roff[r] = coff[c];
+  rlen[r] = clen[c];
+  rlev[r] = clev[c];
+}
+
 static inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+template <class T>
+static int scan_incl(Ctx &c, const T *in, T *out, int n, const char *name) {
+  size_t tb = 0;
+  HYD_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, in, out, n, c.s));
+  char *tmp;
+  HYD_CHECK(dbuf(c, name, (int64_t)std::max<size_t>(tb, 1), &tmp));
+  HYD_HIP(hipcub::DeviceScan::InclusiveSum(tmp, tb, in, out, n, c.s));
+  return HYD_OK;
+}
 
 int surgery(Ctx &c) {
   if (c.nq == 0) return HYD_OK;
@@ -433,9 +494,6 @@ int surgery(Ctx &c) {
   }
   const int nlev = (int)ib1.size();
   const int64_t nbox = bbase[nlev];
-  std::vector<int32_t> lev_of_node(n);
-  for (int l = 0; l < nlev; l++)
-    for (int i = ib1[l]; i <= ib2[l]; i++) lev_of_node[i - 1] = l;
   // device copies
   int32_t *d_ib1, *d_ib2, *d_nbx, *d_nby, *d_lon;
   double *d_bwx, *d_bwy;
@@ -455,7 +513,10 @@ int surgery(Ctx &c) {
   HYD_HIP(hipMemcpyAsync(d_bwx, bwx.data(), 8 * nlev, hipMemcpyHostToDevice, c.s));
   HYD_HIP(hipMemcpyAsync(d_bwy, bwy.data(), 8 * nlev, hipMemcpyHostToDevice, c.s));
   HYD_HIP(hipMemcpyAsync(d_bb, bbase.data(), 8 * (nlev + 1), hipMemcpyHostToDevice, c.s));
-  HYD_HIP(hipMemcpyAsync(d_lon, lev_of_node.data(), 4 * (size_t)n, hipMemcpyHostToDevice, c.s));
+  int32_t *d_levt;
+  HYD_CHECK(dbuf(c, "sg_levt", nlev, &d_levt));
+  HYD_HIP(hipMemcpyAsync(d_levt, levt.data(), 4 * nlev, hipMemcpyHostToDevice, c.s));
+  hipLaunchKernelGGL(sg_level_of_node, g1(n), dim3(256), 0, c.s, d_ib1, nlev, n, d_lon);
   SgLevels L;
   L.ibeg = d_ib1;
   L.iend = d_ib2;
@@ -484,10 +545,16 @@ int surgery(Ctx &c) {
   hipLaunchKernelGGL(sg_box_range, g1(n), dim3(256), 0, c.s, c.xq, c.yq, c.nextq, dx, dy, n, L, k,
                      rng, cnt, errs);
   HYD_HIP(hipGetLastError());
-  std::vector<int32_t> hcnt(n);
+  // off[0] = 0, off[i + 1] = off[i] + cnt[i]
+  int64_t *cnt64;
+  HYD_CHECK(dbuf(c, "sg_cnt64", n, &cnt64));
+  hipLaunchKernelGGL(sg_widen, g1(n), dim3(256), 0, c.s, cnt, n, cnt64);
+  HYD_HIP(hipMemsetAsync(off, 0, 8, c.s));
+  HYD_CHECK(scan_incl<int64_t>(c, cnt64, off + 1, n, "sg_scan1"));
   int32_t herr[4];
-  HYD_HIP(hipMemcpyAsync(hcnt.data(), cnt, 4 * (size_t)n, hipMemcpyDeviceToHost, c.s));
+  int64_t nk = 0;
   HYD_HIP(hipMemcpyAsync(herr, errs, 16, hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipMemcpyAsync(&nk, off + n, 8, hipMemcpyDeviceToHost, c.s));
   HYD_HIP(hipStreamSynchronize(c.s));
   if (herr[2]) {
     set_error("surgery: contour nodes outside the periodic domain");
@@ -497,11 +564,6 @@ int surgery(Ctx &c) {
     set_error("surgery: a segment spans an invalid box range");
     return HYD_EARG;
   }
-  std::vector<int64_t> hoff(n + 1);
-  hoff[0] = 0;
-  for (int i = 0; i < n; i++) hoff[i + 1] = hoff[i] + hcnt[i];
-  const int64_t nk = hoff[n];
-  HYD_HIP(hipMemcpyAsync(off, hoff.data(), 8 * (size_t)(n + 1), hipMemcpyHostToDevice, c.s));
   uint64_t *keys, *keys2;
   HYD_CHECK(dbuf(c, "sg_keys", nk, &keys));
   HYD_CHECK(dbuf(c, "sg_keys2", nk, &keys2));
@@ -588,58 +650,52 @@ int surgery(Ctx &c) {
     std::swap(q1, q2);
   }
   HYD_HIP(hipGetLastError());
-  // contours in the reference's order: by level, then by start node
-  std::vector<int32_t> hm(n), hd(n);
-  HYD_HIP(hipMemcpyAsync(hm.data(), m1, 4 * (size_t)n, hipMemcpyDeviceToHost, c.s));
-  HYD_HIP(hipMemcpyAsync(hd.data(), d1, 4 * (size_t)n, hipMemcpyDeviceToHost, c.s));
-  HYD_HIP(hipStreamSynchronize(c.s));
-  std::vector<int32_t> cid(n, -1), clen, clev;
-  std::vector<int64_t> coff;
-  int64_t tot = 0;
-  for (int l = 0; l < nlev; l++)
-    for (int j = ib1[l]; j <= ib2[l]; j++)
-      if (hm[j - 1] == j) {
-        cid[j - 1] = (int32_t)clen.size();
-        const int len = hd[j - 1] + 1;
-        clen.push_back(len);
-        clev.push_back(levt[l]);
-        coff.push_back(tot);
-        tot += len;
-      }
-  const int ncont = (int)clen.size();
-  int32_t *d_cid, *d_clen;
-  int64_t *d_coff;
+  // contours in the reference's order: by level, then by start node (the
+  // node order, levels being ascending node ranges); numbered by a scan of
+  // the start flags
+  int32_t *flag, *incl, *d_cid, *d_clen, *d_clev, *d_ncont;
+  int64_t *d_coff, *clen64;
   double *xd, *yd;
+  HYD_CHECK(dbuf(c, "sg_flag", n, &flag));
+  HYD_CHECK(dbuf(c, "sg_incl", n, &incl));
   HYD_CHECK(dbuf(c, "sg_cid", n, &d_cid));
-  HYD_CHECK(dbuf(c, "sg_clen", ncont, &d_clen));
-  HYD_CHECK(dbuf(c, "sg_coff", ncont, &d_coff));
+  HYD_CHECK(dbuf(c, "sg_clen", n, &d_clen));
+  HYD_CHECK(dbuf(c, "sg_clev", n, &d_clev));
+  HYD_CHECK(dbuf(c, "sg_coff", n, &d_coff));
+  HYD_CHECK(dbuf(c, "sg_clen64", n, &clen64));
   HYD_CHECK(dbuf(c, "sg_xd", n, &xd));
   HYD_CHECK(dbuf(c, "sg_yd", n, &yd));
-  HYD_HIP(hipMemcpyAsync(d_cid, cid.data(), 4 * (size_t)n, hipMemcpyHostToDevice, c.s));
-  HYD_HIP(hipMemcpyAsync(d_clen, clen.data(), 4 * (size_t)ncont, hipMemcpyHostToDevice, c.s));
-  HYD_HIP(hipMemcpyAsync(d_coff, coff.data(), 8 * (size_t)ncont, hipMemcpyHostToDevice, c.s));
+  hipLaunchKernelGGL(sg_start_flags, g1(n), dim3(256), 0, c.s, m1, n, flag);
+  HYD_CHECK(scan_incl<int32_t>(c, flag, incl, n, "sg_scan2"));
+  d_ncont = incl + (n - 1);  // contours in all
+  hipLaunchKernelGGL(sg_contours, g1(n), dim3(256), 0, c.s, m1, d1, incl, d_lon, d_levt, n, d_cid,
+                     d_clen, d_clev);
+  // coff[c] = sum of the earlier contours' lengths (unused slots hold 0)
+  hipLaunchKernelGGL(sg_fill64, g1(n), dim3(256), 0, c.s, clen64, (int64_t)n, (int64_t)0);
+  hipLaunchKernelGGL(sg_widen, g1(n), dim3(256), 0, c.s, d_clen, n, clen64);
+  HYD_HIP(hipMemsetAsync(d_coff, 0, 8, c.s));
+  if (n > 1) HYD_CHECK(scan_incl<int64_t>(c, clen64, d_coff + 1, n - 1, "sg_scan3"));
   hipLaunchKernelGGL(sg_scatter, g1(n), dim3(256), 0, c.s, c.xq, c.yq, m1, d1, d_cid, d_coff,
                      d_clen, n, xd, yd);
   HYD_HIP(hipGetLastError());
   // renode the contours of more than three nodes (contours.f90:645-647)
-  std::vector<int> rn;
-  for (int j = 0; j < ncont; j++)
-    if (clen[j] > 3) rn.push_back(j);
-  const int nr = (int)rn.size();
-  std::vector<int64_t> roff(nr);
-  std::vector<int32_t> rlen(nr);
-  for (int j = 0; j < nr; j++) {
-    roff[j] = coff[rn[j]];
-    rlen[j] = clen[rn[j]];
-  }
+  int32_t *flag2, *incl2, *d_rlen, *d_npr, *d_rlev;
   int64_t *d_roff, *d_ooff;
-  int32_t *d_rlen, *d_npr;
-  HYD_CHECK(dbuf(c, "sg_roff", std::max(nr, 1), &d_roff));
-  HYD_CHECK(dbuf(c, "sg_ooff", std::max(nr, 1), &d_ooff));
-  HYD_CHECK(dbuf(c, "sg_rlen", std::max(nr, 1), &d_rlen));
-  HYD_CHECK(dbuf(c, "sg_npr", std::max(nr, 1), &d_npr));
-  HYD_HIP(hipMemcpyAsync(d_roff, roff.data(), 8 * (size_t)nr, hipMemcpyHostToDevice, c.s));
-  HYD_HIP(hipMemcpyAsync(d_rlen, rlen.data(), 4 * (size_t)nr, hipMemcpyHostToDevice, c.s));
+  HYD_CHECK(dbuf(c, "sg_flag2", n, &flag2));
+  HYD_CHECK(dbuf(c, "sg_incl2", n, &incl2));
+  HYD_CHECK(dbuf(c, "sg_roff", n, &d_roff));
+  HYD_CHECK(dbuf(c, "sg_ooff", n, &d_ooff));
+  HYD_CHECK(dbuf(c, "sg_rlen", n, &d_rlen));
+  HYD_CHECK(dbuf(c, "sg_rlev", n, &d_rlev));
+  HYD_CHECK(dbuf(c, "sg_npr", n, &d_npr));
+  hipLaunchKernelGGL(sg_renode_flags, g1(n), dim3(256), 0, c.s, d_clen, d_ncont, flag2, n);
+  HYD_CHECK(scan_incl<int32_t>(c, flag2, incl2, n, "sg_scan4"));
+  hipLaunchKernelGGL(sg_renode_list, g1(n), dim3(256), 0, c.s, flag2, incl2, d_coff, d_clen,
+                     d_clev, n, d_roff, d_rlen, d_rlev);
+  HYD_HIP(hipGetLastError());
+  int32_t nr = 0;
+  HYD_HIP(hipMemcpyAsync(&nr, incl2 + (n - 1), 4, hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipStreamSynchronize(c.s));
   double *xr, *yr;
   int64_t capo = std::max<int64_t>(2 * (int64_t)n + 1024, 4096);
   HYD_CHECK(dbuf(c, "sg_xr", capo, &xr));
@@ -653,9 +709,10 @@ int surgery(Ctx &c) {
   }
   HYD_CHECK(rc);
   std::vector<int64_t> ooff(nr);
-  std::vector<int32_t> npr(nr);
+  std::vector<int32_t> npr(nr), rlev(nr);
   HYD_HIP(hipMemcpyAsync(ooff.data(), d_ooff, 8 * (size_t)nr, hipMemcpyDeviceToHost, c.s));
   HYD_HIP(hipMemcpyAsync(npr.data(), d_npr, 4 * (size_t)nr, hipMemcpyDeviceToHost, c.s));
+  HYD_HIP(hipMemcpyAsync(rlev.data(), d_rlev, 4 * (size_t)nr, hipMemcpyDeviceToHost, c.s));
   HYD_HIP(hipStreamSynchronize(c.s));
   // keep contours with more than three nodes after renoding (648-660)
   std::vector<int32_t> i1, i2, np, ind;
@@ -667,7 +724,7 @@ int surgery(Ctx &c) {
       npt += npr[j];
       i2.push_back((int32_t)npt);
       np.push_back(npr[j]);
-      ind.push_back(clev[rn[j]]);
+      ind.push_back(rlev[j]);
       src.push_back(ooff[j]);
     }
   }

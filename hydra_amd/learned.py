@@ -94,6 +94,23 @@ def collect(flow, judge, nsnap, steps_between, sc, max_nodes=None, seed=0):
     return out
 
 
+def rescale(snaps, sc, extra=()):
+    """Re-fix the output scale sd as the rms target over the training
+    snapshots (snapshots of evolved flows carry larger errors than the first
+    state the scale came from); every snapshot's targets, and those of the
+    extra (held-out) sets, are rescaled to match.  Returns the new Scales."""
+    ss = sum(float(np.sum(s.tu.astype(np.float64) ** 2 + s.tv.astype(np.float64) ** 2))
+             for s in snaps)
+    cnt = sum(len(s.tu) for s in snaps)
+    r = float(np.sqrt(ss / max(cnt, 1)))
+    if not (r > 0.0):
+        return sc
+    for s in list(snaps) + [t for group in extra for t in group]:
+        s.tu = (s.tu / r).astype(np.float32)
+        s.tv = (s.tv / r).astype(np.float32)
+    return Scales(sc.su, sc.sq, sc.sd * r)
+
+
 def train(net, snaps, iters, lr=2e-3, lr_final=5e-4, decay_at=0.67):
     """Adam on the snapshots in turn; returns the loss history."""
     hist = []
@@ -141,6 +158,8 @@ def main(argv=None):
     ap.add_argument("--every", type=int, default=8)
     ap.add_argument("--nodes", type=int, default=60000)
     ap.add_argument("--iters", type=int, default=360)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr-final", type=float, default=5e-4)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--save", default=None, help="write the trained weights (.npy)")
@@ -166,11 +185,13 @@ def main(argv=None):
         if fk is not f:
             fk.close()
     test_set = collect(flow(a.seed + 1), judge, 2, a.every, sc, a.nodes, seed=2)
+    sd_first = sc.sd
+    sc = rescale(train_set, sc, extra=(test_set,))
     t1 = time.perf_counter()
     net = new_network(f)
-    hist = train(net, train_set, a.iters)
+    hist = train(net, train_set, a.iters, lr=a.lr, lr_final=a.lr_final)
     t2 = time.perf_counter()
-    out = dict(ng=a.ng, judge_ng=judge.ng, su=sc.su, sq=sc.sq, sd=sc.sd,
+    out = dict(ng=a.ng, judge_ng=judge.ng, su=sc.su, sq=sc.sq, sd=sc.sd, sd_first_state=sd_first,
                snapshots=len(train_set), flows=a.flows, snapshot_s=t1 - t0,
                train_s=t2 - t1, iters=a.iters, loss_first=hist[0], loss_last=hist[-1],
                train_left=residual_ratio(net, train_set[:2]),
