@@ -1133,6 +1133,8 @@ int hyd_set_scalar(hyd_ctx *h, int32_t id, double val) {
 // regularisations counted yet.
 static int advect_start(Ctx &c) {
   HYD_CHECK(evolution_init(c));
+  // surgery's work buffers, allocated here rather than inside the loop
+  if (c.nptq > 0) HYD_CHECK(surgery_reserve(c, c.nptq));
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
   HYD_HIP(hipStreamSynchronize(c.s));
   c.hsc->twist = 0.0;

@@ -293,10 +293,13 @@ template <class T>
 int dbuf(Ctx &c, const char *name, int64_t n, T **out) {
   DBuf &b = c.cg_bufs[name];
   if (b.cap < n * (int64_t)sizeof(T) || !b.p) {
+    // a buffer that has to grow (node counts rise between surgeries) grows
+    // by half again at least, so that reallocation stays rare
+    const int64_t grown = b.p ? b.cap + b.cap / 2 : 0;
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
-    const int64_t bytes = std::max<int64_t>(n * (int64_t)sizeof(T), 256);
+    const int64_t bytes = std::max<int64_t>(std::max<int64_t>(n * (int64_t)sizeof(T), grown), 256);
     if (hipMalloc(&b.p, bytes) != hipSuccess) {
       set_error(std::string("out of device memory for work buffer ") + name);
       return HYD_ENOMEM;
@@ -309,6 +312,7 @@ int dbuf(Ctx &c, const char *name, int64_t n, T **out) {
 
 int recontour(Ctx &c, const double *qq);
 int surgery(Ctx &c);
+int surgery_reserve(Ctx &c, int64_t nodes);
 int prepare(Ctx &c);
 int evolution_init(Ctx &c);
 int advance(Ctx &c);

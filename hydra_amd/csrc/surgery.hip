@@ -439,6 +439,33 @@ static int scan_incl(Ctx &c, const T *in, T *out, int n, const char *name) {
   return HYD_OK;
 }
 
+// The node-sized work buffers of surgery, allocated ahead of the loop for
+// half as many nodes again as the store holds (bytes per node: element
+// size times count per node), so that a surgery inside the time loop
+// allocates nothing.
+int surgery_reserve(Ctx &c, int64_t nodes) {
+  struct Buf {
+    const char *name;
+    int bytes;  // per node
+  };
+  static const Buf bufs[] = {
+      {"sg_lon", 4},    {"sg_dx", 8},     {"sg_dy", 8},    {"sg_dsq", 8},    {"sg_rng", 16},
+      {"sg_cnt", 4},    {"sg_off", 8},    {"sg_cnt64", 8}, {"sg_keys", 24},  {"sg_keys2", 24},
+      {"sg_node", 12},  {"sg_hit", 1},    {"sg_nd", 1},    {"sg_pd", 1},     {"sg_prevarr", 4},
+      {"sg_m1", 4},     {"sg_p1", 4},     {"sg_m2", 4},    {"sg_p2", 4},     {"sg_d1", 4},
+      {"sg_q1", 4},     {"sg_d2", 4},     {"sg_q2", 4},    {"sg_flag", 4},   {"sg_incl", 4},
+      {"sg_cid", 4},    {"sg_clen", 4},   {"sg_clev", 4},  {"sg_coff", 8},   {"sg_clen64", 8},
+      {"sg_xd", 8},     {"sg_yd", 8},     {"sg_flag2", 4}, {"sg_incl2", 4},  {"sg_roff", 8},
+      {"sg_ooff", 8},   {"sg_rlen", 4},   {"sg_rlev", 4},  {"sg_npr", 4},    {"sg_xr", 16},
+      {"sg_yr", 16}};
+  const int64_t n = nodes + nodes / 2 + 1024;
+  for (const Buf &b : bufs) {
+    char *p;
+    HYD_CHECK(dbuf(c, b.name, n * b.bytes, &p));
+  }
+  return HYD_OK;
+}
+
 int surgery(Ctx &c) {
   if (c.nq == 0) return HYD_OK;
   const Consts &K = c.c;
