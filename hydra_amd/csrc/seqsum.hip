@@ -694,7 +694,7 @@ __device__ __forceinline__ double ordered_steps(const double *__restrict__ a, in
 
 __global__ void __launch_bounds__(64) seq_classes(const double *__restrict__ a,
                                                   const SegRec *__restrict__ sg, int cap,
-                                                  const CompRec *__restrict__ cp,
+                                                  CompRec *__restrict__ cp, int64_t nw,
                                                   int32_t *__restrict__ tab_m,
                                                   ClassTab *__restrict__ tab) {
   const int64_t w = blockIdx.x;
@@ -718,9 +718,9 @@ __global__ void __launch_bounds__(64) seq_classes(const double *__restrict__ a,
     R.meta = 0, R.g = 0.0, R.d0 = R.d1 = R.lo = R.hi = R.xh = 0.0, R.chunk = 0;
   }
   const double g = readlane_d(R.g, 0);
-  const int meta0 = __builtin_amdgcn_readlane(R.meta, 0);
   const int e0 = binade(g);
-  bool ok = tot > 0 && !((meta0 >> 19) & 1) && e0 >= 64 && e0 <= 2045;
+  // (a replay record carries its chunk's guess, so a table can start there)
+  bool ok = tot > 0 && e0 >= 64 && e0 <= 2045;
   const double u = ok ? pow2_field(e0) * 0x1p-52 : 0.0;
   const int c = l < kClasses ? l : 0;
   double S = g + (double)c * u;  // exact: c < 2^5 ulps
@@ -777,7 +777,49 @@ __global__ void __launch_bounds__(64) seq_classes(const double *__restrict__ a,
     tab[w].lo[l] = dlo;
     tab[w].hi[l] = dhi;
   }
-  if (l == 0) tab_m[w] = (M > 0 && all_ok && __ballot(ok) != 0) ? M : 0;
+  const bool valid = M > 0 && all_ok && __ballot(ok) != 0;
+  if (l == 0) tab_m[w] = valid ? M : 0;
+  // Two classes whose runs end in g's binade: the window is a composite of
+  // the ordinary kind (start parity = class, translations E_c - G_c exact
+  // in one binade, the class intervals in units of u), so the chain's scan
+  // takes it without a break.
+  if (valid && M == 2) {
+    const double E0 = readlane_d(S, 0), E1 = readlane_d(S, 1);
+    const double lo0 = readlane_d(dlo, 0), hi0 = readlane_d(dhi, 0);
+    const double lo1 = readlane_d(dlo, 1), hi1 = readlane_d(dhi, 1);
+    const int key = segkey(g);
+    if (segkey(E0) == key && segkey(E1) == key && l == 0) {
+      const double ui = pow2_field(2098 - e0);  // 1/u of g's binade
+      const double g1 = g + u;
+      int pn0 = 0, pn1 = 0;
+      if (w + 1 < nw) {
+        const double gn = sg[(w + 1) * kWin * cap].g;
+        if (segkey(gn) == key) {
+          pn0 = parity_of(E0 - gn, ui);
+          pn1 = parity_of(E1 - gn, ui);
+        }
+      }
+      CompRec D = C;
+      D.g = g;
+      D.d0 = E0 - g;
+      D.d1 = E1 - g1;
+      // delta = sign(g) (S - g) / u is an integer: the class intervals,
+      // tightened to integers
+      if (g > 0.0) {
+        D.lo0 = ceil(lo0 * ui);
+        D.hi0 = floor(hi0 * ui);
+        D.lo1 = ceil(1.0 + lo1 * ui);
+        D.hi1 = floor(1.0 + hi1 * ui);
+      } else {
+        D.lo0 = ceil(-(hi0 * ui));
+        D.hi0 = floor(-(lo0 * ui));
+        D.lo1 = ceil(-(1.0 + hi1 * ui));
+        D.hi1 = floor(-(1.0 + lo1 * ui));
+      }
+      D.meta = nm | (1 << 5) | ((((pn0 ^ pn1) << 1) | pn0) << 6);
+      cp[w] = D;
+    }
+  }
 }
 
 // The window's run from S by its class table; false when the table does
@@ -974,8 +1016,8 @@ static int seqsum_launch(const double *a, int64_t n, double *dout, char *w, cons
                      segs, cnt, cps, L.cap);
   hipLaunchKernelGGL(seq_chunks<1>, dim3((unsigned)L.nb), dim3(64), 0, s, a, n, L.nch, tot, blk,
                      segs, cnt, cps, L.cap);
-  hipLaunchKernelGGL(seq_classes, dim3((unsigned)L.nw), dim3(64), 0, s, a, segs, L.cap, cps, tabm,
-                     tab);
+  hipLaunchKernelGGL(seq_classes, dim3((unsigned)L.nw), dim3(64), 0, s, a, segs, L.cap, cps, L.nw,
+                     tabm, tab);
   hipLaunchKernelGGL(seq_chain, dim3(1), dim3(256), 0, s, a, segs, L.cap, cps, L.nw, tabm, tab,
                      dout);
   HYD_HIP(hipGetLastError());
