@@ -123,7 +123,15 @@ def pmc_summary(fetch_dir, write_dir, ng, out_json):
     cw, wr = summarise(write_dir, "WRITE_SIZE")
     kib = COPY_BYTES / 1024.0
     ff, fw = kib / cf, kib / cw
-    out = {"calibration": {"copy_bytes_each_way": COPY_BYTES, "fetch_kib": cf, "write_kib": cw,
+    run = ("python3 $REPO/bench.py --steps 3 --warmup 1 --no-cpu --no-mixed --ref-dm-steps 0 "
+           "--copy-calib")
+    out = {"commands": ["cd /tmp && export TMPDIR=/tmp",
+                        "timeout -s KILL 300 rocprofv3 --pmc FETCH_SIZE --output-format csv "
+                        "--kernel-include-regex 'fft_kernel|k_copy_calib' -d FETCH_DIR -- " + run,
+                        "timeout -s KILL 300 rocprofv3 --pmc WRITE_SIZE --output-format csv "
+                        "--kernel-include-regex 'fft_kernel|k_copy_calib' -d WRITE_DIR -- " + run,
+                        "python3 $REPO/bench.py --pmc-summary FETCH_DIR WRITE_DIR"],
+           "calibration": {"copy_bytes_each_way": COPY_BYTES, "fetch_kib": cf, "write_kib": cw,
                            "fetch_factor": ff, "write_factor": fw}}
     for k in sorted(fe):
         if k not in wr:
