@@ -92,6 +92,33 @@ def test_attached_step_moves_nodes_by_the_correction(trained):
     assert np.array_equal(d.contours()["y"], cb["y"])
 
 
+def test_correction_in_the_mixed_step(trained):
+    """The network attached to a mixed-precision flow (FP32 grids, captured
+    step): the step runs, the nodes stay finite and move by about dt times
+    the correction against the same mixed step without it."""
+    from hydra_amd import Hydra
+    from hydra_amd.ic import random_pv
+    net, sc = trained["net"], trained["sc"]
+
+    def mixed():
+        h = Hydra(NG, node_capacity=1 << 22, precision=1)
+        h.set_field(random_pv(NG, seed=21))
+        h.recontour()
+        h.evolution_init()
+        return h
+
+    a, b = mixed(), mixed()
+    a.set_correction(net, sc.su, sc.sq, sc.sd)
+    for _ in range(3):
+        a.advance()
+        b.advance()
+    ca, cb = a.contours(), b.contours()
+    assert len(ca["x"]) == len(cb["x"])
+    d = np.hypot(ca["x"] - cb["x"], ca["y"] - cb["y"])
+    assert np.all(np.isfinite(d))
+    assert 0.0 < np.sqrt(np.mean(d ** 2)) < 1e-3
+
+
 def test_correction_rejects_other_grid():
     from hydra_amd import HydraError, VelocityCNN
     h = _flow(31)
