@@ -186,11 +186,14 @@ def make_flow(ng, qq, precision=0, ell_cells=0.0, ncontq=80):
 def advect_steps(h, k):
     """k passes of the advect loop (evolution.f90:55-80): surgery when the
     twist exceeds 2.5, recontouring after 20 regularisations (caps.f90:
-    83-91), then one RK4 advance.  Returns the node-updates."""
-    updates = 0
-    for _ in range(k):
-        h.run(1)
-        updates += h.counts()[0]
+    83-91), then one RK4 advance; one library call, as the reference's main
+    loop is one loop.  Returns the node-updates (the library counts each
+    advance's nodes)."""
+    h.advance_counts(reset=True)
+    h.run(k)
+    adv, updates = h.advance_counts(reset=True)
+    if adv != k:
+        raise RuntimeError(f"advect loop took {adv} advances, expected {k}")
     return updates
 
 
@@ -474,16 +477,17 @@ def ensemble_report(nflows, steps, seed):
         run(hs, 2)  # warm-up: first contours, graph capture
         for h in hs:
             h.sync()
-        p0 = [h.counts()[0] for h in hs]
+            h.advance_counts(reset=True)
         t0 = time.perf_counter()
         run(hs, steps)
         for h in hs:
             h.sync()
         wall = time.perf_counter() - t0
         p1 = [h.counts()[0] for h in hs]
+        updates = sum(h.advance_counts()[1] for h in hs)
         for h in hs:
             h.close()
-        return sum(steps * (a + b) / 2 for a, b in zip(p0, p1)) / wall, wall, p1
+        return updates / wall, wall, p1
 
     def one_by_one(hs, k):
         for h in hs:

@@ -102,6 +102,8 @@ def lib():
                                  C.POINTER(C.c_int32)]
         L.hyd_run.argtypes = [C.c_void_p, C.c_int32]
         L.hyd_phase_times.argtypes = [C.c_void_p, _D, C.c_int32]
+        L.hyd_advance_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_int64), C.c_int32]
         L.hyd_take_records.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _D]
         L.hyd_set_saves.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_int32]
         L.hyd_pending_saves.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -452,6 +454,12 @@ class Hydra:
         out = np.zeros(4)
         _check(self.L.hyd_phase_times(self.h, out, int(bool(reset))))
         return {"surgery": (int(out[0]), float(out[1])), "recontour": (int(out[2]), float(out[3]))}
+
+    def advance_counts(self, reset=False):
+        """(RK4 advances, node-updates) taken so far; reset zeroes both."""
+        a, u = C.c_int64(), C.c_int64()
+        _check(self.L.hyd_advance_counts(self.h, C.byref(a), C.byref(u), int(bool(reset))))
+        return a.value, u.value
 
     def inversion(self):
         _check(self.L.hyd_inversion(self.h))

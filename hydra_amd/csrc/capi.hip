@@ -533,6 +533,14 @@ int hyd_phase_times(hyd_ctx *h, double *out, int32_t reset) {
   return HYD_OK;
 }
 
+int hyd_advance_counts(hyd_ctx *h, int64_t *advances, int64_t *node_updates, int32_t reset) {
+  Ctx &c = *C(h);
+  if (advances) *advances = c.n_advances;
+  if (node_updates) *node_updates = c.n_node_updates;
+  if (reset) c.n_advances = c.n_node_updates = 0;
+  return HYD_OK;
+}
+
 int hyd_init(const hyd_params *p, hyd_ctx **ctx) { return create(p, ctx); }
 
 int hyd_destroy(hyd_ctx *ctx) {
@@ -1220,6 +1228,9 @@ int hyd_renode(hyd_ctx *h, const double *xd, const double *yd, int32_t npd, doub
 // is captured as a graph and replayed; the capture is redone whenever the
 // node count changed (surgery, recontouring) and skipped while timing.
 static int advance_mixed_graph(Ctx &c) {
+  c.sweep_clean = false;  // its sweeps leave their counts behind
+  c.n_advances += 1;
+  c.n_node_updates += c.nptq;
   if (c.timing || c.mixed_calls++ == 0) return advance_mixed(c);
   if (!c.step_graph || c.graph_nptq != c.nptq) {
     if (c.step_graph) {
@@ -1280,8 +1291,11 @@ int hyd_surgery(hyd_ctx *h) {
   return HYD_OK;
 }
 
-// advect loop body, evolution.f90:55-80
-static int advect_step(Ctx &c) {
+// advect loop body, evolution.f90:55-80.  lazy: the plain step leaves its
+// scalars on the device (no read-back and no wait at its end); the next
+// step's adapt brings them, and any sweep overflow, to the host, and the
+// caller reads them back once its loop ends.
+static int advect_step(Ctx &c, bool lazy = false) {
   const double twistmax = 2.5;
   const int nregmax = 20;
   if (c.hsc->twist > twistmax) {
@@ -1299,6 +1313,9 @@ static int advect_step(Ctx &c) {
     c.hsc->twist = c.hsc->twist - twistmax;
     HYD_HIP(hipMemcpyAsync(c.dsc, c.hsc, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
   }
+  // the save slots read the step's start time from the host mirror, so a
+  // run with saves keeps the read-back
+  if (lazy && c.p.precision == 0 && !c.saves_on && c.have_field) return advance(c);
   return hyd_advance((hyd_ctx *)&c);
 }
 
@@ -1336,14 +1353,14 @@ int hyd_run(hyd_ctx *h, int32_t nsteps) {
   while (done < nsteps) {
     HYD_CHECK(resume_cycle(c));
     while (done < nsteps) {
-      const int r = advect_step(c);
+      const int r = advect_step(c, true);
       if (r < 0) return r;
       if (r == 1) break;
       done++;
     }
   }
-  HYD_HIP(hipStreamSynchronize(c.s));
-  return HYD_OK;
+  HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+  return c2g_status(c);  // waits for the stream
 }
 
 // An ensemble of independent flows, each running hyd_run's loop for nsteps

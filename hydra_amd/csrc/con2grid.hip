@@ -283,13 +283,26 @@ __global__ void c2g_bucket_sum(Buckets b, const int32_t *flags, double qjump) {
   }
 }
 
-// Empty the table again (after the sweep has read the bucket sums).
-__global__ void c2g_bucket_clear(Buckets b, int32_t *flags) {
-  const int nb_used = min(flags[4], b.bcap);
-  for (int nb = blockIdx.x * blockDim.x + threadIdx.x; nb < nb_used;
-       nb += gridDim.x * blockDim.x) {
+// Empty the table again (after the sweep has read the bucket sums); the
+// bucket count is the copy c2g_qbot kept in flags[6].  With halo > 0 it
+// also zeroes the counter columns halo-1, 2 halo-1, ... (n2 rows each,
+// n2 % 4 == 0): the fused sweep reads each of them from two workgroups and
+// so leaves them to this kernel.
+__global__ void c2g_bucket_clear(Buckets b, int32_t *flags, int32_t *cnt, int n2, int halo) {
+  const int nb_used = min(flags[6], b.bcap);
+  const int stride = gridDim.x * blockDim.x;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int nb = tid; nb < nb_used; nb += stride) {
     b.tkey[b.list[4 * nb]] = -1;
     if (b.list[4 * nb + 3] != b.list[4 * nb + 2]) atomicOr(&flags[8], 2);
+  }
+  if (halo > 0) {
+    const int q = n2 / 4;  // int4 per column
+    const int64_t total = (int64_t)(n2 / halo) * q;
+    for (int64_t k = tid; k < total; k += stride) {
+      const int64_t col = (k / q + 1) * halo - 1;
+      reinterpret_cast<int4 *>(cnt + col * n2)[k % q] = make_int4(0, 0, 0, 0);
+    }
   }
 }
 
@@ -338,11 +351,14 @@ __device__ __forceinline__ int wave_scan_i32(int v) {
 // lane indices, so each step is one dependent add), recording the running
 // value after each; every counter then reads the value after the last
 // non-empty one at or before it.  The additions are the reference's, in its
-// order.  nxf <= kQbotMax.
+// order.  nxf <= kQbotMax.  Being the last reader of the counters and of
+// flags[0..5], it leaves them zero for the next sweep (the bucket count
+// moves to flags[6] for c2g_bucket_clear).
 static constexpr int kQbotMax = 32768;
 static constexpr int kQbotPart = 2048;  // counters per pass, two per thread
-__global__ void __launch_bounds__(1024) c2g_qbot(const int32_t *qjx, double *qbot, int nxf,
-                                                 int64_t nkey, double qjump, Buckets bk) {
+__global__ void __launch_bounds__(1024) c2g_qbot(int32_t *qjx, double *qbot, int nxf,
+                                                 int64_t nkey, double qjump, Buckets bk,
+                                                 int32_t *flags) {
   __shared__ double comp[kQbotPart];  // values of the non-empty counters, in order
   __shared__ double pref[kQbotPart];  // the chain after each of them
   __shared__ int wsum[16];
@@ -355,6 +371,8 @@ __global__ void __launch_bounds__(1024) c2g_qbot(const int32_t *qjx, double *qbo
     const int t0 = 2 * tid, t1 = t0 + 1;
     const int32_t w0 = t0 < m ? qjx[part + t0] : 0;
     const int32_t w1 = t1 < m ? qjx[part + t1] : 0;
+    if (w0) qjx[part + t0] = 0;
+    if (w1) qjx[part + t1] = 0;
     const double c0 = w0 ? cell_value(w0, nkey + part + t0, qjump, bk) : 0.0;
     const double c1 = w1 ? cell_value(w1, nkey + part + t1, qjump, bk) : 0.0;
     const int n0 = w0 != 0, n1 = w1 != 0, cnt = n0 + n1;
@@ -393,16 +411,24 @@ __global__ void __launch_bounds__(1024) c2g_qbot(const int32_t *qjx, double *qbo
     if (t1 < m) qbot[part + t1 + 1] = i1 > 0 ? pref[i1 - 1] : v0;
     __syncthreads();
   }
+  if (tid == 0) {
+    qjx[nxf - 1] = 0;  // the last counter is never part of the chain
+    flags[6] = flags[4];
+#pragma unroll
+    for (int k = 0; k < 6; k++) flags[k] = 0;
+    flags[7] = 0;
+  }
 }
 
-// y sweep (contours.f90:272-277): one wavefront per fine column.
-__global__ void c2g_colscan(const int32_t *cnt, const double *qbot, double *fq, int nxf, int nyf,
+// y sweep (contours.f90:272-277): one wavefront per fine column.  Each
+// non-empty counter is zeroed once read (the next sweep starts from zero).
+__global__ void c2g_colscan(int32_t *cnt, const double *qbot, double *fq, int nxf, int nyf,
                             double qjump, Buckets bk) {
   const int wpb = blockDim.x / 64;
   const int col = blockIdx.x * wpb + threadIdx.x / 64;
   const int l = threadIdx.x & 63;
   if (col >= nxf) return;
-  const int32_t *cc = cnt + (int64_t)col * nyf;
+  int32_t *cc = cnt + (int64_t)col * nyf;
   double *out = fq + (int64_t)col * nyf;
   double v = qbot[col];
   constexpr int U = 8;  // segments of 64 rows loaded ahead
@@ -416,6 +442,7 @@ __global__ void c2g_colscan(const int32_t *cnt, const double *qbot, double *fq, 
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int r = base + 64 * u + l;  // iy-1
+      if (w[u]) cc[r] = 0;
       const double cv =
           w[u] ? cell_value(w[u], (int64_t)col * nyf + r, qjump, bk) : 0.0;
       uint64_t mask = __ballot(w[u] != 0);
@@ -456,12 +483,16 @@ __global__ void c2g_avg9(const T *__restrict__ f, T *__restrict__ out, int nh) {
 // y sweep, 64 rows at a time) and averages them into output columns
 // i0 .. i0+K-1 through LDS, so the fine field never reaches HBM.  Output
 // row 0 needs the last fine row (periodic) and is written at the end.
-// Needs n2 % 64 == 0 and (n2 / 2) % K == 0.
+// Needs n2 % 64 == 0 and (n2 / 2) % K == 0.  Non-empty counters are zeroed
+// as they are read, while their lines are still in the cache (zeroing them
+// after the sweep costs partial-line writes to HBM), except in the first
+// and last column of the workgroup, which the neighbouring workgroups read
+// too (c2g_bucket_clear zeroes those).
 static constexpr int kAvgK = 4;
 static constexpr int kAvgW = 2 * kAvgK + 1;
 
 __global__ void __launch_bounds__(64 * kAvgW)
-    c2g_colscan_avg9(const int32_t *__restrict__ cnt, const double *__restrict__ qbot,
+    c2g_colscan_avg9(int32_t *__restrict__ cnt, const double *__restrict__ qbot,
                      double *__restrict__ out, int n2, double qjump, Buckets bk) {
   constexpr int TPS = 4;  // tiles of 64 rows per stage (one barrier each)
   __shared__ double tile[2][kAvgW][64 * TPS + 1];  // [buffer][column][1 + row], 0: carry
@@ -471,7 +502,7 @@ __global__ void __launch_bounds__(64 * kAvgW)
   const int i0 = blockIdx.x * kAvgK;
   int col = 2 * i0 - 1 + w;
   if (col < 0) col += n2;
-  const int32_t *cc = cnt + (int64_t)col * n2;
+  int32_t *cc = cnt + (int64_t)col * n2;
   double v = qbot[col];
   const int tid = threadIdx.x, oc = tid >> 7, jr = tid & 127;
   // counts of the next stage load while the current one is swept
@@ -493,6 +524,7 @@ __global__ void __launch_bounds__(64 * kAvgW)
     for (int u = 0; u < TPS; u++) {
       if (t0 + u < ntiles) {
         const int r = 64 * (t0 + u) + l;
+        if (wv[u] && w != 0 && w != kAvgW - 1) cc[r] = 0;
         const double cv = wv[u] ? cell_value(wv[u], (int64_t)col * n2 + r, qjump, bk) : 0.0;
         uint64_t mask = __ballot(wv[u] != 0);
         double mine = v;
@@ -564,9 +596,14 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
     set_error("con2grid: fine grid too large for int32 cell keys");
     return HYD_EARG;
   }
-  HYD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * nfine, c.s));
-  HYD_HIP(hipMemsetAsync(qjx, 0, sizeof(int32_t) * nxf, c.s));
-  HYD_HIP(hipMemsetAsync(c.flags, 0, sizeof(int32_t) * 8, c.s));
+  // the context's own counters come back zero from the previous sweep
+  const bool own = cnt == c.cnt && qjx == c.qjx_cnt;
+  if (!(own && c.sweep_clean)) {
+    HYD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * nfine, c.s));
+    HYD_HIP(hipMemsetAsync(qjx, 0, sizeof(int32_t) * nxf, c.s));
+    HYD_HIP(hipMemsetAsync(c.flags, 0, sizeof(int32_t) * 8, c.s));
+  }
+  c.sweep_clean = false;
   C2GArgs a;
   a.xq = xq;
   a.yq = yq;
@@ -607,15 +644,19 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
     hipLaunchKernelGGL(c2g_buckets<1>, dim3(nb), dim3(256), 0, c.s, a);
   }
   hipLaunchKernelGGL(c2g_bucket_sum, dim3(256), dim3(256), 0, c.s, a.bk, c.flags, a.qjump);
-  hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(1024), 0, c.s, qjx, qbot, nxf, nfine, a.qjump, a.bk);
+  hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(1024), 0, c.s, qjx, qbot, nxf, nfine, a.qjump, a.bk,
+                     c.flags);
   if (avg)
     hipLaunchKernelGGL(c2g_colscan_avg9, dim3(nxf / 2 / kAvgK), dim3(64 * kAvgW), 0, c.s, cnt,
                        qbot, avg, nxf, a.qjump, a.bk);
   else
     hipLaunchKernelGGL(c2g_colscan, dim3((nxf + 3) / 4), dim3(256), 0, c.s, cnt, qbot, fq, nxf,
                        nyf, a.qjump, a.bk);
-  hipLaunchKernelGGL(c2g_bucket_clear, dim3(256), dim3(256), 0, c.s, a.bk, c.flags);
+  // the fused sweep's shared columns are 2K-1, 4K-1, ... (0-based)
+  hipLaunchKernelGGL(c2g_bucket_clear, dim3(256), dim3(256), 0, c.s, a.bk, c.flags, cnt, nxf,
+                     avg ? 2 * kAvgK : 0);
   HYD_HIP(hipGetLastError());
+  c.sweep_clean = own;
   return HYD_OK;
 }
 
@@ -869,6 +910,7 @@ int con2grid_mixed(Ctx &c, float *qc, bool defer_mean) {
   const int nxf = k.nxf, nyf = k.nyf;
   const int64_t nfine = (int64_t)nxf * nyf;
   KTimer tm(c, "con2grid32", (int64_t)n * 24 + 4 * N);
+  c.sweep_clean = false;  // this sweep clears at its start and leaves counts behind
   if (nfine % 4 == 0)
     hipLaunchKernelGGL(c2g_fill_i32x4, dim3(4096), dim3(256), 0, c.s, (int4 *)c.cnt, nfine / 4);
   else

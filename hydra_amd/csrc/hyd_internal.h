@@ -100,6 +100,10 @@ struct DBuf {
 // Device context.
 struct Scalars {  // lives in device memory, mirrored on the host
   double t, dt, dt2, dt3, dt6, twist, zzrms, zzmax, uumax, qjump, dtmax, dtacc;
+  // con2grid's sticky error flag as adapt saw it (nonzero: a sweep since the
+  // last check overflowed), so a loop that skips the end-of-step read-back
+  // still learns of it at the next step's adapt
+  double sweep_err;
 };
 
 struct Ctx {
@@ -134,6 +138,9 @@ struct Ctx {
   int32_t *qjx_cnt = nullptr;  // nxf packed bottom-row counts
   double *qbot = nullptr;      // nxf
   int32_t *flags = nullptr;    // small device counters; [8] sticky overflow
+  // cnt, qjx_cnt and flags[0..7] are all zero: every FP64 sweep clears what
+  // it used on its way out, so the next one skips the clearing fills
+  bool sweep_clean = false;
   // order-dependent cells: key -> bucket table, bucket list, entries, sums
   int32_t *rp_tkey = nullptr, *rp_tid = nullptr, *rp_list = nullptr;
   int64_t rp_tsize = 0, rp_bcap = 0;
@@ -200,6 +207,7 @@ struct Ctx {
   // surgery (0) and recontouring (1) calls and their wall seconds
   int64_t phase_n[2] = {0, 0};
   double phase_s[2] = {0.0, 0.0};
+  int64_t n_advances = 0, n_node_updates = 0;  // RK4 advances taken, and their nodes
   bool have_field = false;
   // scalars
   Scalars *dsc = nullptr;   // device

@@ -114,8 +114,9 @@ struct AdaptArgs {
 
 // evolution.f90:553-578: scalars of adapt.
 __global__ void k_adapt_scalars(Scalars *sc, const double *sum, const unsigned long long *mx,
-                                AdaptArgs a) {
+                                AdaptArgs a, const int32_t *sweep_err) {
   if (threadIdx.x != 0) return;
+  sc->sweep_err = sweep_err ? (double)*sweep_err : 0.0;
   const double dtfac = 3.141592653589793238462643383279502884197169399375105820974944592307816 / 10.0;
   const double cflmax = 0.7;
   const double zzrms = sqrt(a.dsumi * sum[0]);
@@ -489,7 +490,8 @@ int adapt(Ctx &c) {
   a.dt_fixed = c.p.dt_fixed;
   a.tgsave = c.p.tgsave;
   a.tcsave = c.p.tcsave;
-  hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, mx, a);
+  hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, mx, a,
+                     c.flags + 8);
   HYD_HIP(hipGetLastError());
   // the integrating factors need dt and zzrms on the host: start the copy
   // now, evaluate the exponentials in adapt_finish while the GPU computes
@@ -595,6 +597,8 @@ static int fork_s2(Ctx &c) {
 int advance(Ctx &c) {
   const int64_t N = c.N;
   c.after_sweep = nullptr;
+  c.n_advances += 1;
+  c.n_node_updates += c.nptq;
   auto fields = [&](int stage) {
     return on_s2(c, [&]() {
       HYD_CHECK(rk_fields(c, stage));
@@ -631,6 +635,9 @@ int advance(Ctx &c) {
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
   HYD_CHECK(contour_grid(c));
   HYD_CHECK(on_s2(c, [&]() { return adapt_finish(c); }));
+  // an overflow of an earlier sweep (the scalars are on the host now):
+  // reported, and the table reset, behind the main stream's queued sweep
+  if (c.hsc->sweep_err != 0.0) return c2g_status(c);
   HYD_CHECK(fields(1));
   // stages 2-4
   for (int stage = 2; stage <= 4; stage++) {
@@ -790,7 +797,8 @@ static int adapt_mixed(Ctx &c) {
   a.dt_fixed = c.p.dt_fixed;
   a.tgsave = c.p.tgsave;
   a.tcsave = c.p.tcsave;
-  hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, mx, a);
+  hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, mx, a,
+                     nullptr);
   hipLaunchKernelGGL(k_exp_table, dim3((c.nqu + 255) / 256), dim3(256), 0, c.s, c.qdiss_u_dev,
                      c.nqu, c.dsc, c.qdiss_exp);
   hipLaunchKernelGGL(k_expand_epq, grid1(N), dim3(256), 0, c.s, c.epq, c.emq, c.qdiss_map,

@@ -264,6 +264,15 @@ interface
     integer(c_int32_t), value :: nsteps
   end function
 
+  ! advances taken and their node-updates (throughput accounting)
+  integer(c_int) function hyd_advance_counts(ctx, advances, updates, reset) &
+      bind(c, name="hyd_advance_counts")
+    import :: c_int, c_ptr, c_int32_t, c_int64_t
+    type(c_ptr), value :: ctx
+    integer(c_int64_t), intent(out) :: advances, updates
+    integer(c_int32_t), value :: reset
+  end function
+
   ! ensemble of independent flows (array of contexts)
   integer(c_int) function hyd_run_batch(ctxs, n, nsteps) bind(c, name="hyd_run_batch")
     import :: c_int, c_ptr, c_int32_t

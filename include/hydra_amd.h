@@ -229,7 +229,9 @@ int hyd_advect_step(hyd_ctx *ctx);
    nsteps advances have been taken.  A call resumes where the previous one
    stopped: inside the advect loop it continues the loop (run(a) then
    run(b) is run(a+b)); after set_field, set_contours or prepare it starts
-   with the recontouring, after recontour with advect's start. */
+   with the recontouring, after recontour with advect's start.  Between the
+   steps of one call the host does not wait for the GPU (each step's adapt
+   brings its scalars back); the call returns once all of them are done. */
 int hyd_run(hyd_ctx *ctx, int32_t nsteps);
 /* Regularisation costs of the runs so far (no reference counterpart; the
    reference's phases are timed from outside): out[0] surgery calls, out[1]
@@ -237,6 +239,11 @@ int hyd_run(hyd_ctx *ctx, int32_t nsteps);
    call timed to its stream's drain).  reset != 0 zeroes the counters after
    reading; out may be NULL. */
 int hyd_phase_times(hyd_ctx *ctx, double *out, int32_t reset);
+/* RK4 advances taken so far and their node-updates (the sum over advances
+   of the node count each one moved), for throughput accounting; no
+   reference counterpart.  reset != 0 zeroes both after reading; either
+   pointer may be NULL. */
+int hyd_advance_counts(hyd_ctx *ctx, int64_t *advances, int64_t *node_updates, int32_t reset);
 
 /* ---- diagnostics records and saves (evolution.f90:72, 111, 581-599) ------ */
 /* The rows the reference writes each step and at each regularisation,
