@@ -225,10 +225,7 @@ __device__ __forceinline__ void bk_visit(const C2GArgs &a, int32_t key, int32_t 
 }
 
 template <int PASS>
-__global__ void c2g_buckets(C2GArgs a) {
-  if (a.flags[0] == 0 && a.flags[1] == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+__device__ __forceinline__ void bk_node(const C2GArgs &a, int i) {
   const SegGeom sg = seg_geom(a, i);
   const double xi = sg.xi, yi = sg.yi, dx = sg.dx, dy = sg.dy, yy = sg.yy;
   const int nxc = sg.nxc, ixci = sg.ixci;
@@ -258,6 +255,15 @@ __global__ void c2g_buckets(C2GArgs a) {
       if (is_flagged(w)) bk_visit<PASS>(a, (int32_t)cell, w, i, sgn);
     }
   }
+}
+
+// Grid-stride over the nodes: the launch is small, since it usually finds
+// no flagged cell and returns at once.
+template <int PASS>
+__global__ void c2g_buckets(C2GArgs a) {
+  if (a.flags[0] == 0 && a.flags[1] == 0) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x)
+    bk_node<PASS>(a, i);
 }
 
 // One thread per bucket: order the entries by node and add them in that
@@ -346,36 +352,40 @@ __device__ __forceinline__ int wave_scan_i32(int v) {
 // qbot (contours.f90:234-238): qbot(1) = 0, qbot(ix+1) = qbot(ix) + qjx(ix)
 // over the non-empty bottom-row counters, in order.  Per pass of kQbotPart
 // counters the block evaluates the counters' values and compacts the
-// non-empty ones into LDS in order (a block scan of their count); wave 0
-// then runs the chain over the compacted list 64 values at a time (fixed
-// lane indices, so each step is one dependent add), recording the running
-// value after each; every counter then reads the value after the last
-// non-empty one at or before it.  The additions are the reference's, in its
-// order.  nxf <= kQbotMax.  Being the last reader of the counters and of
+// non-empty ones into LDS in order (a block scan of their count); one lane
+// then runs the chain over the compacted list, eight values loaded ahead of
+// each group of eight dependent adds, overwriting each value with the chain
+// after it; every counter then reads the value after the last non-empty one
+// at or before it.  The additions are the reference's, in its order.
+// nxf <= kQbotMax.  Being the last reader of the counters and of
 // flags[0..5], it leaves them zero for the next sweep (the bucket count
 // moves to flags[6] for c2g_bucket_clear).
 static constexpr int kQbotMax = 32768;
-static constexpr int kQbotPart = 2048;  // counters per pass, two per thread
+static constexpr int kQbotPart = 4096;  // counters per pass, four per thread
+static constexpr int kQbotPer = kQbotPart / 1024;
 __global__ void __launch_bounds__(1024) c2g_qbot(int32_t *qjx, double *qbot, int nxf,
                                                  int64_t nkey, double qjump, Buckets bk,
                                                  int32_t *flags) {
-  __shared__ double comp[kQbotPart];  // values of the non-empty counters, in order
-  __shared__ double pref[kQbotPart];  // the chain after each of them
+  __shared__ double comp[kQbotPart + 8];  // non-empty values, then the chain after each
   __shared__ int wsum[16];
-  __shared__ double vin;              // the chain's value entering the pass
+  __shared__ double vin;  // the chain's value entering the pass
   const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
-  double v = 0.0;  // qbot(1); wave 0 carries the chain across passes
+  double v = 0.0;  // qbot(1); thread 0 carries the chain across passes
   if (tid == 0) qbot[0] = 0.0;
   for (int part = 0; part < nxf - 1; part += kQbotPart) {
     const int m = min(kQbotPart, nxf - 1 - part);
-    const int t0 = 2 * tid, t1 = t0 + 1;
-    const int32_t w0 = t0 < m ? qjx[part + t0] : 0;
-    const int32_t w1 = t1 < m ? qjx[part + t1] : 0;
-    if (w0) qjx[part + t0] = 0;
-    if (w1) qjx[part + t1] = 0;
-    const double c0 = w0 ? cell_value(w0, nkey + part + t0, qjump, bk) : 0.0;
-    const double c1 = w1 ? cell_value(w1, nkey + part + t1, qjump, bk) : 0.0;
-    const int n0 = w0 != 0, n1 = w1 != 0, cnt = n0 + n1;
+    const int t0 = kQbotPer * tid;
+    int32_t w[kQbotPer];
+#pragma unroll
+    for (int u = 0; u < kQbotPer; u++) w[u] = t0 + u < m ? qjx[part + t0 + u] : 0;
+    double cv[kQbotPer];
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < kQbotPer; u++) {
+      if (w[u]) qjx[part + t0 + u] = 0;
+      cv[u] = w[u] ? cell_value(w[u], nkey + part + t0 + u, qjump, bk) : 0.0;
+      cnt += w[u] != 0;
+    }
     const int incl = wave_scan_i32(cnt);
     if (l == 63) wsum[wv] = incl;
     if (tid == 0) vin = v;
@@ -387,28 +397,42 @@ __global__ void __launch_bounds__(1024) c2g_qbot(int32_t *qjx, double *qbot, int
       total += ws;
     }
     const int ex = off + incl - cnt;  // non-empty counters before t0
-    if (n0) comp[ex] = c0;
-    if (n1) comp[ex + n0] = c1;
-    __syncthreads();
-    if (wv == 0) {
-      for (int j0 = 0; j0 < total; j0 += 64) {
-        const int nk = min(64, total - j0);
-        const double val = l < nk ? comp[j0 + l] : 0.0;
-        double mine = 0.0;
+    {
+      int e = ex;
 #pragma unroll
-        for (int k = 0; k < 64; k++) {
-          if (k >= nk) break;
-          v = readlane_f64(val, k) + v;
-          if (l == k) mine = v;
+      for (int u = 0; u < kQbotPer; u++)
+        if (w[u]) comp[e++] = cv[u];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double x[8], y[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) x[u] = comp[u];
+      for (int j0 = 0; j0 < total; j0 += 8) {
+        // the next group's values are read before this group's adds (the
+        // reads beyond total hit the padding and are not used)
+#pragma unroll
+        for (int u = 0; u < 8; u++) y[u] = comp[min(j0 + 8 + u, kQbotPart + 7)];
+        const int nk = min(8, total - j0);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          if (u < nk) {
+            v = x[u] + v;
+            comp[j0 + u] = v;
+          }
         }
-        if (l < nk) pref[j0 + l] = mine;
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = y[u];
       }
     }
     __syncthreads();
     const double v0 = vin;
-    const int i0 = ex + n0, i1 = i0 + n1;  // non-empty counters up to t0, t1
-    if (t0 < m) qbot[part + t0 + 1] = i0 > 0 ? pref[i0 - 1] : v0;
-    if (t1 < m) qbot[part + t1 + 1] = i1 > 0 ? pref[i1 - 1] : v0;
+    int e = ex;
+#pragma unroll
+    for (int u = 0; u < kQbotPer; u++) {
+      e += w[u] != 0;  // non-empty counters up to t0 + u
+      if (t0 + u < m) qbot[part + t0 + u + 1] = e > 0 ? comp[e - 1] : v0;
+    }
     __syncthreads();
   }
   if (tid == 0) {
@@ -640,8 +664,9 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
   if (n > 0) {
     hipLaunchKernelGGL(c2g_nodes, dim3(nb), dim3(256), 0, c.s, a);
     HYD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(c2g_buckets<0>, dim3(nb), dim3(256), 0, c.s, a);
-    hipLaunchKernelGGL(c2g_buckets<1>, dim3(nb), dim3(256), 0, c.s, a);
+    const int nbk = std::min(nb, 1024);
+    hipLaunchKernelGGL(c2g_buckets<0>, dim3(nbk), dim3(256), 0, c.s, a);
+    hipLaunchKernelGGL(c2g_buckets<1>, dim3(nbk), dim3(256), 0, c.s, a);
   }
   hipLaunchKernelGGL(c2g_bucket_sum, dim3(256), dim3(256), 0, c.s, a.bk, c.flags, a.qjump);
   hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(1024), 0, c.s, qjx, qbot, nxf, nfine, a.qjump, a.bk,

@@ -269,3 +269,103 @@ def test_recontour_cycle_bitwise():
     for name in ("qc", "qd", "qs"):
         assert bits_equal(h.field(name), o.d(name)), name
     h.close()
+
+
+def _hairpins(ng, nest=4):
+    """Closed thin loops whose crossings of one fine x-gridline fall in one
+    fine cell with both signs, `nest` of each: order-dependent cells
+    (summed in node order from the buckets), inside the grid and on the
+    bottom row (loops wrapping the periodic y boundary), plus one large
+    circle so that the grid has ordinary cells too."""
+    h = 2 * np.pi / (4 * ng)  # fine cell
+    xs, ys, npq = [], [], []
+
+    def loop(pts):
+        xs.extend(p[0] for p in pts)
+        ys.extend(p[1] for p in pts)
+        npq.append(len(pts))
+    # interior: rectangles across a few gridlines, both long sides in one fine row
+    for cx, cy in ((0.3, -1.1), (-2.0, 0.7), (1.7, 2.2)):
+        x0 = -np.pi + h * (round((cx + np.pi) / h) + 0.5)
+        y0 = -np.pi + h * (round((cy + np.pi) / h) + 0.2)
+        for k in range(nest):
+            ya, yb = y0 + 0.05 * h * k, y0 + 0.55 * h - 0.05 * h * k
+            loop([(x0 - 0.1 * k * h, ya), (x0 + 3 * h + 0.1 * k * h, ya),
+                  (x0 + 3 * h + 0.1 * k * h, yb), (x0 - 0.1 * k * h, yb)])
+    # bottom row: loops through y = +-pi, both vertical sides in one fine column
+    for cx in (-0.9, 2.6):
+        x0 = -np.pi + h * (round((cx + np.pi) / h) + 0.2)
+        for k in range(nest):
+            xa, xb = x0 + 0.05 * h * k, x0 + 0.55 * h - 0.05 * h * k
+            d = 3 * h + 0.1 * h * k
+            loop([(xa, np.pi - d), (xa, -np.pi + d), (xb, -np.pi + d), (xb, np.pi - d)])
+    # one circle
+    t = np.linspace(0.0, 2 * np.pi, 400, endpoint=False)
+    loop(list(zip(1.3 * np.cos(t) - 0.2, 1.1 * np.sin(t) + 0.1)))
+    n = len(xs)
+    i1 = np.cumsum([0] + npq[:-1]) + 1
+    nxt = np.empty(n, dtype=np.int32)
+    for a, m in zip(i1, npq):
+        idx = np.arange(a, a + m)
+        nxt[idx - 1] = np.roll(idx, -1)
+    nq = len(npq)
+    return dict(x=np.array(xs), y=np.array(ys), nextq=nxt, i1q=i1.astype(np.int32),
+                i2q=(i1 + np.array(npq) - 1).astype(np.int32), npq=np.array(npq, np.int32),
+                indq=np.ones(nq, np.int32), qjump=0.1234567891234)
+
+
+def _order_dependent(c, ng):
+    """Fine cells and bottom-row counters with both signs and at least
+    three of one (crossings located as contours.f90:206-269 does)."""
+    import math
+    nxf, ell, hl = 4 * ng, 2 * np.pi, np.pi
+    glxfi = nxf / ell
+    xg = -hl + (ell / nxf) * np.arange(nxf)
+    cells, bottom = {}, {}
+    x, y, nx = c["x"], c["y"], c["nextq"]
+    for i in range(len(x)):
+        xi, yi = x[i], y[i]
+        xa, ya = x[nx[i] - 1], y[nx[i] - 1]
+        ixci, ixca = 1 + int(glxfi * (xi + hl)), 1 + int(glxfi * (xa + hl))
+        dx = (xa - xi) - ell * int((xa - xi) / hl)
+        yy = ya - yi
+        dy = yy - ell * int(yy / hl)
+        if abs(yy) > hl:
+            cc = math.copysign(1.0, dy)
+            p = -(yi - hl * cc) / (dy + 1e-12)
+            ix = 1 + int(glxfi * math.fmod(ell + hl + xi + p * dx, ell))
+            bottom.setdefault(ix, []).append(-1 if cc > 0 else 1)
+        ixdif = ixca - ixci
+        nxc = ixdif - nxf * int(2 * ixdif / nxf)
+        jump = 1 if nxc > 0 else -1
+        for ncr in range(0, nxc, jump):
+            ix = 1 + (ixci + (0 if jump == 1 else -1) + nxf + ncr) % nxf
+            x2 = xg[ix - 1] - xi
+            y2 = yi + (x2 - ell * int(x2 / hl)) / dx * dy
+            iy = 2 + int(glxfi * (y2 - ell * int(y2 / hl) + hl))
+            cells.setdefault((ix, iy), []).append(1 if dx > 0 else -1)
+
+    def flagged(d):
+        return sum(1 for v in d.values() if v.count(1) and v.count(-1)
+                   and max(v.count(1), v.count(-1)) >= 3)
+    return flagged(cells), flagged(bottom)
+
+
+@pytest.mark.parametrize("ng", NGS)
+def test_con2grid_order_dependent_cells_bitwise(ng, hyd_cache):
+    """Fine cells and bottom-row counters holding both signs with several of
+    each (summed in node order) give the oracle's grid, through the
+    diagnostic sweep (with counts) and through the step's fused sweep."""
+    from oracle.hyo import Oracle
+    c = _hairpins(ng)
+    o = Oracle(ng)
+    o.set_contours(c)
+    qc_o, cnt_o = o.con2grid(with_counts=True)
+    cells, bottom = _order_dependent(c, ng)
+    assert cells >= 6 and bottom >= 2  # the case is exercised
+    h = hyd_cache(ng)
+    for rep in range(2):  # the second sweep starts from the counters the first cleared
+        qc_h, cnt_h = h.con2grid(c["x"], c["y"], c["nextq"], c["qjump"], with_counts=True)
+        assert np.array_equal(cnt_h.ravel(), cnt_o.ravel())
+        assert bits_equal(qc_h, qc_o)
+        assert bits_equal(h.con2grid(c["x"], c["y"], c["nextq"], c["qjump"]), o.con2grid())
