@@ -20,8 +20,12 @@
 //     reference's expression order.
 //   * the mean (contours.f90:330-336) is the exact sequential sum (seqsum).
 //
-// Fine-grid layout here: column-major by fine column, rows iy = 1..nyf,
-// index (ix-1)*nyf + (iy-1) for both the counters and the swept field.
+// Fine-grid layout here: the swept field is column-major by fine column,
+// rows iy = 1..nyf, index (ix-1)*nyf + (iy-1), which is also the key of a
+// cell.  The counters are stored by groups of 4 fine columns, the group's
+// 4 counters of a row side by side (cnt_at): a 128-byte line holds 8 rows
+// of 4 columns, so the crossings a contour makes in neighbouring columns
+// share lines, which halves the cost of the counter atomics.  nxf % 4 == 0.
 #include "hyd_internal.h"
 
 namespace hyd {
@@ -59,6 +63,11 @@ struct C2GArgs {
   int32_t *flags;
   Buckets bk;
 };
+
+// Address of the counter of fine cell (column cx, row cy), both 0-based.
+__device__ __forceinline__ int64_t cnt_at(int cx, int cy, int nyf) {
+  return (int64_t)(cx >> 2) * 4 * nyf + 4 * cy + (cx & 3);
+}
 
 __device__ __forceinline__ bool is_flagged(int32_t w) {
   const int p = w & 0xffff, q = (w >> 16) & 0xffff;
@@ -161,7 +170,7 @@ __global__ void c2g_nodes(C2GArgs a) {
           const double y2 = yi + px0 * dy;
           const int iy = 2 + (int)(a.glyfi * (y2 - a.elly * (double)(int)(y2 * a.hlyi) - a.ymin));
           // row nyf+1 is never swept (contours.f90:274): its jumps are dropped
-          if (iy >= 2 && iy <= a.nyf) cell[b] = (int64_t)(ix - 1) * a.nyf + (iy - 1);
+          if (iy >= 2 && iy <= a.nyf) cell[b] = cnt_at(ix - 1, iy - 1, a.nyf);
         }
       }
       ncr += jump * nb;
@@ -251,7 +260,7 @@ __device__ __forceinline__ void bk_node(const C2GArgs &a, int i) {
       const int iy = 2 + (int)(a.glyfi * (y2 - a.elly * (double)(int)(y2 * a.hlyi) - a.ymin));
       if (iy < 2 || iy > a.nyf) continue;
       const int64_t cell = (int64_t)(ix - 1) * a.nyf + (iy - 1);
-      const int32_t w = a.cnt[cell];
+      const int32_t w = a.cnt[cnt_at(ix - 1, iy - 1, a.nyf)];
       if (is_flagged(w)) bk_visit<PASS>(a, (int32_t)cell, w, i, sgn);
     }
   }
@@ -289,12 +298,10 @@ __global__ void c2g_bucket_sum(Buckets b, const int32_t *flags, double qjump) {
   }
 }
 
-// Empty the table again (after the sweep has read the bucket sums); the
-// bucket count is the copy c2g_qbot kept in flags[6].  With halo > 0 it
-// also zeroes the counter columns halo-1, 2 halo-1, ... (n2 rows each,
-// n2 % 4 == 0): the fused sweep reads each of them from two workgroups and
-// so leaves them to this kernel.
-__global__ void c2g_bucket_clear(Buckets b, int32_t *flags, int32_t *cnt, int n2, int halo) {
+// Empty the table again (after the sweep has read the bucket sums; the
+// bucket count is the copy c2g_qbot kept in flags[6]) and zero the n4 int4
+// of counters for the next sweep, in whole lines.
+__global__ void c2g_bucket_clear(Buckets b, int32_t *flags, int4 *cnt, int64_t n4) {
   const int nb_used = min(flags[6], b.bcap);
   const int stride = gridDim.x * blockDim.x;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -302,14 +309,7 @@ __global__ void c2g_bucket_clear(Buckets b, int32_t *flags, int32_t *cnt, int n2
     b.tkey[b.list[4 * nb]] = -1;
     if (b.list[4 * nb + 3] != b.list[4 * nb + 2]) atomicOr(&flags[8], 2);
   }
-  if (halo > 0) {
-    const int q = n2 / 4;  // int4 per column
-    const int64_t total = (int64_t)(n2 / halo) * q;
-    for (int64_t k = tid; k < total; k += stride) {
-      const int64_t col = (k / q + 1) * halo - 1;
-      reinterpret_cast<int4 *>(cnt + col * n2)[k % q] = make_int4(0, 0, 0, 0);
-    }
-  }
+  for (int64_t k = tid; k < n4; k += stride) cnt[k] = make_int4(0, 0, 0, 0);
 }
 
 // Value of a cell's node-order sum from its counts.
@@ -444,15 +444,13 @@ __global__ void __launch_bounds__(1024) c2g_qbot(int32_t *qjx, double *qbot, int
   }
 }
 
-// y sweep (contours.f90:272-277): one wavefront per fine column.  Each
-// non-empty counter is zeroed once read (the next sweep starts from zero).
-__global__ void c2g_colscan(int32_t *cnt, const double *qbot, double *fq, int nxf, int nyf,
+// y sweep (contours.f90:272-277): one wavefront per fine column.
+__global__ void c2g_colscan(const int32_t *cnt, const double *qbot, double *fq, int nxf, int nyf,
                             double qjump, Buckets bk) {
   const int wpb = blockDim.x / 64;
   const int col = blockIdx.x * wpb + threadIdx.x / 64;
   const int l = threadIdx.x & 63;
   if (col >= nxf) return;
-  int32_t *cc = cnt + (int64_t)col * nyf;
   double *out = fq + (int64_t)col * nyf;
   double v = qbot[col];
   constexpr int U = 8;  // segments of 64 rows loaded ahead
@@ -461,12 +459,11 @@ __global__ void c2g_colscan(int32_t *cnt, const double *qbot, double *fq, int nx
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int r = base + 64 * u + l;
-      w[u] = (r < nyf && r >= 1) ? cc[r] : 0;
+      w[u] = (r < nyf && r >= 1) ? cnt[cnt_at(col, r, nyf)] : 0;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int r = base + 64 * u + l;  // iy-1
-      if (w[u]) cc[r] = 0;
       const double cv =
           w[u] ? cell_value(w[u], (int64_t)col * nyf + r, qjump, bk) : 0.0;
       uint64_t mask = __ballot(w[u] != 0);
@@ -507,16 +504,12 @@ __global__ void c2g_avg9(const T *__restrict__ f, T *__restrict__ out, int nh) {
 // y sweep, 64 rows at a time) and averages them into output columns
 // i0 .. i0+K-1 through LDS, so the fine field never reaches HBM.  Output
 // row 0 needs the last fine row (periodic) and is written at the end.
-// Needs n2 % 64 == 0 and (n2 / 2) % K == 0.  Non-empty counters are zeroed
-// as they are read, while their lines are still in the cache (zeroing them
-// after the sweep costs partial-line writes to HBM), except in the first
-// and last column of the workgroup, which the neighbouring workgroups read
-// too (c2g_bucket_clear zeroes those).
+// Needs n2 % 64 == 0 and (n2 / 2) % K == 0.
 static constexpr int kAvgK = 4;
 static constexpr int kAvgW = 2 * kAvgK + 1;
 
 __global__ void __launch_bounds__(64 * kAvgW)
-    c2g_colscan_avg9(int32_t *__restrict__ cnt, const double *__restrict__ qbot,
+    c2g_colscan_avg9(const int32_t *__restrict__ cnt, const double *__restrict__ qbot,
                      double *__restrict__ out, int n2, double qjump, Buckets bk) {
   constexpr int TPS = 4;  // tiles of 64 rows per stage (one barrier each)
   __shared__ double tile[2][kAvgW][64 * TPS + 1];  // [buffer][column][1 + row], 0: carry
@@ -526,7 +519,7 @@ __global__ void __launch_bounds__(64 * kAvgW)
   const int i0 = blockIdx.x * kAvgK;
   int col = 2 * i0 - 1 + w;
   if (col < 0) col += n2;
-  int32_t *cc = cnt + (int64_t)col * n2;
+  const int32_t *cc = cnt + cnt_at(col, 0, n2);  // row r at cc[4 r]
   double v = qbot[col];
   const int tid = threadIdx.x, oc = tid >> 7, jr = tid & 127;
   // counts of the next stage load while the current one is swept
@@ -534,21 +527,20 @@ __global__ void __launch_bounds__(64 * kAvgW)
 #pragma unroll
   for (int u = 0; u < TPS; u++) {
     const int r = 64 * u + l;
-    wn[u] = (u < ntiles && r >= 1) ? cc[r] : 0;
+    wn[u] = (u < ntiles && r >= 1) ? cc[4 * r] : 0;
   }
   for (int st = 0; st * TPS < ntiles; st++) {
     const int t0 = st * TPS, b = st & 1;
 #pragma unroll
     for (int u = 0; u < TPS; u++) {
       wv[u] = wn[u];
-      wn[u] = t0 + TPS + u < ntiles ? cc[64 * (t0 + TPS + u) + l] : 0;
+      wn[u] = t0 + TPS + u < ntiles ? cc[4 * (64 * (t0 + TPS + u) + l)] : 0;
     }
     if (l == 0) tile[b][w][0] = v;  // row 64 t0 - 1 (unused for st = 0)
 #pragma unroll
     for (int u = 0; u < TPS; u++) {
       if (t0 + u < ntiles) {
         const int r = 64 * (t0 + u) + l;
-        if (wv[u] && w != 0 && w != kAvgW - 1) cc[r] = 0;
         const double cv = wv[u] ? cell_value(wv[u], (int64_t)col * n2 + r, qjump, bk) : 0.0;
         uint64_t mask = __ballot(wv[u] != 0);
         double mine = v;
@@ -620,6 +612,10 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
     set_error("con2grid: fine grid too large for int32 cell keys");
     return HYD_EARG;
   }
+  if (nxf % 4) {
+    set_error("con2grid: fine grid must have a multiple of 4 columns");
+    return HYD_EARG;
+  }
   // the context's own counters come back zero from the previous sweep
   const bool own = cnt == c.cnt && qjx == c.qjx_cnt;
   if (!(own && c.sweep_clean)) {
@@ -677,9 +673,8 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
   else
     hipLaunchKernelGGL(c2g_colscan, dim3((nxf + 3) / 4), dim3(256), 0, c.s, cnt, qbot, fq, nxf,
                        nyf, a.qjump, a.bk);
-  // the fused sweep's shared columns are 2K-1, 4K-1, ... (0-based)
-  hipLaunchKernelGGL(c2g_bucket_clear, dim3(256), dim3(256), 0, c.s, a.bk, c.flags, cnt, nxf,
-                     avg ? 2 * kAvgK : 0);
+  hipLaunchKernelGGL(c2g_bucket_clear, dim3(1024), dim3(256), 0, c.s, a.bk, c.flags, (int4 *)cnt,
+                     nfine / 4);
   HYD_HIP(hipGetLastError());
   c.sweep_clean = own;
   return HYD_OK;
@@ -812,12 +807,11 @@ __global__ void c2g_colscan_int(const int32_t *cnt, const int32_t *qb, float *fq
   const int col = blockIdx.x * wpb + threadIdx.x / 64;
   const int l = threadIdx.x & 63;
   if (col >= nxf) return;
-  const int32_t *cc = cnt + (int64_t)col * nyf;
   float *out = fq + (int64_t)col * nyf;
   int32_t run = qb[col];
   for (int base = 0; base < nyf; base += 64) {
     const int r = base + l;
-    int32_t v = (r < nyf && r >= 1) ? level_jump(cc[r]) : 0;
+    int32_t v = (r < nyf && r >= 1) ? level_jump(cnt[cnt_at(col, r, nyf)]) : 0;
     for (int o = 1; o < 64; o <<= 1) {
       const int32_t u = __shfl_up(v, o);
       if (l >= o) v += u;
@@ -843,7 +837,7 @@ __global__ void __launch_bounds__(64 * kAvgW)
   const int i0 = blockIdx.x * kAvgK;
   int col = 2 * i0 - 1 + w;
   if (col < 0) col += n2;
-  const int32_t *cc = cnt + (int64_t)col * n2;
+  const int32_t *cc = cnt + cnt_at(col, 0, n2);  // row r at cc[4 r]
   int32_t run = qb[col];
   const int tid = threadIdx.x, oc = tid >> 7, jr = tid & 127;
   // counts of the next stage load while the current one is scanned
@@ -851,14 +845,14 @@ __global__ void __launch_bounds__(64 * kAvgW)
 #pragma unroll
   for (int u = 0; u < TPS; u++) {
     const int r = 64 * u + l;
-    wn[u] = (u < ntiles && r >= 1) ? cc[r] : 0;
+    wn[u] = (u < ntiles && r >= 1) ? cc[4 * r] : 0;
   }
   for (int st = 0; st * TPS < ntiles; st++) {
     const int t0 = st * TPS, b = st & 1;
 #pragma unroll
     for (int u = 0; u < TPS; u++) {
       wv[u] = level_jump(wn[u]);
-      wn[u] = t0 + TPS + u < ntiles ? cc[64 * (t0 + TPS + u) + l] : 0;
+      wn[u] = t0 + TPS + u < ntiles ? cc[4 * (64 * (t0 + TPS + u) + l)] : 0;
     }
     if (l == 0) tile[b][w][0] = (float)((double)run * qjump);  // row 64 t0 - 1 (unused for st = 0)
 #pragma unroll
