@@ -597,7 +597,11 @@ static int fft_launch1(Ctx &c, bool fwd, const TI *in, TO *out, int n, int m, in
 // Autotuner: for each layout class (forward strided->rows, forward
 // strided->strided, reverse strided->strided, reverse rows->strided) time
 // the candidate batches V on scratch fields and keep the fastest.  Every V
-// gives bit-identical results (each vector's arithmetic is the same).
+// gives bit-identical results (each vector's arithmetic is the same).  The
+// FP64 passes are timed as the step launches most of them, two fields per
+// launch, over three rounds that visit the candidates in turn (so that a
+// slow spell of the device does not favour one), each candidate keeping
+// its best round.
 int fft_autotune(Ctx &c) {
   const int n = c.c.nx;
   hipEvent_t e0, e1;
@@ -607,29 +611,37 @@ int fft_autotune(Ctx &c) {
   const int inr[4] = {0, 0, 0, 1}, outr[4] = {1, 0, 0, 0};
   int rc = HYD_OK;
   for (int cls = 0; cls < 4 && rc == HYD_OK; cls++) {
-    float best = 1e30f;
-    int bestV = 0;
-    for (int V = 1; V <= kMaxV && V <= n; V *= 2) {
-      if (fft_lds_bytes(n, V, sizeof(double)) > 160 * 1024) break;
-      c.fft_v[cls] = V;
-      const double *trig = fw[cls] ? c.xtrig : c.ytrig;
-      rc = fft_launch(c, fw[cls], c.wa, c.wb, n, n, inr[cls], outr[cls], 0, nullptr, 1.0, trig);
-      if (rc != HYD_OK) break;
-      if (hipEventRecord(e0, c.s) != hipSuccess) { rc = HYD_EHIP; break; }
-      for (int r = 0; r < 5 && rc == HYD_OK; r++)
-        rc = fft_launch(c, fw[cls], c.wa, c.wb, n, n, inr[cls], outr[cls], 0, nullptr, 1.0, trig);
-      if (rc != HYD_OK) break;
-      float ms = 0.0f;
-      if (hipEventRecord(e1, c.s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        rc = HYD_EHIP;
-        break;
-      }
-      if (ms < best) {
-        best = ms;
-        bestV = V;
+    float best[kMaxV + 1];
+    for (int V = 0; V <= kMaxV; V++) best[V] = 1e30f;
+    const double *trig = fw[cls] ? c.xtrig : c.ytrig;
+    const double *in[2] = {c.wa, c.wc};
+    double *out[2] = {c.wb, c.wd};
+    const double post[2] = {1.0, 1.0};
+    auto run = [&]() {
+      return fft_launch_t<double, double, double>(c, fw[cls], 2, in, out, n, n, inr[cls],
+                                                  outr[cls], nullptr, nullptr, post, trig);
+    };
+    for (int round = 0; round < 3 && rc == HYD_OK; round++) {
+      for (int V = 1; V <= kMaxV && V <= n; V *= 2) {
+        if (fft_lds_bytes(n, V, sizeof(double)) > 160 * 1024) break;
+        c.fft_v[cls] = V;
+        rc = run();
+        if (rc != HYD_OK) break;
+        if (hipEventRecord(e0, c.s) != hipSuccess) { rc = HYD_EHIP; break; }
+        for (int r = 0; r < 8 && rc == HYD_OK; r++) rc = run();
+        if (rc != HYD_OK) break;
+        float ms = 0.0f;
+        if (hipEventRecord(e1, c.s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = HYD_EHIP;
+          break;
+        }
+        best[V] = std::min(best[V], ms);
       }
     }
+    int bestV = 1;
+    for (int V = 2; V <= kMaxV; V *= 2)
+      if (best[V] < best[bestV]) bestV = V;
     c.fft_v[cls] = bestV;
   }
   // the FP32 passes of the mixed step (half the LDS per vector)
