@@ -109,7 +109,7 @@ static void free_ctx(Ctx *c) {
   if (c->hsc) hipHostFree(c->hsc);
   if (c->exp_host) hipHostFree(c->exp_host);
   if (c->ev_sc) hipEventDestroy(c->ev_sc);
-  for (hipEvent_t e : {c->ev_fork, c->ev_join})
+  for (hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_cleared})
     if (e) hipEventDestroy(e);
   free_congen(*c);
   diag_free(*c);
@@ -251,6 +251,7 @@ static int create(const hyd_params *p, hyd_ctx **out) {
       hipEventCreateWithFlags(&c->ev_sc, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_cleared, hipEventDisableTiming) != hipSuccess ||
       hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipHostMalloc / hipEventCreate / hipStreamCreate failed");
     free_ctx(c);

@@ -617,6 +617,7 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
     return HYD_EARG;
   }
   // the context's own counters come back zero from the previous sweep
+  HYD_CHECK(sweep_clear_join(c));
   const bool own = cnt == c.cnt && qjx == c.qjx_cnt;
   if (!(own && c.sweep_clean)) {
     HYD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * nfine, c.s));
@@ -673,10 +674,50 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
   else
     hipLaunchKernelGGL(c2g_colscan, dim3((nxf + 3) / 4), dim3(256), 0, c.s, cnt, qbot, fq, nxf,
                        nyf, a.qjump, a.bk);
-  hipLaunchKernelGGL(c2g_bucket_clear, dim3(1024), dim3(256), 0, c.s, a.bk, c.flags, (int4 *)cnt,
-                     nfine / 4);
+  c.clr_cnt = cnt;
+  c.clr_n4 = nfine / 4;
+  c.clear_pending = true;
+  if (!c.defer_clear) HYD_CHECK(sweep_clear(c));
   HYD_HIP(hipGetLastError());
   c.sweep_clean = own;
+  return HYD_OK;
+}
+
+static Buckets ctx_buckets(const Ctx &c) {
+  Buckets b;
+  b.tkey = c.rp_tkey;
+  b.tid = c.rp_tid;
+  b.list = c.rp_list;
+  b.slots = c.replay;
+  b.vals = c.replay_val;
+  b.tmask = (uint32_t)(c.rp_tsize - 1);
+  b.bcap = (int32_t)c.rp_bcap;
+  b.scap = c.replay_cap;
+  return b;
+}
+
+// The pending clearing of the last sweep, on the current stream c.s (on
+// the second stream it leaves an event for the next sweep to wait on).
+int sweep_clear(Ctx &c) {
+  if (!c.clear_pending) return HYD_OK;
+  c.clear_pending = false;
+  hipLaunchKernelGGL(c2g_bucket_clear, dim3(1024), dim3(256), 0, c.s, ctx_buckets(c), c.flags,
+                     (int4 *)c.clr_cnt, c.clr_n4);
+  HYD_HIP(hipGetLastError());
+  if (c.s == c.s2) {
+    HYD_HIP(hipEventRecord(c.ev_cleared, c.s));
+    c.clear_event = true;
+  }
+  return HYD_OK;
+}
+
+// Before a sweep (on c.s): the previous sweep's clearing has run.
+int sweep_clear_join(Ctx &c) {
+  if (c.clear_pending) return sweep_clear(c);  // nobody ran it: here, in order
+  if (c.clear_event) {
+    HYD_HIP(hipStreamWaitEvent(c.s, c.ev_cleared, 0));
+    c.clear_event = false;
+  }
   return HYD_OK;
 }
 
@@ -929,6 +970,7 @@ int con2grid_mixed(Ctx &c, float *qc, bool defer_mean) {
   const int nxf = k.nxf, nyf = k.nyf;
   const int64_t nfine = (int64_t)nxf * nyf;
   KTimer tm(c, "con2grid32", (int64_t)n * 24 + 4 * N);
+  HYD_CHECK(sweep_clear_join(c));  // no earlier clearing may land inside this sweep
   c.sweep_clean = false;  // this sweep clears at its start and leaves counts behind
   if (nfine % 4 == 0)
     hipLaunchKernelGGL(c2g_fill_i32x4, dim3(4096), dim3(256), 0, c.s, (int4 *)c.cnt, nfine / 4);

@@ -223,6 +223,14 @@ struct Ctx {
   double *diag_out = nullptr;    // 8 scratch results
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // The clearing that ends a sweep (counters zeroed, order-dependent cell
+  // table emptied) may be left to the caller (defer_clear), who runs it
+  // later on either stream (sweep_clear); the next sweep first waits for it
+  // (sweep_clear_join), or runs it itself if nobody has.
+  bool defer_clear = false, clear_pending = false, clear_event = false;
+  int32_t *clr_cnt = nullptr;
+  int64_t clr_n4 = 0;
+  hipEvent_t ev_cleared = nullptr;
   // work that the plain step starts on its second stream once the contour
   // grid's node and sweep kernels are queued (so that those run alone on
   // the GPU and the second stream's work overlaps the single-workgroup
@@ -296,6 +304,8 @@ int velint(Ctx &c, const double *uu, const double *vv, const double *xq, const d
 int con2grid(Ctx &c, const double *xq, const double *yq, const int32_t *nextq, int n,
              double *qc, int32_t *counts_out, bool defer_mean = false);
 int c2g_status(Ctx &c);
+int sweep_clear(Ctx &c);
+int sweep_clear_join(Ctx &c);
 
 template <class T>
 int dbuf(Ctx &c, const char *name, int64_t n, T **out) {

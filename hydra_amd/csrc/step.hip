@@ -586,6 +586,15 @@ static int on_s2(Ctx &c, Fn fn) {
   return rc;
 }
 
+// A contour grid whose sweep leaves its clearing to the caller (who runs
+// it on the second stream, off the main stream's path).
+static int contour_grid_deferred(Ctx &c) {
+  c.defer_clear = true;
+  const int rc = contour_grid(c);
+  c.defer_clear = false;
+  return rc;
+}
+
 // Fork: the second stream continues after everything enqueued so far on
 // the main stream.
 static int fork_s2(Ctx &c) {
@@ -612,6 +621,7 @@ int advance(Ctx &c) {
     c.after_sweep = [&c, stage]() {
       HYD_CHECK(fork_s2(c));
       return on_s2(c, [&]() {
+        HYD_CHECK(sweep_clear(c));  // the sweep just queued
         HYD_CHECK(source(c, 1));
         HYD_CHECK(rk_fields(c, stage));
         HYD_HIP(hipEventRecord(c.ev_join, c.s));
@@ -619,14 +629,18 @@ int advance(Ctx &c) {
       });
     };
   };
-  // stage 1
-  HYD_CHECK(inversion(c));
+  // stage 1 (its sweep's clearing runs on the second stream after adapt)
+  HYD_CHECK(contour_grid_deferred(c));
+  HYD_CHECK(inversion_fields(c));
   hipLaunchKernelGGL(k_reset, grid1(N), dim3(256), 0, c.s, c.qs, c.qd, c.qc, c.flo, c.fhi, N);
   // saves that fall at this step's start (set by the previous adapt)
   if (c.saves_on && (c.gsave || c.csave)) HYD_CHECK(save_slots(c, c.gsave != 0, c.csave != 0));
   HYD_CHECK(adapt(c));
   HYD_CHECK(fork_s2(c));
-  HYD_CHECK(on_s2(c, [&]() { return source(c, 0); }));
+  HYD_CHECK(on_s2(c, [&]() {
+    HYD_CHECK(sweep_clear(c));
+    return source(c, 0);
+  }));
   HYD_CHECK(rk_nodes(c, 1));
   // stage 2's contour grid needs only the stage-1 nodes: it is queued
   // before the host evaluates the integrating factors, so that the GPU has
@@ -643,7 +657,7 @@ int advance(Ctx &c) {
   for (int stage = 2; stage <= 4; stage++) {
     if (stage > 2) {
       if (stage != 3) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
-      HYD_CHECK(contour_grid(c));
+      HYD_CHECK(contour_grid_deferred(c));  // clearing in the after-sweep work
     }
     HYD_HIP(hipStreamWaitEvent(c.s, c.ev_join, 0));
     HYD_CHECK(inversion_fields(c));
