@@ -97,7 +97,8 @@ static void free_ctx(Ctx *c) {
                   c->rp_tid, c->rp_list, c->xq, c->yq, c->xqi, c->yqi, c->xqf, c->yqf, c->uq,
                   c->vq, c->nextq, c->nxt_tmp, c->dsc, c->dsum, c->seq_work, c->uu32,
                   c->vv32, c->fa32, c->fb32, c->fc32, c->tmp32, c->fq32, c->mq32, c->xtrig32,
-                  c->ytrig32, c->qb32, c->qdiss_u_dev, c->part, c->c2g_q, c->c2g_w, c->c2g_q32};
+                  c->ytrig32, c->qb32, c->qdiss_u_dev, c->part, c->c2g_q, c->c2g_w, c->c2g_q32,
+                  c->amax};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int k = 0; k < 4; k++) {
@@ -229,6 +230,7 @@ static int create(const hyd_params *p, hyd_ctx **out) {
   TRY(dalloc(&c->nxt_tmp, P));
   TRY(dalloc(&c->dsc, 1));
   TRY(dalloc(&c->dsum, 8));
+  TRY(dalloc(&c->amax, 2048));
   c->seq_cap = (std::max<int64_t>(N, 4 * N) + 255) / 256 + 64;
   {
     void *w = nullptr;

@@ -332,6 +332,15 @@ __device__ __forceinline__ double cell_value(int32_t w, int64_t key, double qjum
   return nb >= 0 ? b.vals[nb] : 0.0 / 0.0;
 }
 
+// cell_value for the common cells first: a single crossing of either sign
+// (0 + qjump and 0 - qjump are exact).
+__device__ __forceinline__ double cell_value_fast(int32_t w, int64_t key, double qjump,
+                                                  const Buckets &b) {
+  if (w == 1) return qjump;
+  if (w == (1 << 16)) return -qjump;
+  return cell_value(w, key, qjump, b);
+}
+
 // Inclusive wavefront scan of ints (row shifts, then row broadcasts).
 template <int CTRL>
 __device__ __forceinline__ int c2g_dpp(int v) {
@@ -541,7 +550,7 @@ __global__ void __launch_bounds__(64 * kAvgW)
     for (int u = 0; u < TPS; u++) {
       if (t0 + u < ntiles) {
         const int r = 64 * (t0 + u) + l;
-        const double cv = wv[u] ? cell_value(wv[u], (int64_t)col * n2 + r, qjump, bk) : 0.0;
+        const double cv = wv[u] ? cell_value_fast(wv[u], (int64_t)col * n2 + r, qjump, bk) : 0.0;
         uint64_t mask = __ballot(wv[u] != 0);
         double mine = v;
         while (mask) {

@@ -213,6 +213,7 @@ struct Ctx {
   Scalars *dsc = nullptr;   // device
   Scalars *hsc = nullptr;   // pinned host mirror
   double *dsum = nullptr;   // device scratch for sequential sums (8 slots)
+  unsigned long long *amax = nullptr;  // adapt's per-block maxima (2 x 1024)
   double *exp_host = nullptr;    // pinned: host-evaluated exp table of the step
   hipEvent_t ev_sc = nullptr;    // adapt's scalars have reached the host mirror
   // second stream of the plain step (default priority) and its fork/join events

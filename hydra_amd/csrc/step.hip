@@ -72,20 +72,35 @@ PW_KERNEL(k_qq_kd, (double *__restrict__ ss, const double *__restrict__ qq,
                     const double *__restrict__ pp, double kdsq, int64_t n),
           { ss[k] = qq[k] + kdsq * pp[k]; })
 
-// zz = zz**2 and the maxima of zz**2 and uu**2+vv**2 (evolution.f90:552-556)
+// zz = zz**2 and the maxima of zz**2 and uu**2+vv**2 (evolution.f90:552-556),
+// one pair of maxima per block in mx[2 b], mx[2 b + 1] (k_adapt_scalars
+// takes the maximum over the blocks: no same-address atomics)
 __global__ void __launch_bounds__(256) k_adapt_reduce(double *__restrict__ zz,
                                                       const double *__restrict__ uu,
                                                       const double *__restrict__ vv, int64_t n,
                                                       unsigned long long *__restrict__ mx) {
   double z2m = 0.0, u2m = 0.0;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    const double z = zz[k];
-    const double z2 = z * z;
-    zz[k] = z2;
-    const double u = uu[k], v = vv[k];
-    z2m = fmax(z2m, z2);
-    u2m = fmax(u2m, u * u + v * v);
+  // four elements per thread and pass, all loads in flight before any use
+  constexpr int U = 4;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 < n; k0 += U * stride) {
+    double z[U], u[U], v[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int64_t k = k0 + j * stride;
+      const bool in = k < n;
+      z[j] = in ? zz[k] : 0.0;
+      u[j] = in ? uu[k] : 0.0;
+      v[j] = in ? vv[k] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int64_t k = k0 + j * stride;
+      const double z2 = z[j] * z[j];
+      if (k < n) zz[k] = z2;
+      z2m = fmax(z2m, z2);
+      u2m = fmax(u2m, u[j] * u[j] + v[j] * v[j]);
+    }
   }
   // non-negative doubles order like their bit patterns: one atomic per block
   for (int off = 32; off > 0; off >>= 1) {
@@ -103,8 +118,8 @@ __global__ void __launch_bounds__(256) k_adapt_reduce(double *__restrict__ zz,
       z2m = fmax(z2m, sz[w]);
       u2m = fmax(u2m, su[w]);
     }
-    atomicMax(&mx[0], (unsigned long long)__double_as_longlong(z2m));
-    atomicMax(&mx[1], (unsigned long long)__double_as_longlong(u2m));
+    mx[2 * blockIdx.x] = (unsigned long long)__double_as_longlong(z2m);
+    mx[2 * blockIdx.x + 1] = (unsigned long long)__double_as_longlong(u2m);
   }
 }
 
@@ -112,16 +127,27 @@ struct AdaptArgs {
   double dsumi, small, glx, srwfm, dt_fixed, tgsave, tcsave;
 };
 
-// evolution.f90:553-578: scalars of adapt.
+// evolution.f90:553-578: scalars of adapt.  mx holds nparts pairs of
+// maxima (of zz**2 and of uu**2+vv**2, as bit patterns of non-negative
+// doubles); one wavefront.
 __global__ void k_adapt_scalars(Scalars *sc, const double *sum, const unsigned long long *mx,
-                                AdaptArgs a, const int32_t *sweep_err) {
+                                int nparts, AdaptArgs a, const int32_t *sweep_err) {
+  double zm2 = 0.0, um2 = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 64) {
+    zm2 = fmax(zm2, __longlong_as_double((long long)mx[2 * i]));
+    um2 = fmax(um2, __longlong_as_double((long long)mx[2 * i + 1]));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    zm2 = fmax(zm2, __shfl_down(zm2, off));
+    um2 = fmax(um2, __shfl_down(um2, off));
+  }
   if (threadIdx.x != 0) return;
   sc->sweep_err = sweep_err ? (double)*sweep_err : 0.0;
   const double dtfac = 3.141592653589793238462643383279502884197169399375105820974944592307816 / 10.0;
   const double cflmax = 0.7;
   const double zzrms = sqrt(a.dsumi * sum[0]);
-  const double zzmax = sqrt(__longlong_as_double((long long)mx[0])) + a.small;
-  const double uumax = sqrt(__longlong_as_double((long long)mx[1])) + a.small;
+  const double zzmax = sqrt(zm2) + a.small;
+  const double uumax = sqrt(um2) + a.small;
   const double t1 = a.glx * cflmax / uumax;
   const double zm = zzmax > a.srwfm ? zzmax : a.srwfm;
   const double t2 = dtfac / zm;
@@ -473,13 +499,11 @@ int adapt(Ctx &c) {
     src = ss;
   }
   HYD_CHECK(fft_spctop(c, src, zz, PRE_NONE, 1.0, c.wc));
-  unsigned long long *mx = (unsigned long long *)(c.dsum + 4);
-  HYD_HIP(hipMemsetAsync(mx, 0, 2 * sizeof(unsigned long long), c.s));
+  const int64_t nb = std::min<int64_t>((N + 255) / 256, 1024);
   {
     KTimer t(c, "adapt_reduce", 32 * N);
-    const int64_t nb = std::min<int64_t>((N + 255) / 256, 1024);
     hipLaunchKernelGGL(k_adapt_reduce, dim3((unsigned)nb), dim3(256), 0, c.s, zz, c.uu, c.vv, N,
-                       mx);
+                       c.amax);
   }
   HYD_CHECK(seqsum(c, zz, N, c.dsum + 1));
   AdaptArgs a;
@@ -490,8 +514,8 @@ int adapt(Ctx &c) {
   a.dt_fixed = c.p.dt_fixed;
   a.tgsave = c.p.tgsave;
   a.tcsave = c.p.tcsave;
-  hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, mx, a,
-                     c.flags + 8);
+  hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, c.amax,
+                     (int)nb, a, c.flags + 8);
   HYD_HIP(hipGetLastError());
   // the integrating factors need dt and zzrms on the host: start the copy
   // now, evaluate the exponentials in adapt_finish while the GPU computes
@@ -726,6 +750,7 @@ PW_KERNEL(k_advect_prod32, (float *__restrict__ qqx, const float *__restrict__ q
           })
 
 // sum of zz^2 per block (FP64 accumulation) and the maxima of zz^2, uu^2+vv^2
+// (one pair per block in mx[2 b], mx[2 b + 1])
 __global__ void __launch_bounds__(256) k_adapt_reduce32(const float *__restrict__ zz,
                                                         const float *__restrict__ uu,
                                                         const float *__restrict__ vv, int64_t n,
@@ -760,13 +785,9 @@ __global__ void __launch_bounds__(256) k_adapt_reduce32(const float *__restrict_
       u2m = fmax(u2m, su[w]);
     }
     part[blockIdx.x] = s;
-    atomicMax(&mx[0], (unsigned long long)__double_as_longlong(z2m));
-    atomicMax(&mx[1], (unsigned long long)__double_as_longlong(u2m));
+    mx[2 * blockIdx.x] = (unsigned long long)__double_as_longlong(z2m);
+    mx[2 * blockIdx.x + 1] = (unsigned long long)__double_as_longlong(u2m);
   }
-}
-
-__global__ void k_clear_max(unsigned long long *mx) {
-  if (threadIdx.x < 2) mx[threadIdx.x] = 0;
 }
 
 // total of 1024 partial sums (one workgroup, fixed-order tree)
@@ -798,10 +819,8 @@ static int adapt_mixed(Ctx &c) {
     src = c.wa;
   }
   HYD_CHECK(fft_spctop32(c, src, c.fc32, PRE_NONE, 1.0));
-  unsigned long long *mx = (unsigned long long *)(c.dsum + 4);
-  hipLaunchKernelGGL(k_clear_max, dim3(1), dim3(64), 0, c.s, mx);
   hipLaunchKernelGGL(k_adapt_reduce32, dim3(1024), dim3(256), 0, c.s, c.fc32, c.uu32, c.vv32, N,
-                     c.part, mx);
+                     c.part, c.amax);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, c.s, c.part, c.dsum + 1);
   AdaptArgs a;
   a.dsumi = c.c.dsumi;
@@ -811,8 +830,8 @@ static int adapt_mixed(Ctx &c) {
   a.dt_fixed = c.p.dt_fixed;
   a.tgsave = c.p.tgsave;
   a.tcsave = c.p.tcsave;
-  hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, mx, a,
-                     nullptr);
+  hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, c.amax, 1024,
+                     a, nullptr);
   hipLaunchKernelGGL(k_exp_table, dim3((c.nqu + 255) / 256), dim3(256), 0, c.s, c.qdiss_u_dev,
                      c.nqu, c.dsc, c.qdiss_exp);
   hipLaunchKernelGGL(k_expand_epq, grid1(N), dim3(256), 0, c.s, c.epq, c.emq, c.qdiss_map,
