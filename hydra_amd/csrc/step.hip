@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) k_adapt_reduce(double *__restrict__ zz,
       u2m = fmax(u2m, u[j] * u[j] + v[j] * v[j]);
     }
   }
-  // non-negative doubles order like their bit patterns: one atomic per block
+  // block maxima (non-negative doubles order like their bit patterns)
   for (int off = 32; off > 0; off >>= 1) {
     z2m = fmax(z2m, __shfl_down(z2m, off));
     u2m = fmax(u2m, __shfl_down(u2m, off));
