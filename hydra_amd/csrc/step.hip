@@ -473,6 +473,8 @@ void monitor_record(Ctx &c, const Scalars &s) {
 // step; a contour save falls in it when t + dt >= tcont.  Runs on the main
 // stream before the second stream forks (the energy's sums use the shared
 // sequential-sum work area).
+int adapt_copy(Ctx &c);
+
 static int adapt_saves(Ctx &c) {
   HYD_HIP(hipEventSynchronize(c.ev_sc));
   const double t = c.hsc->t, dt = c.hsc->dt;
@@ -517,12 +519,23 @@ int adapt(Ctx &c) {
   hipLaunchKernelGGL(k_adapt_scalars, dim3(1), dim3(64), 0, c.s, c.dsc, c.dsum + 1, c.amax,
                      (int)nb, a, c.flags + 8);
   HYD_HIP(hipGetLastError());
-  // the integrating factors need dt and zzrms on the host: start the copy
-  // now, evaluate the exponentials in adapt_finish while the GPU computes
-  // the sources of stage 1 (which do not use them)
+  // the integrating factors need dt and zzrms on the host: the copy is
+  // started now (with saves, whose decision needs them at once) or at the
+  // head of the second stream's work (adapt_copy), and the exponentials are
+  // evaluated in adapt_finish while the GPU computes the sources of stage 1
+  // (which do not use them)
+  if (c.saves_on) {
+    HYD_CHECK(adapt_copy(c));
+    HYD_CHECK(adapt_saves(c));
+  }
+  return HYD_OK;
+}
+
+// adapt's scalars to the host mirror, on the current stream; ev_sc marks
+// their arrival.
+int adapt_copy(Ctx &c) {
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
   HYD_HIP(hipEventRecord(c.ev_sc, c.s));
-  if (c.saves_on) HYD_CHECK(adapt_saves(c));
   return HYD_OK;
 }
 
@@ -662,6 +675,7 @@ int advance(Ctx &c) {
   HYD_CHECK(adapt(c));
   HYD_CHECK(fork_s2(c));
   HYD_CHECK(on_s2(c, [&]() {
+    if (!c.saves_on) HYD_CHECK(adapt_copy(c));  // off the main stream's path
     HYD_CHECK(sweep_clear(c));
     return source(c, 0);
   }));
@@ -670,6 +684,8 @@ int advance(Ctx &c) {
   // before the host evaluates the integrating factors, so that the GPU has
   // it to run meanwhile; the factors are uploaded and expanded on the
   // second stream, ahead of the stage-1 field update that needs them
+  // (the tick changes t: the scalars' copy on the second stream has read them)
+  if (!c.saves_on) HYD_HIP(hipStreamWaitEvent(c.s, c.ev_sc, 0));
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
   HYD_CHECK(contour_grid(c));
   HYD_CHECK(on_s2(c, [&]() { return adapt_finish(c); }));
