@@ -275,27 +275,23 @@ __global__ void c2g_buckets(C2GArgs a) {
     bk_node<PASS>(a, i);
 }
 
-// One thread per bucket: order the entries by node and add them in that
-// order (the reference's node loop).
-__global__ void c2g_bucket_sum(Buckets b, const int32_t *flags, double qjump) {
-  const int nb_used = min(flags[4], b.bcap);
-  for (int nb = blockIdx.x * blockDim.x + threadIdx.x; nb < nb_used;
-       nb += gridDim.x * blockDim.x) {
-    const int base = b.list[4 * nb + 1], sz = b.list[4 * nb + 2];
-    int32_t *e = b.slots + base;
-    for (int t = 1; t < sz; t++) {
-      const int32_t v = e[t];
-      int u = t - 1;
-      while (u >= 0 && (e[u] >> 1) > (v >> 1)) {
-        e[u + 1] = e[u];
-        u--;
-      }
-      e[u + 1] = v;
+// Bucket nb: order the entries by node and add them in that order (the
+// reference's node loop).
+__device__ void bucket_sum(const Buckets &b, int nb, double qjump) {
+  const int base = b.list[4 * nb + 1], sz = b.list[4 * nb + 2];
+  int32_t *e = b.slots + base;
+  for (int t = 1; t < sz; t++) {
+    const int32_t v = e[t];
+    int u = t - 1;
+    while (u >= 0 && (e[u] >> 1) > (v >> 1)) {
+      e[u + 1] = e[u];
+      u--;
     }
-    double s = 0.0;
-    for (int t = 0; t < sz; t++) s = s + ((e[t] & 1) ? qjump : -qjump);
-    b.vals[nb] = s;
+    e[u + 1] = v;
   }
+  double s = 0.0;
+  for (int t = 0; t < sz; t++) s = s + ((e[t] & 1) ? qjump : -qjump);
+  b.vals[nb] = s;
 }
 
 // Empty the table again (after the sweep has read the bucket sums; the
@@ -366,9 +362,10 @@ __device__ __forceinline__ int wave_scan_i32(int v) {
 // each group of eight dependent adds, overwriting each value with the chain
 // after it; every counter then reads the value after the last non-empty one
 // at or before it.  The additions are the reference's, in its order.
-// nxf <= kQbotMax.  Being the last reader of the counters and of
-// flags[0..5], it leaves them zero for the next sweep (the bucket count
-// moves to flags[6] for c2g_bucket_clear).
+// nxf <= kQbotMax.  The bucket sums of the order-dependent cells (usually
+// none) are made first, one thread per bucket.  Being the last reader of
+// the counters and of flags[0..5], it leaves them zero for the next sweep
+// (the bucket count moves to flags[6] for c2g_bucket_clear).
 static constexpr int kQbotMax = 32768;
 static constexpr int kQbotPart = 4096;  // counters per pass, four per thread
 static constexpr int kQbotPer = kQbotPart / 1024;
@@ -379,6 +376,9 @@ __global__ void __launch_bounds__(1024) c2g_qbot(int32_t *qjx, double *qbot, int
   __shared__ int wsum[16];
   __shared__ double vin;  // the chain's value entering the pass
   const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
+  const int nb_used = min(flags[4], bk.bcap);
+  for (int nb = tid; nb < nb_used; nb += blockDim.x) bucket_sum(bk, nb, qjump);
+  __syncthreads();
   double v = 0.0;  // qbot(1); thread 0 carries the chain across passes
   if (tid == 0) qbot[0] = 0.0;
   for (int part = 0; part < nxf - 1; part += kQbotPart) {
@@ -674,7 +674,6 @@ int fine_sweep(Ctx &c, const double *xq, const double *yq, const int32_t *nextq,
     hipLaunchKernelGGL(c2g_buckets<0>, dim3(nbk), dim3(256), 0, c.s, a);
     hipLaunchKernelGGL(c2g_buckets<1>, dim3(nbk), dim3(256), 0, c.s, a);
   }
-  hipLaunchKernelGGL(c2g_bucket_sum, dim3(256), dim3(256), 0, c.s, a.bk, c.flags, a.qjump);
   hipLaunchKernelGGL(c2g_qbot, dim3(1), dim3(1024), 0, c.s, qjx, qbot, nxf, nfine, a.qjump, a.bk,
                      c.flags);
   if (avg)
