@@ -644,34 +644,40 @@ int fft_autotune(Ctx &c) {
       if (best[V] < best[bestV]) bestV = V;
     c.fft_v[cls] = bestV;
   }
-  // the FP32 passes of the mixed step (half the LDS per vector)
+  // the FP32 passes of the mixed step (half the LDS per vector), timed the
+  // same way
   for (int cls = 0; cls < 4 && rc == HYD_OK && c.p.precision == 1; cls++) {
-    float best = 1e30f;
-    int bestV = 0;
-    for (int V = 1; V <= kMaxV && V <= n; V *= 2) {
-      if (fft_lds_bytes(n, V, sizeof(float)) > 160 * 1024) break;
-      c.fft_v32[cls] = V;
-      const float *trig = fw[cls] ? c.xtrig32 : c.ytrig32;
-      auto run = [&]() {
-        return fft_launch1<float, float, float>(c, fw[cls], c.fa32, c.fb32, n, n, inr[cls],
-                                                outr[cls], 0, nullptr, 1.0, trig);
-      };
-      rc = run();
-      if (rc != HYD_OK) break;
-      if (hipEventRecord(e0, c.s) != hipSuccess) { rc = HYD_EHIP; break; }
-      for (int r = 0; r < 5 && rc == HYD_OK; r++) rc = run();
-      if (rc != HYD_OK) break;
-      float ms = 0.0f;
-      if (hipEventRecord(e1, c.s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        rc = HYD_EHIP;
-        break;
-      }
-      if (ms < best) {
-        best = ms;
-        bestV = V;
+    float best[kMaxV + 1];
+    for (int V = 0; V <= kMaxV; V++) best[V] = 1e30f;
+    const float *trig = fw[cls] ? c.xtrig32 : c.ytrig32;
+    const float *in[2] = {c.fa32, c.fc32};
+    float *out[2] = {c.fb32, c.fd32};
+    const double post[2] = {1.0, 1.0};
+    auto run = [&]() {
+      return fft_launch_t<float, float, float>(c, fw[cls], 2, in, out, n, n, inr[cls], outr[cls],
+                                               nullptr, nullptr, post, trig);
+    };
+    for (int round = 0; round < 3 && rc == HYD_OK; round++) {
+      for (int V = 1; V <= kMaxV && V <= n; V *= 2) {
+        if (fft_lds_bytes(n, V, sizeof(float)) > 160 * 1024) break;
+        c.fft_v32[cls] = V;
+        rc = run();
+        if (rc != HYD_OK) break;
+        if (hipEventRecord(e0, c.s) != hipSuccess) { rc = HYD_EHIP; break; }
+        for (int r = 0; r < 8 && rc == HYD_OK; r++) rc = run();
+        if (rc != HYD_OK) break;
+        float ms = 0.0f;
+        if (hipEventRecord(e1, c.s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = HYD_EHIP;
+          break;
+        }
+        best[V] = std::min(best[V], ms);
       }
     }
+    int bestV = 1;
+    for (int V = 2; V <= kMaxV; V *= 2)
+      if (best[V] < best[bestV]) bestV = V;
     c.fft_v32[cls] = bestV;
   }
   hipEventDestroy(e0);
