@@ -33,7 +33,7 @@ static constexpr int kMaxF = 4;
 static constexpr int kMaxV = 8;  // vectors per block: 1, 2, 4 or 8
 // threads per block of every FFT launch (a constant, so that the item loops
 // of a stage have compile-time trip counts)
-static constexpr int kFftThreads = 256;
+static constexpr int kFftThreads = 512;
 // Twiddles read from the (L1/L2-resident) global table rather than staged in
 // LDS: the smaller LDS footprint lets more blocks share a CU.
 static constexpr bool kTrigInLds = false;
