@@ -541,7 +541,9 @@ static int fft_launch_t(Ctx &c, bool fwd, int F, const TI *const *in, TO *const 
   // divides m, else min(4096/n, kMaxV)
   const int cls = fwd ? (out_rows ? 0 : 1) : (in_rows ? 3 : 2);
   // tuned for the context's length, per arithmetic type
-  int V = n == c.c.nx ? (sizeof(T) == 4 ? c.fft_v32[cls] : c.fft_v[cls]) : 0;
+  int V = 0;
+  if (n == c.c.nx)
+    V = sizeof(T) == 4 ? c.fft_v32[cls] : (F == 1 && c.fft_v1[cls] ? c.fft_v1[cls] : c.fft_v[cls]);
   if (V < 1 || V > kMaxV || V > m || m % V) {
     // the largest power of two <= 4096/n and kMaxV that divides m
     V = std::max(1, std::min(4096 / n, kMaxV));
@@ -610,7 +612,11 @@ int fft_autotune(Ctx &c) {
   const bool fw[4] = {true, true, false, false};
   const int inr[4] = {0, 0, 0, 1}, outr[4] = {1, 0, 0, 0};
   int rc = HYD_OK;
-  for (int cls = 0; cls < 4 && rc == HYD_OK; cls++) {
+  // two-field launches first (fft_v), then one-field ones (fft_v1: half
+  // the grid, so a smaller batch may fill the device better)
+  for (int pass = 0; pass < 2 * 4 && rc == HYD_OK; pass++) {
+    const int cls = pass % 4, F = pass < 4 ? 2 : 1;
+    int *tab = F == 2 ? c.fft_v : c.fft_v1;
     float best[kMaxV + 1];
     for (int V = 0; V <= kMaxV; V++) best[V] = 1e30f;
     const double *trig = fw[cls] ? c.xtrig : c.ytrig;
@@ -618,13 +624,13 @@ int fft_autotune(Ctx &c) {
     double *out[2] = {c.wb, c.wd};
     const double post[2] = {1.0, 1.0};
     auto run = [&]() {
-      return fft_launch_t<double, double, double>(c, fw[cls], 2, in, out, n, n, inr[cls],
+      return fft_launch_t<double, double, double>(c, fw[cls], F, in, out, n, n, inr[cls],
                                                   outr[cls], nullptr, nullptr, post, trig);
     };
     for (int round = 0; round < 3 && rc == HYD_OK; round++) {
       for (int V = 1; V <= kMaxV && V <= n; V *= 2) {
         if (fft_lds_bytes(n, V, sizeof(double)) > 160 * 1024) break;
-        c.fft_v[cls] = V;
+        tab[cls] = V;
         rc = run();
         if (rc != HYD_OK) break;
         if (hipEventRecord(e0, c.s) != hipSuccess) { rc = HYD_EHIP; break; }
@@ -642,7 +648,7 @@ int fft_autotune(Ctx &c) {
     int bestV = 1;
     for (int V = 2; V <= kMaxV; V *= 2)
       if (best[V] < best[bestV]) bestV = V;
-    c.fft_v[cls] = bestV;
+    tab[cls] = bestV;
   }
   // the FP32 passes of the mixed step (half the LDS per vector), timed the
   // same way
