@@ -80,8 +80,9 @@ def pmc_summary(fetch_dir, write_dir, ng, out_json):
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the same
     counter on the calibration copy, as bytes per algorithmic byte
     (aggregated over the batch variants VT).  A dispatch's algorithmic bytes
-    are (sizeof in + sizeof out) x n x vectors; the vectors are its blocks of
-    256 threads times the VT in the kernel's name."""
+    are (sizeof in + sizeof out) x n x vectors; the vectors are its blocks
+    (grid size / workgroup size, both in the record) times the VT in the
+    kernel's name."""
     import csv
     import glob
     import re
@@ -109,8 +110,8 @@ def pmc_summary(fetch_dir, write_dir, ng, out_json):
                 continue
             n = int(m.group(1))
             ti, to = size[m.group(3)], size[m.group(5)]
-            # blocks of 256 threads, VT vectors each (all fields of the launch)
-            vectors = int(row["Grid_Size"]) // 256 * int(m.group(6))
+            # its blocks, VT vectors each (all fields of the launch)
+            vectors = int(row["Grid_Size"]) // int(row["Workgroup_Size"]) * int(m.group(6))
             alg = (ti + to) * n * vectors
             key = "fft_kernel<%s, %s, %s, %s, %s>" % m.group(1, 2, 3, 4, 5)
             rec = per.setdefault(key, [0.0, 0.0, 0])
