@@ -119,6 +119,7 @@ static void free_ctx(Ctx *c) {
     hipEventDestroy(r.b);
   }
   if (c->step_graph) hipGraphExecDestroy(c->step_graph);
+  if (c->step_graph_tail) hipGraphExecDestroy(c->step_graph_tail);
   if (c->s2) hipStreamDestroy(c->s2);
   if (c->s) hipStreamDestroy(c->s);
   delete c;
@@ -1227,34 +1228,60 @@ int hyd_renode(hyd_ctx *h, const double *xd, const double *yd, int32_t npd, doub
   return HYD_OK;
 }
 
+static int capture_graph(Ctx &c, int (*part)(Ctx &), hipGraphExec_t *out) {
+  if (*out) {
+    HYD_HIP(hipGraphExecDestroy(*out));
+    *out = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HYD_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+  const int rc = part(c);
+  const hipError_t e = hipStreamEndCapture(c.s, &g);
+  if (rc != HYD_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  HYD_HIP(e);
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  HYD_HIP(ei);
+  return HYD_OK;
+}
+
 // The mixed-precision step has no host round trip: after one eager call it
-// is captured as a graph and replayed; the capture is redone whenever the
-// node count changed (surgery, recontouring) and skipped while timing.
-static int advance_mixed_graph(Ctx &c) {
+// is captured as two graphs (head and tail) and replayed; the capture is
+// redone whenever the node count changed (surgery, recontouring) and
+// skipped while timing.  With copy_scalars the step's scalars are copied to
+// the host mirror between head and tail (ev_sc; sc_pending set).
+static int advance_mixed_graph(Ctx &c, bool copy_scalars = false) {
   c.sweep_clean = false;  // its sweeps leave their counts behind
   c.n_advances += 1;
   c.n_node_updates += c.nptq;
-  if (c.timing || c.mixed_calls++ == 0) return advance_mixed(c);
-  if (!c.step_graph || c.graph_nptq != c.nptq) {
-    if (c.step_graph) {
-      HYD_HIP(hipGraphExecDestroy(c.step_graph));
-      c.step_graph = nullptr;
-    }
-    hipGraph_t g = nullptr;
-    HYD_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
-    const int rc = advance_mixed(c);
-    const hipError_t e = hipStreamEndCapture(c.s, &g);
-    if (rc != HYD_OK) {
-      if (g) hipGraphDestroy(g);
-      return rc;
-    }
-    HYD_HIP(e);
-    const hipError_t ei = hipGraphInstantiate(&c.step_graph, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    HYD_HIP(ei);
+  const bool eager = c.timing || c.mixed_calls++ == 0;
+  if (!eager && (!c.step_graph || !c.step_graph_tail || c.graph_nptq != c.nptq)) {
+    HYD_CHECK(capture_graph(c, advance_mixed_head, &c.step_graph));
+    HYD_CHECK(capture_graph(c, advance_mixed_tail, &c.step_graph_tail));
     c.graph_nptq = c.nptq;
   }
-  HYD_HIP(hipGraphLaunch(c.step_graph, c.s));
+  if (eager) HYD_CHECK(advance_mixed_head(c));
+  else HYD_HIP(hipGraphLaunch(c.step_graph, c.s));
+  if (copy_scalars) {
+    HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+    HYD_HIP(hipEventRecord(c.ev_sc, c.s));
+    c.sc_pending = true;
+  }
+  if (eager) return advance_mixed_tail(c);
+  HYD_HIP(hipGraphLaunch(c.step_graph_tail, c.s));
+  return HYD_OK;
+}
+
+// A lazily run mixed step's scalars (adapt's, t at the step's start): on the
+// host, and its monitor row recorded.
+static int mixed_scalars_in(Ctx &c) {
+  if (!c.sc_pending) return HYD_OK;
+  c.sc_pending = false;
+  HYD_HIP(hipEventSynchronize(c.ev_sc));
+  monitor_record(c, *c.hsc);
   return HYD_OK;
 }
 
@@ -1301,6 +1328,7 @@ int hyd_surgery(hyd_ctx *h) {
 static int advect_step(Ctx &c, bool lazy = false) {
   const double twistmax = 2.5;
   const int nregmax = 20;
+  HYD_CHECK(mixed_scalars_in(c));  // the previous step's, if it ran lazily
   if (c.hsc->twist > twistmax) {
     c.ireg = c.ireg + 1;
     if (c.ireg == nregmax) {
@@ -1318,7 +1346,10 @@ static int advect_step(Ctx &c, bool lazy = false) {
   }
   // the save slots read the step's start time from the host mirror, so a
   // run with saves keeps the read-back
-  if (lazy && c.p.precision == 0 && !c.saves_on && c.have_field) return advance(c);
+  if (lazy && !c.saves_on && c.have_field) {
+    if (c.p.precision == 0) return advance(c);
+    return advance_mixed_graph(c, true);  // scalars read back by the next step
+  }
   return hyd_advance((hyd_ctx *)&c);
 }
 
@@ -1362,6 +1393,7 @@ int hyd_run(hyd_ctx *h, int32_t nsteps) {
       done++;
     }
   }
+  HYD_CHECK(mixed_scalars_in(c));
   HYD_HIP(hipMemcpyAsync(c.hsc, c.dsc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
   return c2g_status(c);  // waits for the stream
 }

@@ -242,8 +242,10 @@ struct Ctx {
   int64_t seq_cap = 0;      // chunks it holds
   // work buffers of recontour / renode, allocated on first use
   std::map<std::string, DBuf> cg_bufs;
-  // captured mixed-precision step (replayed while the node count is unchanged)
-  hipGraphExec_t step_graph = nullptr;
+  // captured mixed-precision step, head and tail (replayed while the node
+  // count is unchanged)
+  hipGraphExec_t step_graph = nullptr, step_graph_tail = nullptr;
+  bool sc_pending = false;  // a mixed step's scalars are on their way to the host (ev_sc)
   int graph_nptq = -1;
   int mixed_calls = 0;
   // timing
@@ -276,6 +278,8 @@ int fft_spctop32(Ctx &c, const double *svar, float *rvar, int pre, double post_s
 int con2grid_mixed(Ctx &c, float *qc, bool defer_mean = false);
 int fft_ptospc32_mean(Ctx &c, const float *rvar, const double *mean, double *svar);
 int advance_mixed(Ctx &c);
+int advance_mixed_head(Ctx &c);
+int advance_mixed_tail(Ctx &c);
 int fft_autotune(Ctx &c);
 int fft_reverse_1d(Ctx &c, double *x, int m);   // revfft(m, ng, x)
 

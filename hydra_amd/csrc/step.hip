@@ -903,8 +903,24 @@ static int inversion_fields_mixed(Ctx &c) {
 
 // The same stage overlap as the plain step (advance); the fork and join are
 // events, so a stream capture of the main stream records both streams.
-int advance_mixed(Ctx &c) {
+// The mixed step in two parts: the head (stage 1's inversion and adapt,
+// after which the step's scalars are final but for t) and the tail (the
+// four stages), so that a loop can read the scalars back between the two
+// and decide the next step while the tail runs.
+int advance_mixed_head(Ctx &c) {
   const int64_t N = c.N;
+  HYD_CHECK(contour_grid_mixed(c));
+  HYD_CHECK(inversion_fields_mixed(c));
+  hipLaunchKernelGGL(k_reset, grid1(N), dim3(256), 0, c.s, c.qs, c.qd, c.qc, c.flo, c.fhi, N);
+  return adapt_mixed(c);
+}
+
+int advance_mixed(Ctx &c) {
+  HYD_CHECK(advance_mixed_head(c));
+  return advance_mixed_tail(c);
+}
+
+int advance_mixed_tail(Ctx &c) {
   auto fields = [&](int stage) {
     return on_s2(c, [&]() {
       HYD_CHECK(rk_fields(c, stage));
@@ -912,10 +928,6 @@ int advance_mixed(Ctx &c) {
       return HYD_OK;
     });
   };
-  HYD_CHECK(contour_grid_mixed(c));
-  HYD_CHECK(inversion_fields_mixed(c));
-  hipLaunchKernelGGL(k_reset, grid1(N), dim3(256), 0, c.s, c.qs, c.qd, c.qc, c.flo, c.fhi, N);
-  HYD_CHECK(adapt_mixed(c));
   for (int stage = 1; stage <= 4; stage++) {
     if (stage > 1) {
       if (stage != 3) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c.s, c.dsc);
