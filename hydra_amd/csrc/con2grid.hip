@@ -362,11 +362,10 @@ __device__ __forceinline__ int wave_scan_i32(int v) {
 // each group of eight dependent adds, overwriting each value with the chain
 // after it; every counter then reads the value after the last non-empty one
 // at or before it.  The additions are the reference's, in its order.
-// nxf <= kQbotMax.  The bucket sums of the order-dependent cells (usually
+// Any nxf (passes of kQbotPart).  The bucket sums of the order-dependent cells (usually
 // none) are made first, one thread per bucket.  Being the last reader of
 // the counters and of flags[0..5], it leaves them zero for the next sweep
 // (the bucket count moves to flags[6] for c2g_bucket_clear).
-static constexpr int kQbotMax = 32768;
 static constexpr int kQbotPart = 4096;  // counters per pass, four per thread
 static constexpr int kQbotPer = kQbotPart / 1024;
 __global__ void __launch_bounds__(1024) c2g_qbot(int32_t *qjx, double *qbot, int nxf,
