@@ -542,8 +542,12 @@ static int fft_launch_t(Ctx &c, bool fwd, int F, const TI *const *in, TO *const 
   const int cls = fwd ? (out_rows ? 0 : 1) : (in_rows ? 3 : 2);
   // tuned for the context's length, per arithmetic type
   int V = 0;
-  if (n == c.c.nx)
-    V = sizeof(T) == 4 ? c.fft_v32[cls] : (F == 1 && c.fft_v1[cls] ? c.fft_v1[cls] : c.fft_v[cls]);
+  if (n == c.c.nx) {
+    if (sizeof(T) == 4) V = c.fft_v32[cls];
+    else if (F == 1 && c.fft_v1[cls]) V = c.fft_v1[cls];
+    else if (F >= 3 && c.fft_v4[cls]) V = c.fft_v4[cls];
+    else V = c.fft_v[cls];
+  }
   if (V < 1 || V > kMaxV || V > m || m % V) {
     // the largest power of two <= 4096/n and kMaxV that divides m
     V = std::max(1, std::min(4096 / n, kMaxV));
@@ -613,16 +617,18 @@ int fft_autotune(Ctx &c) {
   const int inr[4] = {0, 0, 0, 1}, outr[4] = {1, 0, 0, 0};
   int rc = HYD_OK;
   // two-field launches first (fft_v), then one-field ones (fft_v1: half
-  // the grid, so a smaller batch may fill the device better)
-  for (int pass = 0; pass < 2 * 4 && rc == HYD_OK; pass++) {
-    const int cls = pass % 4, F = pass < 4 ? 2 : 1;
-    int *tab = F == 2 ? c.fft_v : c.fft_v1;
+  // the grid, so a smaller batch may fill the device better) and four-field
+  // ones (fft_v4; outputs in the fine-grid scratch)
+  for (int pass = 0; pass < 3 * 4 && rc == HYD_OK; pass++) {
+    const int cls = pass % 4, F = pass < 4 ? 2 : (pass < 8 ? 1 : 4);
+    int *tab = F == 2 ? c.fft_v : (F == 1 ? c.fft_v1 : c.fft_v4);
     float best[kMaxV + 1];
     for (int V = 0; V <= kMaxV; V++) best[V] = 1e30f;
     const double *trig = fw[cls] ? c.xtrig : c.ytrig;
-    const double *in[2] = {c.wa, c.wc};
-    double *out[2] = {c.wb, c.wd};
-    const double post[2] = {1.0, 1.0};
+    const double *in[4] = {c.wa, c.wc, c.wb, c.wd};
+    double *out[4] = {F == 4 ? c.fq : c.wb, F == 4 ? c.fq + c.N : c.wd, c.fq + 2 * c.N,
+                      c.fq + 3 * c.N};
+    const double post[4] = {1.0, 1.0, 1.0, 1.0};
     auto run = [&]() {
       return fft_launch_t<double, double, double>(c, fw[cls], F, in, out, n, n, inr[cls],
                                                   outr[cls], nullptr, nullptr, post, trig);

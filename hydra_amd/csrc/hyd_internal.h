@@ -113,6 +113,7 @@ struct Ctx {
   FftPlan plan;
   int fft_v[4] = {0, 0, 0, 0};    // autotuned FFT batch per layout class (FP64 passes)
   int fft_v1[4] = {0, 0, 0, 0};   // the same for one-field FP64 launches (a smaller grid)
+  int fft_v4[4] = {0, 0, 0, 0};   // and for three- and four-field FP64 launches
   int fft_v32[4] = {0, 0, 0, 0};  // the same for the FP32 passes of the mixed step
   hipStream_t s = nullptr;
   int dev = 0;
